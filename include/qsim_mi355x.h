@@ -1,0 +1,165 @@
+/*
+ * qsim_mi355x.h - C ABI of the MI355X (gfx950) state-vector library.
+ *
+ * The library replaces the array arithmetic that qibo's NumpyBackend performs
+ * for the state-vector hot path (qibo 0.3.5, src/qibo/backends/abstract.py).
+ * Each entry point names the reference method it stands in for.
+ *
+ * Conventions shared by every entry point:
+ *   - All state / probability / index buffers are DEVICE pointers owned by the
+ *     caller (the Python host keeps them in torch tensors). Small inputs that
+ *     are marked "host" (qubit lists, gate matrices, term tables) are read
+ *     during the call and copied into kernel arguments.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *     Every call only enqueues work on `stream` and returns; no call
+ *     synchronises the device unless its comment says so.
+ *   - Qubit numbering is qibo's: qubit q of an n-qubit state is bit
+ *     (n-1-q) of the amplitude index (qubit 0 is the most significant bit).
+ *   - Matrices are row-major complex (interleaved re, im) of the state dtype.
+ *     A matrix acting on targets (t0, t1, ...) indexes its rows/columns with t0
+ *     as the most significant bit, as gate.qubits orders them in qibo.
+ *   - Return value: 0 on success, a negative QSIM_E* code otherwise; the
+ *     message of the last failure on the calling thread is qsim_last_error().
+ *   - Every entry point is re-entrant: no global mutable state is shared
+ *     between calls (qibo's parallel.py calls execute_circuit from threads).
+ */
+#ifndef QSIM_MI355X_H
+#define QSIM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSIM_ABI_VERSION 1
+
+/* state dtypes */
+#define QSIM_C128 0 /* complex128: double re, double im */
+#define QSIM_C64 1  /* complex64:  float re, float im   */
+
+/* probability dtypes */
+#define QSIM_F64 0
+#define QSIM_F32 1
+
+/* error codes */
+#define QSIM_OK 0
+#define QSIM_EINVAL -1
+#define QSIM_ENOMEM -2
+#define QSIM_EHIP -3
+
+/* Largest qubit count any entry point accepts. */
+#define QSIM_MAX_QUBITS 36
+
+/* ABI version of the loaded library (QSIM_ABI_VERSION it was built with). */
+int qsim_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qsim_last_error(void);
+
+/* |0...0>: zero the 2^n amplitudes and set amplitude 0 to 1.
+ * Replaces Backend.zero_state (abstract.py:2243-2275). */
+int qsim_zero_state(void* state, int nqubits, int dtype, void* stream);
+
+/* Dense 2^k x 2^k matrix on `targets` (k = ntargets, 1..3), applied only on
+ * the amplitudes whose `controls` bits are all 1, in place.
+ * Replaces Backend.apply_gate's transpose/matmul/transpose for dense gates
+ * (abstract.py:2350-2361) and Backend._apply_gate_controlled_by
+ * (abstract.py:3176-3197). `matrix` is a host pointer. */
+int qsim_apply_dense(void* state, int nqubits, int dtype, const int* targets, int ntargets,
+                     const int* controls, int ncontrols, const void* matrix, void* stream);
+
+/* Diagonal matrix (2^k complex entries, k = ntargets, 0..5) on `targets`,
+ * restricted to amplitudes whose `controls` bits are all 1. k = 0 multiplies
+ * the controlled subspace by diag[0]. Same reference as qsim_apply_dense;
+ * used for gates whose matrix is diagonal (RZ, U1, CZ, CU1, ...), which
+ * touch only the amplitudes they change. `diag` is a host pointer. */
+int qsim_apply_diag(void* state, int nqubits, int dtype, const int* targets, int ntargets,
+                    const int* controls, int ncontrols, const void* diag, void* stream);
+
+/* Exchange of the |..0..1..> and |..1..0..> amplitudes of qubits (q0, q1)
+ * under `controls`: the SWAP matrix (npmatrices.py:264-268) applied by
+ * touching only the two amplitudes it moves. */
+int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const int* controls,
+                    int ncontrols, void* stream);
+
+/* Dense 2^k x 2^k block on k = ntargets (1..5) targets with the matrix in
+ * DEVICE memory: the application of a FusedGate (abstract.py:2680-2717 builds
+ * the matrix; apply_gate applies it). Arithmetic in the state's precision
+ * (FP64 FMA for complex128, FP32 for complex64). */
+int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, int ntargets,
+                     const void* matrix_dev, void* stream);
+
+/* Probabilities of measuring `qubits` (m = nmeasured, in the given order, the
+ * first being the most significant bit of the output index):
+ *   p = abs(state)**2 with numpy's complex abs, summed over unmeasured qubits.
+ * Replaces Backend.calculate_probabilities + _order_probabilities
+ * (abstract.py:2734-2758, 3371-3381). Output dtype is F64 for C128 states and
+ * F32 for C64 states (the state's real dtype, as `rtype` there). When all
+ * qubits are measured the values are bitwise those of numpy's
+ * `np.abs(state)**2`; marginals use a fixed-order FP64 reduction.
+ * `workspace` must hold qsim_probabilities_workspace(nqubits, nmeasured) bytes. */
+size_t qsim_probabilities_workspace(int nqubits, int nmeasured);
+int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qubits, int nmeasured,
+                       void* probs, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Inclusive prefix sum of n probabilities (F64, or F32 widened to F64) that is
+ * bitwise equal to the sequential left-to-right sum numpy's `cumsum` performs
+ * inside np.random.choice (called from Backend.sample_shots via random_choice,
+ * abstract.py:2774-2781, 1265-1309). Writes cdf[0..n-1] (float64).
+ * Probabilities must be finite and >= 0.
+ * `workspace` must hold qsim_cumsum_workspace(n) bytes. */
+size_t qsim_cumsum_workspace(int64_t n);
+int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* Inverse-CDF draw: out[k] = number of i with fl(cdf[i] / cdf[n-1]) <= u[k],
+ * i.e. numpy's `(cdf / cdf[-1]).searchsorted(u, side="right")` on the
+ * normalised cumulative sum, for nshots device uniforms u in [0, 1). */
+int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int64_t nshots,
+                       int64_t* out, void* stream);
+
+/* out[s, b] = (samples[s] >> (nbits-1-b)) & 1, as Backend.samples_to_binary
+ * (abstract.py:2783-2786). out is int64 [nshots, nbits]. */
+int qsim_samples_to_binary(const int64_t* samples, int64_t nshots, int nbits, int64_t* out,
+                           void* stream);
+
+/* counts[v] += number of s with samples[s] == v, for v < nbins (int64 counts),
+ * the accumulation of Backend.update_frequencies (abstract.py:2795-2801). */
+int qsim_histogram(const int64_t* samples, int64_t nshots, int64_t* counts, int64_t nbins,
+                   void* stream);
+
+/* Expectation of a sum of Pauli strings, accumulated in FP64:
+ *   out = sum_t coeffs[t] * Re <psi| i^{ny_t} X^{xmask_t} Z^{zmask_t} |psi>
+ * where masks are over amplitude-index bits (bit n-1-q for qubit q) and ny_t is
+ * the number of Y factors of term t (Y = i X Z). Replaces the per-term einsum
+ * loop of Backend.exp_value_observable_symbolic (abstract.py:2946-3054).
+ * `out` is one device double. Terms sharing an X mask share one state pass.
+ * `workspace` must hold qsim_expval_workspace(nqubits, nterms) bytes. */
+size_t qsim_expval_workspace(int nqubits, int nterms);
+int qsim_expval_pauli(const void* state, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
+                      const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* out = sum_t coeffs[t] * i^{ny_t} X^{xmask_t} Z^{zmask_t} |psi>  (out of
+ * place, out must not alias psi). The Hamiltonian-vector product that seeds the
+ * adjoint sweep; no counterpart in the reference, which differentiates by
+ * re-execution (derivative.py:7-226). */
+int qsim_apply_pauli_sum(const void* state, int nqubits, int dtype, int nterms,
+                         const uint64_t* xmasks, const uint64_t* zmasks, const int* ny,
+                         const double* coeffs, void* out, void* stream);
+
+/* out = Re <bra| M_targets |ket> for a dense 2^k x 2^k host matrix (k = 1..3)
+ * under `controls`, accumulated in FP64 into one device double. The
+ * derivative term 2 Re <lambda| dU/dtheta |psi> of the adjoint sweep.
+ * `workspace` must hold qsim_expval_workspace(nqubits, 1) bytes. */
+int qsim_inner_matrix(const void* bra, const void* ket, int nqubits, int dtype, const int* targets,
+                      int ntargets, const int* controls, int ncontrols, const void* matrix,
+                      double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSIM_MI355X_H */

@@ -1,0 +1,98 @@
+"""ctypes binding of ``libqsim_mi355x.so`` (C ABI in include/qsim_mi355x.h).
+
+The library is loaded from this package directory, where ``_build.py`` puts
+it. There is no fallback: if the library is missing or its ABI version does
+not match, importing the backend raises ``ImportError`` naming the package,
+which qibo's ``construct_backend`` turns into ``MissingBackend``
+(``src/qibo/backends/__init__.py:342-350``).
+"""
+
+import ctypes
+import os
+import threading
+
+ABI_VERSION = 1
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
+
+C128, C64 = 0, 1
+F64, F32 = 0, 1
+OK, EINVAL, ENOMEM, EHIP = 0, -1, -2, -3
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); every exported entry point of the header
+SIGNATURES = {
+    "qsim_version": (_i, []),
+    "qsim_last_error": (ctypes.c_char_p, []),
+    "qsim_zero_state": (_i, [_vp, _i, _i, _vp]),
+    "qsim_apply_dense": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
+    "qsim_apply_diag": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
+    "qsim_apply_swap": (_i, [_vp, _i, _i, _i, _i, _c_int_p, _i, _vp]),
+    "qsim_apply_fused": (_i, [_vp, _i, _i, _c_int_p, _i, _vp, _vp]),
+    "qsim_probabilities_workspace": (_sz, [_i, _i]),
+    "qsim_probabilities": (_i, [_vp, _i, _i, _c_int_p, _i, _vp, _vp, _sz, _vp]),
+    "qsim_cumsum_workspace": (_sz, [_i64]),
+    "qsim_cumsum_exact": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "qsim_search_sorted": (_i, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    "qsim_samples_to_binary": (_i, [_vp, _i64, _i, _vp, _vp]),
+    "qsim_histogram": (_i, [_vp, _i64, _vp, _i64, _vp]),
+    "qsim_expval_workspace": (_sz, [_i, _i]),
+    "qsim_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "qsim_apply_pauli_sum": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qsim_inner_matrix": (_i, [_vp, _vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp, _vp, _sz, _vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class QsimError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+def load(path: str = LIB_PATH):
+    """Load the library once (thread-safe) and check its ABI version."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise ImportError(
+                f"qibo_amd: native library {path} is missing; build it with "
+                "`python -m qibo_amd._build` (hipcc, gfx950)"
+            )
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        ver = lib.qsim_version()
+        if ver != ABI_VERSION:
+            raise ImportError(f"qibo_amd: library ABI {ver} != expected {ABI_VERSION}")
+        _lib = lib
+        return lib
+
+
+def check(rc: int, lib=None):
+    """Raise on a negative return code with the library's thread-local message."""
+    if rc == OK:
+        return
+    lib = lib or _lib
+    msg = lib.qsim_last_error().decode() if lib is not None else ""
+    if rc == ENOMEM:
+        raise MemoryError(msg)
+    if rc == EINVAL:
+        raise ValueError(msg)
+    raise QsimError(f"qsim error {rc}: {msg}")
+
+
+def int_array(values):
+    """ctypes int array (or None for an empty sequence)."""
+    values = [int(v) for v in values]
+    if not values:
+        return None
+    return (ctypes.c_int * len(values))(*values)

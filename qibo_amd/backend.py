@@ -1,0 +1,678 @@
+"""``MI355XBackend``: qibo's ``Backend`` contract on top of the HIP library.
+
+The state vector is a torch tensor in HBM (complex128 or complex64) that the
+backend owns and updates in place; every gate pass, probability reduction,
+cumulative sum and sampling search is one or more calls into
+``libqsim_mi355x.so`` on the tensor's current HIP stream. torch is used for
+allocation, streams and a few O(nshots) host-side conversions only.
+
+Method-by-method counterpart of qibo 0.3.5 (``src/qibo/backends/abstract.py``):
+
+* ``execute_circuit`` / ``_execute_circuit`` -- :2442-2512, :3306-3344
+* ``zero_state`` -- :2243-2275
+* ``apply_gate`` (dense, controlled, controlled_by) -- :2322-2361, :3176-3197
+* ``matrix`` / ``matrix_parametrized`` / ``matrix_fused`` -- :2649-2717
+* ``calculate_probabilities`` -- :2734-2758, :3371-3381
+* ``sample_shots`` / ``sample_frequencies`` / ``update_frequencies`` --
+  :2760-2801 with ``random_choice`` :1265-1309
+* ``samples_to_binary`` / ``samples_to_decimal`` / ``calculate_frequencies``
+  -- :2723-2793
+* ``exp_value_observable_symbolic`` -- :2946-3054
+
+A gate's matrix is inspected once per call to pick the cheapest exact kernel
+form: rows that act as the identity on a qubit being 0 make that qubit a
+control (CNOT, CZ, CU1, TOFFOLI, ...), diagonal matrices only touch the
+amplitudes they change, and the SWAP permutation only moves two amplitudes
+per group. Each form computes the same products as the dense matrix, minus
+the terms multiplied by an exact zero.
+"""
+
+import collections
+import math
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import config
+from .config import SHOT_BATCH_SIZE, log, raise_error
+from .matrices import GateMatrices
+
+_DTYPES = {
+    "complex128": (torch.complex128, np.complex128, _lib.C128, torch.float64),
+    "complex64": (torch.complex64, np.complex64, _lib.C64, torch.float32),
+}
+
+_SWAP4 = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
+
+
+class _Op:
+    """A planned kernel call for one gate."""
+
+    __slots__ = ("kind", "targets", "controls", "data")
+
+    def __init__(self, kind, targets, controls, data):
+        self.kind = kind          # "noop" | "diag" | "swap" | "dense" | "fused"
+        self.targets = tuple(targets)
+        self.controls = tuple(controls)
+        self.data = data          # numpy matrix / diagonal, or None
+
+
+def _peel_controls(m: np.ndarray, qubits: Sequence[int]) -> Tuple[np.ndarray, list, list]:
+    """Move qubits on which ``m`` is ``[[I, 0], [0, U]]`` into the control list.
+
+    ``m`` acts on ``qubits`` with qubits[0] as the most significant bit.
+    Returns (reduced matrix on the remaining targets, targets, controls).
+    """
+    targets = list(qubits)
+    controls = []
+    changed = True
+    while changed and targets:
+        changed = False
+        k = len(targets)
+        dim = 2 ** (k - 1)
+        tensor = m.reshape((2,) * (2 * k))
+        for t in range(k):
+            def block(rb, cb, t=t):
+                sel = [slice(None)] * (2 * k)
+                sel[t], sel[k + t] = rb, cb
+                return tensor[tuple(sel)].reshape(dim, dim)
+
+            if np.any(block(0, 1) != 0) or np.any(block(1, 0) != 0):
+                continue
+            if not np.array_equal(block(0, 0), np.eye(dim)):
+                continue
+            m = block(1, 1)
+            controls.append(targets.pop(t))
+            changed = True
+            break
+    return m, targets, controls
+
+
+def plan_matrix(matrix: np.ndarray, targets: Sequence[int], controls: Sequence[int]) -> _Op:
+    """Cheapest exact kernel form of ``matrix`` on ``targets`` under ``controls``."""
+    matrix = np.asarray(matrix)
+    m, tg, extra = _peel_controls(matrix, targets)
+    ctrl = list(controls) + extra
+    dim = m.shape[0]
+    if dim == 1 or np.count_nonzero(m - np.diag(np.diagonal(m))) == 0:
+        d = np.diagonal(m).copy()
+        if np.all(d == 1):
+            return _Op("noop", tg, ctrl, None)
+        if len(tg) > 5:
+            raise_error(NotImplementedError, "diagonal gates on more than 5 targets")
+        return _Op("diag", tg, ctrl, d)
+    if len(tg) == 2 and np.array_equal(m, _SWAP4):
+        return _Op("swap", tg, ctrl, None)
+    if len(tg) <= 3:
+        return _Op("dense", tg, ctrl, m)
+    # dense block on 4-5 targets: the block kernel takes no controls, so keep
+    # any controls inside the matrix ([[I, 0], [0, U]] on controls + targets)
+    if controls:
+        full = np.eye(2 ** (len(controls) + len(targets)), dtype=matrix.dtype)
+        full[-len(matrix):, -len(matrix):] = matrix
+        matrix, targets = full, tuple(controls) + tuple(targets)
+    if len(targets) > 5:
+        raise_error(NotImplementedError, "dense blocks on more than 5 qubits are not supported")
+    return _Op("fused", targets, (), matrix)
+
+
+class GatePlanner:
+    """Host half of gate application: matrices and the kernel plan of a gate.
+
+    Needs no device; ``MI355XBackend`` inherits it. ``matrices`` holds the
+    numpy gate formulas of the backend dtype.
+    """
+
+    def __init__(self, dtype: str = "complex128"):
+        self.dtype = dtype
+        self.matrices = GateMatrices(dtype)
+
+    # -- matrices (host) -----------------------------------------------------
+    def matrix(self, gate):
+        name = gate.__class__.__name__
+        _matrix = getattr(self.matrices, name)
+        if name == "I":
+            _matrix = _matrix(2 ** len(gate.target_qubits))
+        elif name == "Align":
+            _matrix = _matrix(0, 2)
+        elif callable(_matrix):
+            return self.matrix_parametrized(gate)
+        return np.asarray(_matrix)
+
+    def matrix_parametrized(self, gate):
+        name = gate.__class__.__name__
+        _matrix = getattr(self.matrices, name)
+        if name == "GeneralizedRBS":
+            _matrix = _matrix(qubits_in=gate.init_args[0], qubits_out=gate.init_args[1],
+                              theta=gate.init_kwargs["theta"], phi=gate.init_kwargs["phi"])
+        elif name == "FanOut":
+            _matrix = _matrix(*gate.init_args)
+        else:
+            _matrix = _matrix(*gate.parameters)
+        return np.asarray(_matrix)
+
+    def matrix_fused(self, fgate):
+        """Product of the group's matrices embedded on its sorted qubits.
+
+        Same construction as qibo's matrix_fused (abstract.py:2680-2717): each
+        gate matrix (with its ``controlled_by`` controls as a block diagonal) is
+        extended by a Kronecker identity, its axes are permuted onto the group's
+        qubits, and the result left-multiplies the running product. Dense
+        numpy products replace the sparse ones (at most 32 x 32).
+        """
+        rank = len(fgate.target_qubits)
+        cdt = _DTYPES[self.dtype][1]
+        matrix = np.eye(2**rank, dtype=cdt)
+        for gate in fgate.gates:
+            gmatrix = np.asarray(gate.matrix(self))
+            if len(gate.control_qubits) > 0 and len(gmatrix) < 2 ** len(gate.qubits):
+                big = np.eye(2 ** len(gate.qubits), dtype=gmatrix.dtype)
+                big[-len(gmatrix):, -len(gmatrix):] = gmatrix
+                gmatrix = big
+            eye = np.eye(2 ** (rank - len(gate.qubits)), dtype=gmatrix.dtype)
+            gmatrix = np.kron(gmatrix, eye)
+            shape = gmatrix.shape
+            gmatrix = gmatrix.reshape(2 * rank * (2,))
+            qubits = list(gate.qubits)
+            order = qubits + [q for q in fgate.target_qubits if q not in qubits]
+            perm = [int(i) for i in np.argsort(np.asarray(order, dtype=np.int64))]
+            gmatrix = np.transpose(gmatrix, perm + [p + rank for p in perm]).reshape(shape)
+            matrix = gmatrix @ matrix
+        return matrix.astype(cdt)
+
+    # -- gate application ----------------------------------------------------
+    def plan_gate(self, gate) -> _Op:
+        from .special import FusedGate
+
+        if isinstance(gate, FusedGate):
+            return plan_matrix(self.matrix_fused(gate), gate.target_qubits, ())
+        if gate.is_controlled_by:
+            return plan_matrix(gate.matrix(self), gate.target_qubits, gate.control_qubits)
+        return plan_matrix(gate.matrix(self), gate.qubits, ())
+
+
+class MI355XBackend(GatePlanner):
+    """State-vector simulation backend running on one MI355X (gfx950)."""
+
+    def __init__(self, dtype: str = "complex128", device: int = None):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise_error(RuntimeError, "qibo_amd: no HIP device is visible to torch.")
+        if device is None:
+            device = int(os.environ.get("QIBO_AMD_DEVICE", "0"))
+        self.torch_device = torch.device("cuda", device)
+        self.device = f"/GPU:{device}"
+        self.name = "qibo_amd"
+        self.platform = "mi355x"
+        self.engine = np
+        self.nthreads = 1
+        self.numeric_types = (int, float, complex, np.int32, np.int64, np.float32, np.float64,
+                              np.complex64, np.complex128)
+        self.tensor_types = (torch.Tensor, np.ndarray)
+        self.oom_error = MemoryError
+        self.supports_multigpu = False
+        self.versions = {"qibo_amd": "0.1.0", "torch": torch.__version__}
+        self.dtype = None
+        self.matrices = None
+        self.set_dtype(dtype)
+        self._ws = {}  # workspace cache: (device, purpose, stream) -> tensor
+        self._timer = None  # (selector, [(start, end) events]) while kernel timing is on
+
+    # -- qibo Backend attributes -----------------------------------------
+    @property
+    def qubits(self):
+        return None
+
+    @property
+    def connectivity(self):
+        return None
+
+    @property
+    def natives(self):
+        return None
+
+    def __repr__(self):
+        return f"{self.name} ({self.platform})"
+
+    # -- configuration -----------------------------------------------------
+    def set_dtype(self, dtype: str) -> None:
+        if dtype not in _DTYPES:
+            raise_error(
+                ValueError,
+                f"Unknown ``dtype`` ``{dtype}``. For this backend ({self}), ``dtype`` must be "
+                f"either one of the following string: {tuple(_DTYPES)}",
+            )
+        if dtype != self.dtype:
+            self.dtype = dtype
+            self.matrices = GateMatrices(dtype)
+
+    def set_device(self, device: str) -> None:
+        if not device.startswith("/GPU:"):
+            raise_error(ValueError, f"Device {device} is not available for {self} backend.")
+        idx = int(device.split(":")[1])
+        self.torch_device = torch.device("cuda", idx)
+        self.device = device
+
+    def set_threads(self, nthreads: int) -> None:
+        if nthreads > 1:
+            log.warning("qibo_amd runs on the GPU; set_threads is ignored.")
+
+    def set_seed(self, seed) -> None:
+        # Sampling draws its uniforms from numpy's global RandomState, exactly
+        # as the reference does through np.random.choice (abstract.py:181-187).
+        np.random.seed(seed)
+
+    # -- helpers -----------------------------------------------------------
+    @property
+    def _tdtype(self):
+        return _DTYPES[self.dtype][0]
+
+    def _stream(self, t: torch.Tensor = None):
+        dev = t.device if t is not None else self.torch_device
+        return ctypes_stream(torch.cuda.current_stream(dev))
+
+    def _workspace(self, nbytes: int, purpose: str, device=None) -> torch.Tensor:
+        device = device or self.torch_device
+        key = (device, purpose, torch.cuda.current_stream(device).cuda_stream)
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            self._ws[key] = ws
+        return ws
+
+    def _check_budget(self, nbytes: int):
+        budget = config.HBM_BUDGET_BYTES
+        if budget is not None and nbytes > budget:
+            raise MemoryError(f"state of {nbytes} bytes exceeds the HBM budget of {budget} bytes")
+
+    def _alloc_state(self, nqubits: int, dtype=None) -> torch.Tensor:
+        tdt = _DTYPES[dtype or self.dtype][0]
+        nbytes = (2**nqubits) * (16 if tdt == torch.complex128 else 8)
+        self._check_budget(nbytes)
+        try:
+            return torch.empty(2**nqubits, dtype=tdt, device=self.torch_device)
+        except torch.cuda.OutOfMemoryError as exc:
+            raise MemoryError(str(exc)) from exc
+
+    @staticmethod
+    def _dtype_code(state: torch.Tensor) -> int:
+        if state.dtype == torch.complex128:
+            return _lib.C128
+        if state.dtype == torch.complex64:
+            return _lib.C64
+        raise_error(TypeError, f"State dtype {state.dtype} is not complex64/complex128.")
+
+    def cast(self, array, dtype=None, copy: bool = False):
+        if dtype is None:
+            dtype = self.dtype
+        tdt = _torch_dtype(dtype)
+        if isinstance(array, torch.Tensor):
+            out = array.to(device=self.torch_device, dtype=tdt)
+            if copy and out.data_ptr() == array.data_ptr():
+                out = out.clone()
+            return out
+        arr = np.asarray(array)
+        return torch.as_tensor(np.ascontiguousarray(arr), device=self.torch_device).to(tdt)
+
+    def to_numpy(self, array):
+        if isinstance(array, torch.Tensor):
+            return array.detach().cpu().numpy()
+        return np.asarray(array)
+
+    def zeros(self, shape, dtype=None):
+        return torch.zeros(shape, dtype=_torch_dtype(dtype or self.dtype), device=self.torch_device)
+
+    def concatenate(self, arrays, axis=0):
+        return torch.cat([self.cast(a, dtype=a.dtype) if isinstance(a, torch.Tensor) else
+                          torch.as_tensor(a, device=self.torch_device) for a in arrays], dim=axis)
+
+    def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):
+        np.testing.assert_allclose(self.to_numpy(value), self.to_numpy(target), rtol=rtol, atol=atol)
+
+    # -- state -------------------------------------------------------------
+    def _validate_nqubits(self, nqubits, density_matrix=False):
+        if not isinstance(nqubits, int) or nqubits < 1:
+            raise_error(ValueError, f"nqubits must be a positive integer, but got {nqubits}.")
+        if nqubits > _MAX_QUBITS:
+            raise_error(ValueError, f"qibo_amd supports at most {_MAX_QUBITS} qubits.")
+        if config.MAX_QUBITS != -1 and nqubits > config.MAX_QUBITS:
+            mem_gib = 2**nqubits * 16 / 1024**3
+            raise_error(
+                ValueError,
+                f"Requested {nqubits} qubits for state vector allocation, which would require "
+                f"~{mem_gib:.1f} GiB of memory. The maximum allowed is {config.MAX_QUBITS} qubits.",
+            )
+
+    def zero_state(self, nqubits: int, density_matrix: bool = False, dtype=None):
+        if density_matrix:
+            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+        self._validate_nqubits(nqubits)
+        state = self._alloc_state(nqubits, dtype)
+        _lib.check(self.lib.qsim_zero_state(state.data_ptr(), nqubits, self._dtype_code(state),
+                                            self._stream(state)), self.lib)
+        return state
+
+    # -- per-launch timing (HIP events on the launch stream) -------------------
+    _SELECTORS = {
+        "dense1": lambda op: op.kind == "dense" and len(op.targets) == 1 and not op.controls,
+        "diag1": lambda op: op.kind == "diag" and len(op.targets) == 1 and not op.controls,
+        "fused": lambda op: op.kind == "fused",
+        "any": lambda op: op.kind != "noop",
+    }
+
+    def enable_kernel_timer(self, selector: str = "dense1"):
+        """Record HIP events around every launch whose plan matches ``selector``."""
+        self._timer = (self._SELECTORS[selector], [])
+        return self._timer
+
+    def disable_kernel_timer(self) -> List[float]:
+        """Stop timing; returns the duration (ms) of every timed launch."""
+        if self._timer is None:
+            return []
+        _, pairs = self._timer
+        self._timer = None
+        for _, end in pairs:
+            end.synchronize()
+        return [a.elapsed_time(b) for a, b in pairs]
+
+    def apply_op(self, op: _Op, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        if op.kind == "noop":
+            return state
+        timer = self._timer
+        if timer is not None and timer[0](op):
+            stream = torch.cuda.current_stream(state.device)
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            self._launch(op, state, nqubits)
+            end.record(stream)
+            timer[1].append((start, end))
+            return state
+        return self._launch(op, state, nqubits)
+
+    def _launch(self, op: _Op, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        code = self._dtype_code(state)
+        npdt = np.complex128 if code == _lib.C128 else np.complex64
+        stream = self._stream(state)
+        tg = _lib.int_array(op.targets)
+        ct = _lib.int_array(op.controls)
+        if op.kind == "diag":
+            d = np.ascontiguousarray(op.data, dtype=npdt)
+            rc = self.lib.qsim_apply_diag(state.data_ptr(), nqubits, code, tg, len(op.targets), ct,
+                                          len(op.controls), d.ctypes.data, stream)
+        elif op.kind == "swap":
+            rc = self.lib.qsim_apply_swap(state.data_ptr(), nqubits, code, op.targets[0], op.targets[1],
+                                          ct, len(op.controls), stream)
+        elif op.kind == "dense":
+            m = np.ascontiguousarray(op.data, dtype=npdt)
+            rc = self.lib.qsim_apply_dense(state.data_ptr(), nqubits, code, tg, len(op.targets), ct,
+                                           len(op.controls), m.ctypes.data, stream)
+        else:  # fused: matrix in device memory
+            if op.controls:
+                raise_error(NotImplementedError, "controlled blocks on more than 3 targets")
+            if len(op.targets) > 5:
+                raise_error(NotImplementedError, "dense blocks on more than 5 qubits are not supported")
+            m = torch.from_numpy(np.ascontiguousarray(op.data, dtype=npdt)).to(state.device,
+                                                                                non_blocking=False)
+            rc = self.lib.qsim_apply_fused(state.data_ptr(), nqubits, code, tg, len(op.targets),
+                                           m.data_ptr(), stream)
+            _keep_alive(m, state)
+        _lib.check(rc, self.lib)
+        return state
+
+    def apply_gate(self, gate, state, nqubits):
+        if state.dim() != 1:
+            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+        return self.apply_op(self.plan_gate(gate), state, nqubits)
+
+    # -- circuit execution -----------------------------------------------------
+    def execute_circuit(self, circuit, initial_state=None, nshots: int = 1000):
+        from .circuit import Circuit
+
+        nqubits = circuit.nqubits
+        if circuit.density_matrix:
+            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+        self._validate_nqubits(nqubits)
+        if isinstance(initial_state, Circuit):
+            return self.execute_circuit(initial_state + circuit, None, nshots)
+        if initial_state is not None:
+            initial_state = self.cast(initial_state, dtype=self.dtype, copy=True)
+            if tuple(initial_state.shape) != (2**nqubits,):
+                raise_error(
+                    ValueError,
+                    f"Given initial state has shape {tuple(initial_state.shape)}instead of the "
+                    f"expected {(2**nqubits,)}.",
+                )
+        if circuit.has_collapse or circuit.repeated_execution:
+            raise_error(
+                NotImplementedError,
+                "Collapsing measurements and noise channels need repeated execution, which "
+                "qibo_amd does not implement.",
+            )
+        try:
+            return self._execute_circuit(circuit, initial_state=initial_state, nshots=nshots)
+        except self.oom_error:
+            raise_error(
+                RuntimeError,
+                f"State does not fit in {self.device} memory.Please switch the execution device "
+                "to a different one using ``qibo.set_device``.",
+            )
+
+    def _execute_circuit(self, circuit, initial_state=None, nshots: int = 1000):
+        from .result import CircuitResult, QuantumState
+
+        nqubits = circuit.nqubits
+        state = self.zero_state(nqubits) if initial_state is None else initial_state
+        for gate in circuit.queue:
+            state = gate.apply(self, state, nqubits)
+        if circuit.measurements:
+            circuit._final_state = CircuitResult(state, circuit.measurements, backend=self, nshots=nshots)
+        else:
+            circuit._final_state = QuantumState(state, backend=self)
+        return circuit._final_state
+
+    def execute_circuits(self, circuits, initial_states=None, nshots=1000, processes=None):
+        if initial_states is None:
+            initial_states = [None] * len(circuits)
+        return [self.execute_circuit(c, s, nshots) for c, s in zip(circuits, initial_states)]
+
+    # -- probabilities and sampling -------------------------------------------
+    def calculate_probabilities(self, state, qubits, nqubits, density_matrix=False):
+        if density_matrix:
+            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+        if not isinstance(state, torch.Tensor) or state.device.type != "cuda":
+            state = self.cast(state, dtype=_np_cdtype(state))
+        qubits = [int(q) for q in qubits]
+        code = self._dtype_code(state)
+        rdt = torch.float64 if code == _lib.C128 else torch.float32
+        m = len(qubits)
+        out = torch.empty(2**m, dtype=rdt, device=state.device)
+        wsb = self.lib.qsim_probabilities_workspace(nqubits, m)
+        ws = self._workspace(wsb, "probs", state.device) if wsb else None
+        rc = self.lib.qsim_probabilities(state.data_ptr(), nqubits, code, _lib.int_array(qubits), m,
+                                         out.data_ptr(), ws.data_ptr() if ws is not None else None,
+                                         wsb, self._stream(state))
+        _lib.check(rc, self.lib)
+        return out
+
+    def _as_prob_tensor(self, probabilities) -> torch.Tensor:
+        if isinstance(probabilities, torch.Tensor):
+            p = probabilities.to(self.torch_device)
+            if p.dtype not in (torch.float64, torch.float32):
+                p = p.to(torch.float64)
+            return p.contiguous()
+        arr = np.asarray(probabilities)
+        if arr.dtype not in (np.float64, np.float32):
+            arr = arr.astype(np.float64)
+        return torch.as_tensor(np.ascontiguousarray(arr), device=self.torch_device)
+
+    def _validate_probabilities(self, p: torch.Tensor):
+        """The checks np.random.choice applies to ``p`` before drawing."""
+        atol = math.sqrt(np.finfo(np.float64).eps)
+        if p.dtype == torch.float32:
+            atol = max(atol, math.sqrt(np.finfo(np.float32).eps))
+        pd = p.to(torch.float64)
+        total = float(pd.sum())
+        if math.isnan(total):
+            raise ValueError("probabilities contain NaN")
+        if bool((pd < 0).any()):
+            raise ValueError("probabilities are not non-negative")
+        if abs(total - 1.0) > atol:
+            raise ValueError("probabilities do not sum to 1")
+
+    def exact_cdf(self, p: torch.Tensor) -> torch.Tensor:
+        """Sequential-order cumulative sum of ``p`` as float64 (numpy's cumsum)."""
+        n = p.numel()
+        cdf = torch.empty(n, dtype=torch.float64, device=p.device)
+        wsb = self.lib.qsim_cumsum_workspace(n)
+        ws = self._workspace(wsb, "cumsum", p.device)
+        pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
+        rc = self.lib.qsim_cumsum_exact(p.data_ptr(), pcode, n, cdf.data_ptr(), ws.data_ptr(), wsb,
+                                        self._stream(p))
+        _lib.check(rc, self.lib)
+        return cdf
+
+    def search_cdf(self, cdf: torch.Tensor, uniforms: np.ndarray) -> torch.Tensor:
+        k = len(uniforms)
+        out = torch.empty(k, dtype=torch.int64, device=cdf.device)
+        if k == 0:
+            return out
+        u = torch.as_tensor(np.ascontiguousarray(uniforms, dtype=np.float64), device=cdf.device)
+        rc = self.lib.qsim_search_sorted(cdf.data_ptr(), cdf.numel(), u.data_ptr(), k, out.data_ptr(),
+                                         self._stream(cdf))
+        _lib.check(rc, self.lib)
+        _keep_alive(u, cdf)
+        return out
+
+    def sample_shots(self, probabilities, nshots: int):
+        p = self._as_prob_tensor(probabilities)
+        self._validate_probabilities(p)
+        cdf = self.exact_cdf(p)
+        u = np.random.random_sample(int(nshots))
+        return self.search_cdf(cdf, u)
+
+    def sample_frequencies(self, probabilities, nshots: int):
+        p = self._as_prob_tensor(probabilities)
+        nprobs = p / p.sum()
+        self._validate_probabilities(nprobs)
+        cdf = self.exact_cdf(nprobs)
+        freqs = torch.zeros(nprobs.numel(), dtype=torch.int64, device=p.device)
+        batches = [SHOT_BATCH_SIZE] * (nshots // SHOT_BATCH_SIZE) + [nshots % SHOT_BATCH_SIZE]
+        for k in batches:
+            samples = self.search_cdf(cdf, np.random.random_sample(k))
+            if k:
+                _lib.check(self.lib.qsim_histogram(samples.data_ptr(), k, freqs.data_ptr(),
+                                                   freqs.numel(), self._stream(freqs)), self.lib)
+        nz = torch.nonzero(freqs).flatten()
+        vals = freqs[nz]
+        return collections.Counter(dict(zip(nz.cpu().tolist(), vals.cpu().tolist())))
+
+    def update_frequencies(self, frequencies, probabilities, nsamples):
+        samples = self.sample_shots(probabilities, nsamples)
+        res, counts = torch.unique(samples, return_counts=True)
+        frequencies[res] += counts
+        return frequencies
+
+    def samples_to_binary(self, samples, nqubits):
+        s = samples if isinstance(samples, torch.Tensor) else torch.as_tensor(np.asarray(samples))
+        s = s.to(device=self.torch_device, dtype=torch.int64).contiguous()
+        out = torch.empty((s.numel(), nqubits), dtype=torch.int64, device=s.device)
+        _lib.check(self.lib.qsim_samples_to_binary(s.data_ptr(), s.numel(), nqubits, out.data_ptr(),
+                                                   self._stream(s)), self.lib)
+        return out
+
+    def samples_to_decimal(self, samples, nqubits):
+        s = samples if isinstance(samples, torch.Tensor) else torch.as_tensor(np.asarray(samples))
+        s = s.to(device=self.torch_device, dtype=torch.int64)
+        w = (2 ** torch.arange(nqubits - 1, -1, -1, device=s.device, dtype=torch.int64))
+        # int32 arithmetic of the reference (wraps for >= 32 measured qubits)
+        return (s * w).sum(dim=1).to(torch.int32)
+
+    def calculate_frequencies(self, samples):
+        s = samples if isinstance(samples, torch.Tensor) else torch.as_tensor(np.asarray(samples))
+        res, counts = torch.unique(s, return_counts=True)
+        return collections.Counter(dict(zip(res.cpu().tolist(), counts.cpu().tolist())))
+
+    def aggregate_shots(self, shots):
+        return torch.stack([self.cast(s, dtype=s.dtype) for s in shots])
+
+    # -- expectation values ------------------------------------------------------
+    def pauli_tables(self, terms: Sequence[str], term_qubits, nqubits: int):
+        """X/Z bit masks and Y counts of Pauli strings over amplitude bits."""
+        xm, zm, ny = [], [], []
+        for term, qubits in zip(terms, term_qubits):
+            x = z = y = 0
+            for factor, q in zip(term, qubits):
+                bit = 1 << (nqubits - 1 - int(q))
+                if factor == "X":
+                    x ^= bit
+                elif factor == "Z":
+                    z ^= bit
+                elif factor == "Y":
+                    x ^= bit
+                    z ^= bit
+                    y += 1
+                elif factor != "I":
+                    raise_error(ValueError, f"Unknown Pauli factor {factor}.")
+            xm.append(x)
+            zm.append(z)
+            ny.append(y)
+        return (np.asarray(xm, dtype=np.uint64), np.asarray(zm, dtype=np.uint64),
+                np.asarray(ny, dtype=np.int32))
+
+    def expectation_pauli(self, state: torch.Tensor, nqubits: int, xm, zm, ny, coeffs) -> torch.Tensor:
+        """Device float64 scalar ``sum_t c_t <psi|P_t|psi>`` (no host sync)."""
+        nterms = len(coeffs)
+        out = torch.empty(1, dtype=torch.float64, device=state.device)
+        wsb = self.lib.qsim_expval_workspace(nqubits, nterms)
+        ws = self._workspace(wsb, "expval", state.device)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        xm = np.ascontiguousarray(xm, dtype=np.uint64)
+        zm = np.ascontiguousarray(zm, dtype=np.uint64)
+        ny = np.ascontiguousarray(ny, dtype=np.int32)
+        rc = self.lib.qsim_expval_pauli(state.data_ptr(), nqubits, self._dtype_code(state), nterms,
+                                        xm.ctypes.data, zm.ctypes.data, ny.ctypes.data, c.ctypes.data,
+                                        out.data_ptr(), ws.data_ptr(), wsb, self._stream(state))
+        _lib.check(rc, self.lib)
+        return out[0]
+
+    def exp_value_observable_symbolic(self, circuit, terms, term_qubits, term_coefficients, nqubits):
+        result = circuit._final_state if circuit._final_state is not None else self.execute_circuit(circuit)
+        state = result.state()
+        xm, zm, ny = self.pauli_tables(terms, term_qubits, nqubits)
+        coeffs = [float(np.real(c)) for c in term_coefficients]
+        return float(self.expectation_pauli(state, nqubits, xm, zm, ny, coeffs))
+
+
+_MAX_QUBITS = 36
+
+
+def ctypes_stream(stream) -> int:
+    return stream.cuda_stream
+
+
+def _torch_dtype(dtype):
+    if isinstance(dtype, torch.dtype):
+        return dtype
+    name = getattr(dtype, "name", None) or (dtype if isinstance(dtype, str) else np.dtype(dtype).name)
+    table = {
+        "complex128": torch.complex128, "complex64": torch.complex64, "float64": torch.float64,
+        "float32": torch.float32, "int64": torch.int64, "int32": torch.int32, "int8": torch.int8,
+    }
+    if name not in table:
+        raise_error(ValueError, f"Unsupported dtype {dtype}.")
+    return table[name]
+
+
+def _np_cdtype(array):
+    dt = getattr(array, "dtype", None)
+    if dt is not None and np.dtype(dt) == np.complex64:
+        return "complex64"
+    return "complex128"
+
+
+def _keep_alive(tmp: torch.Tensor, user: torch.Tensor):
+    """Keep a temporary device buffer alive until the stream has consumed it."""
+    tmp.record_stream(torch.cuda.current_stream(user.device))

@@ -1,0 +1,342 @@
+// Gate passes on a state vector held in HBM, in place, for gfx950.
+//
+// A gate touching k target bits under c control bits is applied by walking
+// the 2^(n-k-c) "base" indices (all target bits 0, all control bits 1) and
+// updating the 2^k amplitudes of each base from registers. Base indices are
+// consecutive integers with zero bits inserted at the fixed positions, so
+// consecutive lanes of a wavefront touch consecutive addresses for every
+// target position: for a high target the two halves of a pair are two
+// contiguous streams, for a low target a lane's pair shares a 128-byte line
+// with its neighbours' pairs and both loads of the line are issued back to
+// back, so each line crosses HBM once. Each thread handles kItems bases with
+// all loads issued before the arithmetic, to keep enough bytes in flight.
+//
+// Reference semantics: qibo's Backend.apply_gate (abstract.py:2322-2361) and
+// _apply_gate_controlled_by (abstract.py:3176-3197); the matrix row/column
+// index has gate.qubits[0] as its most significant bit
+// (einsum_utils.permutations, einsum_utils.py:94-110).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "qsim_common.h"
+
+namespace qsim {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+bool validate_qubits(int nqubits, const int* a, int na, const int* b, int nb, const char* what) {
+  if (nqubits < 1 || nqubits > QSIM_MAX_QUBITS) {
+    set_error(std::string(what) + ": nqubits out of range");
+    return false;
+  }
+  uint64_t seen = 0;
+  const int* lists[2] = {a, b};
+  const int counts[2] = {na, nb};
+  for (int l = 0; l < 2; ++l) {
+    if (counts[l] < 0 || (counts[l] > 0 && lists[l] == nullptr)) {
+      set_error(std::string(what) + ": bad qubit list");
+      return false;
+    }
+    for (int i = 0; i < counts[l]; ++i) {
+      const int q = lists[l][i];
+      if (q < 0 || q >= nqubits) {
+        set_error(std::string(what) + ": qubit " + std::to_string(q) + " out of range for " +
+                  std::to_string(nqubits) + " qubits");
+        return false;
+      }
+      const uint64_t bit = uint64_t(1) << q;
+      if (seen & bit) {
+        set_error(std::string(what) + ": qubit " + std::to_string(q) + " given twice");
+        return false;
+      }
+      seen |= bit;
+    }
+  }
+  return true;
+}
+
+int sorted_positions(int nqubits, const int* a, int na, const int* b, int nb, uint8_t* out) {
+  int k = 0;
+  for (int i = 0; i < na; ++i) out[k++] = uint8_t(nqubits - 1 - a[i]);
+  for (int i = 0; i < nb; ++i) out[k++] = uint8_t(nqubits - 1 - b[i]);
+  std::sort(out, out + k);
+  return k;
+}
+
+static inline uint64_t qmask(int nqubits, int q) { return uint64_t(1) << (nqubits - 1 - q); }
+
+// ---------------------------------------------------------------------------
+// |0...0>
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_zero_state(cplx<R>* __restrict__ psi, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    cplx<R> v;
+    v.re = (i == 0) ? R(1) : R(0);
+    v.im = R(0);
+    psi[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dense 2^NT x 2^NT on targets, under controls
+// ---------------------------------------------------------------------------
+template <typename R, int NT>
+struct DenseArgs {
+  uint64_t nwork;              // number of base indices
+  uint64_t cmask;              // control bits, all set in every base
+  uint64_t off[1 << NT];       // offset of matrix index r from the base
+  cplx<R> m[(1 << NT) * (1 << NT)];
+  int nfixed;                  // controls + targets
+  uint8_t pos[kMaxFixed];      // their bit positions, ascending
+};
+
+constexpr int kThreads = 256;
+
+template <typename R, int NT, int kItems>
+__global__ __launch_bounds__(kThreads) void k_dense(const DenseArgs<R, NT> a, cplx<R>* __restrict__ psi) {
+  constexpr int D = 1 << NT;
+  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
+  uint64_t base[kItems];
+  cplx<R> v[kItems][D];
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t j = first + uint64_t(it) * kThreads;
+    base[it] = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
+#pragma clang loop unroll(full)
+    for (int r = 0; r < D; ++r) v[it][r] = ld(psi, base[it] | a.off[r]);
+  }
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t j = first + uint64_t(it) * kThreads;
+    if (j >= a.nwork) continue;
+#pragma clang loop unroll(full)
+    for (int r = 0; r < D; ++r) {
+      cplx<R> acc = cmul(a.m[r * D], v[it][0]);
+#pragma clang loop unroll(full)
+      for (int c = 1; c < D; ++c) acc = cfma(a.m[r * D + c], v[it][c], acc);
+      st(psi, base[it] | a.off[r], acc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// diagonal on targets, under controls: one amplitude per work item
+// ---------------------------------------------------------------------------
+template <typename R>
+struct DiagArgs {
+  uint64_t nwork;          // 2^(n - ncontrols)
+  uint64_t cmask;
+  cplx<R> d[32];
+  int nfixed;              // controls only
+  int nt;                  // number of targets (0..5)
+  uint8_t tbit[5];         // bit positions of targets, target 0 = MSB of diag index
+  uint8_t pos[kMaxFixed];
+};
+
+template <typename R, int kItems>
+__global__ __launch_bounds__(kThreads) void k_diag(const DiagArgs<R> a, cplx<R>* __restrict__ psi) {
+  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
+  uint64_t idx[kItems];
+  cplx<R> v[kItems];
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t j = first + uint64_t(it) * kThreads;
+    idx[it] = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
+    v[it] = ld(psi, idx[it]);
+  }
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t j = first + uint64_t(it) * kThreads;
+    if (j >= a.nwork) continue;
+    int r = 0;
+    for (int t = 0; t < a.nt; ++t) r = (r << 1) | int((idx[it] >> a.tbit[t]) & 1);
+    st(psi, idx[it], cmul(a.d[r], v[it]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// swap of two amplitudes per base, under controls
+// ---------------------------------------------------------------------------
+struct SwapArgs {
+  uint64_t nwork;
+  uint64_t cmask;
+  uint64_t m0, m1;
+  int nfixed;
+  uint8_t pos[kMaxFixed];
+};
+
+template <typename R, int kItems>
+__global__ __launch_bounds__(kThreads) void k_swap(const SwapArgs a, cplx<R>* __restrict__ psi) {
+  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
+  uint64_t ia[kItems], ib[kItems];
+  cplx<R> x[kItems], y[kItems];
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t j = first + uint64_t(it) * kThreads;
+    const uint64_t base = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
+    ia[it] = base | a.m0;
+    ib[it] = base | a.m1;
+    x[it] = ld(psi, ia[it]);
+    y[it] = ld(psi, ib[it]);
+  }
+#pragma clang loop unroll(full)
+  for (int it = 0; it < kItems; ++it) {
+    if (first + uint64_t(it) * kThreads < a.nwork) {
+      st(psi, ia[it], y[it]);
+      st(psi, ib[it], x[it]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+constexpr int kItemsDefault = 4;
+
+template <typename R, int NT>
+static int launch_dense(void* state, int n, const int* targets, const int* controls, int nc,
+                        const void* matrix, hipStream_t s) {
+  DenseArgs<R, NT> a;
+  std::memset(&a, 0, sizeof(a));
+  a.nfixed = sorted_positions(n, controls, nc, targets, NT, a.pos);
+  a.nwork = uint64_t(1) << (n - a.nfixed);
+  a.cmask = 0;
+  for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
+  constexpr int D = 1 << NT;
+  for (int r = 0; r < D; ++r) {
+    uint64_t off = 0;
+    for (int t = 0; t < NT; ++t)
+      if ((r >> (NT - 1 - t)) & 1) off |= qmask(n, targets[t]);
+    a.off[r] = off;
+  }
+  std::memcpy(a.m, matrix, sizeof(a.m));
+  const int items = kItemsDefault;
+  const int blocks = ceil_div_i(a.nwork, uint64_t(items) * kThreads);
+  hipLaunchKernelGGL((k_dense<R, NT, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
+                     reinterpret_cast<cplx<R>*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+template <typename R>
+static int launch_diag(void* state, int n, const int* targets, int nt, const int* controls, int nc,
+                       const void* diag, hipStream_t s) {
+  DiagArgs<R> a;
+  std::memset(&a, 0, sizeof(a));
+  a.nfixed = sorted_positions(n, controls, nc, nullptr, 0, a.pos);
+  a.nwork = uint64_t(1) << (n - a.nfixed);
+  a.cmask = 0;
+  for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
+  a.nt = nt;
+  for (int t = 0; t < nt; ++t) a.tbit[t] = uint8_t(n - 1 - targets[t]);
+  std::memcpy(a.d, diag, sizeof(cplx<R>) * (size_t(1) << nt));
+  const int blocks = ceil_div_i(a.nwork, uint64_t(kItemsDefault) * kThreads);
+  hipLaunchKernelGGL((k_diag<R, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
+                     reinterpret_cast<cplx<R>*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+template <typename R>
+static int launch_swap(void* state, int n, int q0, int q1, const int* controls, int nc, hipStream_t s) {
+  SwapArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const int t[2] = {q0, q1};
+  a.nfixed = sorted_positions(n, controls, nc, t, 2, a.pos);
+  a.nwork = uint64_t(1) << (n - a.nfixed);
+  a.cmask = 0;
+  for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
+  a.m0 = qmask(n, q1);  // |..q0=0..q1=1..>
+  a.m1 = qmask(n, q0);  // |..q0=1..q1=0..>
+  const int blocks = ceil_div_i(a.nwork, uint64_t(kItemsDefault) * kThreads);
+  hipLaunchKernelGGL((k_swap<R, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
+                     reinterpret_cast<cplx<R>*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+extern "C" {
+
+int qsim_version(void) { return QSIM_ABI_VERSION; }
+
+const char* qsim_last_error(void) { return g_last_error.c_str(); }
+
+int qsim_zero_state(void* state, int nqubits, int dtype, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_zero_state: null state");
+  QSIM_CHECK_ARG(nqubits >= 1 && nqubits <= QSIM_MAX_QUBITS, "qsim_zero_state: nqubits out of range");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_zero_state: bad dtype");
+  const uint64_t n = uint64_t(1) << nqubits;
+  const int blocks = ceil_div_i(n, kThreads);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    hipLaunchKernelGGL(k_zero_state<double>, dim3(blocks), dim3(kThreads), 0, s, (c128*)state, n);
+  else
+    hipLaunchKernelGGL(k_zero_state<float>, dim3(blocks), dim3(kThreads), 0, s, (c64*)state, n);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_apply_dense(void* state, int nqubits, int dtype, const int* targets, int ntargets,
+                     const int* controls, int ncontrols, const void* matrix, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr && matrix != nullptr, "qsim_apply_dense: null pointer");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_apply_dense: bad dtype");
+  QSIM_CHECK_ARG(ntargets >= 1 && ntargets <= 3, "qsim_apply_dense: 1..3 targets supported");
+  if (!validate_qubits(nqubits, targets, ntargets, controls, ncontrols, "qsim_apply_dense"))
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128) {
+    switch (ntargets) {
+      case 1: return launch_dense<double, 1>(state, nqubits, targets, controls, ncontrols, matrix, s);
+      case 2: return launch_dense<double, 2>(state, nqubits, targets, controls, ncontrols, matrix, s);
+      default: return launch_dense<double, 3>(state, nqubits, targets, controls, ncontrols, matrix, s);
+    }
+  }
+  switch (ntargets) {
+    case 1: return launch_dense<float, 1>(state, nqubits, targets, controls, ncontrols, matrix, s);
+    case 2: return launch_dense<float, 2>(state, nqubits, targets, controls, ncontrols, matrix, s);
+    default: return launch_dense<float, 3>(state, nqubits, targets, controls, ncontrols, matrix, s);
+  }
+}
+
+int qsim_apply_diag(void* state, int nqubits, int dtype, const int* targets, int ntargets,
+                    const int* controls, int ncontrols, const void* diag, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr && diag != nullptr, "qsim_apply_diag: null pointer");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_apply_diag: bad dtype");
+  QSIM_CHECK_ARG(ntargets >= 0 && ntargets <= 5, "qsim_apply_diag: 0..5 targets supported");
+  if (!validate_qubits(nqubits, targets, ntargets, controls, ncontrols, "qsim_apply_diag"))
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return launch_diag<double>(state, nqubits, targets, ntargets, controls, ncontrols, diag, s);
+  return launch_diag<float>(state, nqubits, targets, ntargets, controls, ncontrols, diag, s);
+}
+
+int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const int* controls,
+                    int ncontrols, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_apply_swap: null state");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_apply_swap: bad dtype");
+  const int t[2] = {q0, q1};
+  if (!validate_qubits(nqubits, t, 2, controls, ncontrols, "qsim_apply_swap")) return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128) return launch_swap<double>(state, nqubits, q0, q1, controls, ncontrols, s);
+  return launch_swap<float>(state, nqubits, q0, q1, controls, ncontrols, s);
+}
+
+}  // extern "C"

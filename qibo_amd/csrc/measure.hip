@@ -1,0 +1,680 @@
+// Probabilities, the exact sequential-equivalent cumulative sum, inverse-CDF
+// search and sample post-processing, for gfx950.
+//
+// This translation unit is compiled with -ffp-contract=off: every floating
+// point operation below must round exactly like numpy's, so no multiply may be
+// fused into a following add behind our back. Fused multiply-adds appear only
+// where numpy itself uses one (its complex abs).
+//
+// Exact cumulative sum. numpy's cumsum (inside np.random.choice) is the
+// sequential chain s_i = fl(s_{i-1} + p_i). A tree scan rounds differently and
+// moves sampled indices. The chain is parallelised exactly as follows. While s
+// stays inside one binade [2^e, 2^(e+1)) every double there is an integer
+// multiple m of u = 2^(e-52), and for p >= 0
+//     fl(s + p) = u * (m + c(p, m mod 2))
+// where c is p/u rounded to nearest, a tie going to the value that makes
+// m + c even. So each element is the map m -> m + c[m & 1], a pair of
+// integers, and composing two such maps gives another one:
+//     (g o f)[b] = f[b] + g[(b + f[b]) & 1].
+// Composition is associative, so tiles of 4096 elements reduce to one map in
+// parallel, a single wavefront chains the tile maps (scanning 64 at a time),
+// and each tile then re-derives every thread's exact start value and finishes
+// its 16 elements with ordinary IEEE additions, which are now exact replicas of
+// numpy's. A tile is only handled by maps if an approximate prefix puts it well
+// inside one binade, and the chain re-checks that exactly (start in the binade,
+// end below 2^(e+1)); the few tiles where s crosses a power of two are summed
+// element by element in order.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "qsim_common.h"
+
+namespace qsim {
+
+// ---------------------------------------------------------------------------
+// numpy's complex abs (loops_unary_complex: larger * sqrt(fma(r, r, 1)),
+// r = smaller / larger, r = 0 when larger == 0 or smaller == inf), squared.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double np_abs2(c128 z) {
+  double re = fabs(z.re), im = fabs(z.im);
+  const double inf = __builtin_inf();
+  if (re == inf) im = inf;
+  if (im == inf) re = inf;
+  if (re != re) im = re;
+  if (im != im) re = im;
+  const double l = fmax(re, im);
+  const double sm = fmin(im, re);
+  const double r = (l == 0.0 || sm == inf) ? 0.0 : __ddiv_rn(sm, l);
+  const double h = __dmul_rn(__dsqrt_rn(__fma_rn(r, r, 1.0)), l);
+  return __dmul_rn(h, h);
+}
+
+__device__ __forceinline__ float np_abs2(c64 z) {
+  float re = fabsf(z.re), im = fabsf(z.im);
+  const float inf = __builtin_inff();
+  if (re == inf) im = inf;
+  if (im == inf) re = inf;
+  if (re != re) im = re;
+  if (im != im) re = im;
+  const float l = fmaxf(re, im);
+  const float sm = fminf(im, re);
+  const float r = (l == 0.0f || sm == inf) ? 0.0f : __fdiv_rn(sm, l);
+  const float h = __fmul_rn(__fsqrt_rn(__fmaf_rn(r, r, 1.0f)), l);
+  return __fmul_rn(h, h);
+}
+
+constexpr int kT = 256;
+
+// Block-wide sum of one double per thread in a fixed order (deterministic).
+__device__ double block_sum_fixed(double v, double* lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lds[threadIdx.x] = lds[threadIdx.x] + lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// probabilities
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kT) void k_probs_all(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                  uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i < n) out[i] = np_abs2(psi[i]);
+}
+
+struct PermArgs {
+  uint64_t nout;
+  int m;
+  uint8_t ibit[kMaxFixed];  // input bit of output bit (m-1-k), k = 0..m-1
+};
+
+// All qubits measured but in another order: gather with a bit permutation.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_probs_perm(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                   const PermArgs a) {
+  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  if (o >= a.nout) return;
+  uint64_t i = 0;
+  for (int k = 0; k < a.m; ++k) i |= ((o >> (a.m - 1 - k)) & 1) << a.ibit[k];
+  out[o] = np_abs2(psi[i]);
+}
+
+struct MargArgs {
+  uint64_t nout;      // 2^m
+  uint64_t nin;       // 2^(n-m) inputs per output
+  uint64_t nchunks;   // blocks per output (partial sums)
+  int m;
+  int nmpos;          // = m
+  uint8_t ibit[kMaxFixed];   // input bit of output bit (m-1-k)
+  uint8_t mpos[kMaxFixed];   // measured bit positions, ascending
+};
+
+__device__ __forceinline__ uint64_t scatter_out(uint64_t o, const MargArgs& a) {
+  uint64_t i = 0;
+  for (int k = 0; k < a.m; ++k) i |= ((o >> (a.m - 1 - k)) & 1) << a.ibit[k];
+  return i;
+}
+
+// One thread per output: few inputs per output.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_thread(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                    const MargArgs a) {
+  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  if (o >= a.nout) return;
+  const uint64_t ob = scatter_out(o, a);
+  double acc = 0.0;
+  for (uint64_t v = 0; v < a.nin; ++v) acc += double(np_abs2(psi[insert_zeros(v, a.mpos, a.nmpos) | ob]));
+  out[o] = R(acc);
+}
+
+// One block per (output, chunk): many inputs per output.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_block(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                   double* __restrict__ partial, const MargArgs a) {
+  __shared__ double lds[kT];
+  const uint64_t b = blockIdx.x;
+  const uint64_t o = b / a.nchunks, c = b % a.nchunks;
+  const uint64_t len = a.nin / a.nchunks;
+  const uint64_t ob = scatter_out(o, a);
+  double acc = 0.0;
+  for (uint64_t v = c * len + threadIdx.x; v < (c + 1) * len; v += kT)
+    acc += double(np_abs2(psi[insert_zeros(v, a.mpos, a.nmpos) | ob]));
+  const double tot = block_sum_fixed(acc, lds);
+  if (threadIdx.x == 0) {
+    if (a.nchunks == 1)
+      out[o] = R(tot);
+    else
+      partial[b] = tot;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_final(const double* __restrict__ partial, R* __restrict__ out,
+                                                   uint64_t nout, uint64_t nchunks) {
+  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  if (o >= nout) return;
+  double acc = 0.0;
+  for (uint64_t c = 0; c < nchunks; ++c) acc += partial[o * nchunks + c];
+  out[o] = R(acc);
+}
+
+static void marg_plan(int n, int m, uint64_t* nchunks, bool* per_thread) {
+  const uint64_t nout = uint64_t(1) << m, nin = uint64_t(1) << (n - m);
+  if (nin <= 64) {
+    *per_thread = true;
+    *nchunks = 1;
+    return;
+  }
+  *per_thread = false;
+  uint64_t c = 1;
+  while (c * 2 * nout <= 4096 && (nin / (c * 2)) >= 256) c *= 2;
+  *nchunks = c;
+}
+
+template <typename R>
+static int probabilities_impl(const void* state, int n, const int* qubits, int m, void* probs, void* ws,
+                              size_t ws_bytes, hipStream_t s) {
+  const cplx<R>* psi = reinterpret_cast<const cplx<R>*>(state);
+  R* out = reinterpret_cast<R*>(probs);
+  bool identity = (m == n);
+  for (int k = 0; identity && k < m; ++k) identity = (qubits[k] == k);
+  if (identity) {
+    const uint64_t N = uint64_t(1) << n;
+    hipLaunchKernelGGL(k_probs_all<R>, dim3(ceil_div_i(N, kT)), dim3(kT), 0, s, psi, out, N);
+    QSIM_CHECK_LAUNCH();
+    return QSIM_OK;
+  }
+  if (m == n) {
+    PermArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nout = uint64_t(1) << m;
+    a.m = m;
+    for (int k = 0; k < m; ++k) a.ibit[k] = uint8_t(n - 1 - qubits[k]);
+    hipLaunchKernelGGL(k_probs_perm<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    QSIM_CHECK_LAUNCH();
+    return QSIM_OK;
+  }
+  MargArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.m = m;
+  a.nmpos = m;
+  a.nout = uint64_t(1) << m;
+  a.nin = uint64_t(1) << (n - m);
+  for (int k = 0; k < m; ++k) a.ibit[k] = uint8_t(n - 1 - qubits[k]);
+  sorted_positions(n, qubits, m, nullptr, 0, a.mpos);
+  bool per_thread;
+  marg_plan(n, m, &a.nchunks, &per_thread);
+  if (per_thread) {
+    hipLaunchKernelGGL(k_marg_thread<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    QSIM_CHECK_LAUNCH();
+    return QSIM_OK;
+  }
+  const uint64_t need = (a.nchunks > 1) ? a.nout * a.nchunks * sizeof(double) : 0;
+  QSIM_CHECK_ARG(need <= ws_bytes && (need == 0 || ws != nullptr), "qsim_probabilities: workspace too small");
+  double* partial = reinterpret_cast<double*>(ws);
+  hipLaunchKernelGGL(k_marg_block<R>, dim3(unsigned(a.nout * a.nchunks)), dim3(kT), 0, s, psi, out, partial, a);
+  QSIM_CHECK_LAUNCH();
+  if (a.nchunks > 1) {
+    hipLaunchKernelGGL(k_marg_final<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, partial, out, a.nout,
+                       a.nchunks);
+    QSIM_CHECK_LAUNCH();
+  }
+  return QSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// exact cumulative sum
+// ---------------------------------------------------------------------------
+constexpr int kPer = 16;                 // elements per thread
+constexpr int kTile = kT * kPer;         // 4096 elements per tile
+constexpr int64_t kInvalid = int64_t(1) << 60;
+constexpr int64_t kTwo53 = int64_t(1) << 53;
+
+struct Map {
+  int64_t c0, c1;  // m -> m + c[m & 1]
+};
+
+__device__ __forceinline__ int64_t sat_add(int64_t a, int64_t b) {
+  const int64_t r = a + b;
+  return r > kInvalid ? kInvalid : r;
+}
+
+// g after f
+__device__ __forceinline__ Map compose(Map f, Map g) {
+  Map h;
+  h.c0 = sat_add(f.c0, (f.c0 & 1) ? g.c1 : g.c0);
+  h.c1 = sat_add(f.c1, ((1 + f.c1) & 1) ? g.c1 : g.c0);
+  return h;
+}
+
+// Biased exponent of the binade whose grid s lies on (subnormals share the
+// grid of the first normal binade).
+__device__ __forceinline__ int binade_of(double s) {
+  const int E = int((__double_as_longlong(s) >> 52) & 0x7ff);
+  return E < 1 ? 1 : E;
+}
+
+__device__ __forceinline__ bool in_binade(double s, int Eb) {
+  const int E = int((__double_as_longlong(s) >> 52) & 0x7ff);
+  return E == Eb || (Eb == 1 && E == 0);
+}
+
+__device__ __forceinline__ int64_t m_of(double s, int Eb) {
+  return int64_t(__double_as_longlong(s)) - (int64_t(Eb - 1) << 52);
+}
+
+__device__ __forceinline__ double from_m(int64_t m, int Eb) {
+  return __longlong_as_double((long long)((int64_t(Eb - 1) << 52) + m));
+}
+
+// The map of adding p (>= 0, finite) on the grid of binade Eb.
+__device__ __forceinline__ Map elem_map(double p, int Eb) {
+  const uint64_t bits = uint64_t(__double_as_longlong(p));
+  const int Ep = int((bits >> 52) & 0x7ff);
+  uint64_t mp = bits & ((uint64_t(1) << 52) - 1);
+  if (Ep != 0) mp |= uint64_t(1) << 52;
+  if (mp == 0) return {0, 0};
+  const int sh = Eb - (Ep < 1 ? 1 : Ep);  // p / u = mp * 2^-sh
+  if (sh < 0) return {kInvalid, kInvalid};
+  if (sh == 0) return {int64_t(mp), int64_t(mp)};
+  if (sh >= 63) return {0, 0};
+  const uint64_t k = mp >> sh;
+  const uint64_t rem = mp & ((uint64_t(1) << sh) - 1);
+  const uint64_t half = uint64_t(1) << (sh - 1);
+  if (rem < half) return {int64_t(k), int64_t(k)};
+  if (rem > half) return {int64_t(k + 1), int64_t(k + 1)};
+  // tie: pick the neighbour that makes m + c even
+  return {int64_t(k + (k & 1)), int64_t(k + ((k + 1) & 1))};
+}
+
+template <typename P>
+__device__ __forceinline__ double load_p(const P* p, int64_t i) {
+  return double(p[i]);
+}
+
+__device__ __forceinline__ Map shfl_up_map(Map v, int delta) {
+  Map r;
+  r.c0 = __shfl_up(v.c0, delta, 64);
+  r.c1 = __shfl_up(v.c1, delta, 64);
+  return r;
+}
+
+__device__ __forceinline__ Map shfl_map(Map v, int lane) {
+  Map r;
+  r.c0 = __shfl(v.c0, lane, 64);
+  r.c1 = __shfl(v.c1, lane, 64);
+  return r;
+}
+
+// Inclusive scan of maps over the 64 lanes of a wavefront (lane order).
+__device__ __forceinline__ Map wave_scan_incl(Map v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const Map o = shfl_up_map(v, off);
+    if (lane >= off) v = compose(o, v);
+  }
+  return v;
+}
+
+struct TileInfo {
+  int64_t c0, c1;
+  int32_t Eb;
+  int32_t flag;  // k_tile_map: 1 = maps usable; k_carry: 0 = apply by maps, 1 = done sequentially
+};
+
+// (A) approximate tile sums
+template <typename P>
+__global__ __launch_bounds__(kT) void k_tile_sum(const P* __restrict__ p, int64_t n, double* __restrict__ tsum) {
+  __shared__ double lds[kT];
+  const int64_t base = int64_t(blockIdx.x) * kTile;
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    if (i < n) acc += load_p(p, i);
+  }
+  const double tot = block_sum_fixed(acc, lds);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// (B) approximate exclusive prefix of the tile sums (one block)
+__global__ __launch_bounds__(1024) void k_approx_prefix(const double* __restrict__ tsum, int64_t ntiles,
+                                                        double* __restrict__ astart) {
+  __shared__ double lds[1024];
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t lo = int64_t(threadIdx.x) * per;
+  const int64_t hi = lo + per < ntiles ? lo + per : ntiles;
+  double acc = 0.0;
+  for (int64_t t = lo; t < hi; ++t) acc += tsum[t];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const double v = threadIdx.x >= off ? lds[threadIdx.x - off] : 0.0;
+    __syncthreads();
+    lds[threadIdx.x] += v;
+    __syncthreads();
+  }
+  double run = threadIdx.x > 0 ? lds[threadIdx.x - 1] : 0.0;
+  for (int64_t t = lo; t < hi; ++t) {
+    astart[t] = run;
+    run += tsum[t];
+  }
+}
+
+// Stage a tile into LDS with coalesced loads; returns the thread's 16
+// consecutive elements' offset in the tile.
+template <typename P>
+__device__ __forceinline__ void stage_tile(const P* __restrict__ p, int64_t n, int64_t base, double* tile) {
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    tile[k * kT + threadIdx.x] = i < n ? load_p(p, i) : 0.0;
+  }
+  __syncthreads();
+}
+
+// Exclusive scan of one map per thread across the block (thread order).
+__device__ __forceinline__ Map block_scan_excl(Map v, Map* wave_tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const Map incl = wave_scan_incl(v);
+  Map excl = shfl_up_map(incl, 1);
+  if (lane == 0) excl = {0, 0};
+  if (lane == 63) wave_tot[wave] = incl;
+  __syncthreads();
+  Map pre = {0, 0};
+  for (int w = 0; w < wave; ++w) pre = compose(pre, wave_tot[w]);
+  __syncthreads();
+  return compose(pre, excl);
+}
+
+// (C) per-tile decision and map
+template <typename P>
+__global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_t n, const double* __restrict__ astart,
+                                                 const double* __restrict__ tsum, TileInfo* __restrict__ info) {
+  __shared__ double tile[kTile];
+  __shared__ Map wave_tot[kT / 64];
+  const int64_t t = blockIdx.x;
+  const double A = astart[t];
+  const double Z = A + tsum[t];
+  const int Eb = binade_of(A);
+  const double margin = 1.0 + 0x1p-20;
+  const double lo = ldexp(1.0, Eb - 1023), hi = ldexp(1.0, Eb - 1022);
+  const bool safe = (Eb == 1 || A >= lo * margin) && (Z * margin < hi);
+  if (!safe) {
+    if (threadIdx.x == 0) info[t] = {0, 0, Eb, 0};
+    return;
+  }
+  stage_tile(p, n, t * kTile, tile);
+  Map f = {0, 0};
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) f = compose(f, elem_map(tile[threadIdx.x * kPer + k], Eb));
+  const Map excl = block_scan_excl(f, wave_tot);
+  if (threadIdx.x == kT - 1) {
+    const Map tot = compose(excl, f);
+    const bool ok = tot.c0 < kTwo53 && tot.c1 < kTwo53;
+    info[t] = {tot.c0, tot.c1, Eb, ok ? 1 : 0};
+  }
+}
+
+// (D) exact chain over tiles: one wavefront
+template <typename P>
+__global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n, int64_t ntiles,
+                                              TileInfo* __restrict__ info, double* __restrict__ tstart,
+                                              double* __restrict__ cdf) {
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (int64_t g = 0; g < ntiles; g += 64) {
+    const int64_t t = g + lane;
+    TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
+    const int Eb0 = __shfl(ti.Eb, 0, 64);
+    if (t >= ntiles) ti.Eb = Eb0;
+    const bool mine = ti.flag == 1 && ti.Eb == Eb0;
+    const bool uniform = __all(mine) && in_binade(s, Eb0);
+    if (uniform) {
+      const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
+      const Map tot = shfl_map(incl, 63);
+      const int64_t m0 = m_of(s, Eb0);
+      const int64_t mend = m0 + ((m0 & 1) ? tot.c1 : tot.c0);
+      if (mend < kTwo53) {
+        Map excl = shfl_up_map(incl, 1);
+        if (lane == 0) excl = {0, 0};
+        if (t < ntiles) {
+          const int64_t ml = m0 + ((m0 & 1) ? excl.c1 : excl.c0);
+          tstart[t] = from_m(ml, Eb0);
+          info[t].flag = 0;
+        }
+        s = from_m(mend, Eb0);
+        continue;
+      }
+    }
+    // slow path: tile by tile, every lane computing the same scalar chain
+    const int64_t gend = g + 64 < ntiles ? g + 64 : ntiles;
+    for (int64_t u = g; u < gend; ++u) {
+      const TileInfo tu = info[u];
+      if (lane == 0) tstart[u] = s;
+      bool ok = tu.flag == 1 && in_binade(s, tu.Eb);
+      int64_t m2 = 0;
+      if (ok) {
+        const int64_t m = m_of(s, tu.Eb);
+        m2 = m + ((m & 1) ? tu.c1 : tu.c0);
+        ok = m2 < kTwo53;
+      }
+      if (ok) {
+        s = from_m(m2, tu.Eb);
+        if (lane == 0) info[u].flag = 0;
+      } else {
+        const int64_t lo = u * kTile;
+        const int64_t hi = lo + kTile < n ? lo + kTile : n;
+        for (int64_t i = lo; i < hi; ++i) {
+          s = s + load_p(p, i);
+          if (lane == 0) cdf[i] = s;
+        }
+        if (lane == 0) info[u].flag = 1;
+      }
+    }
+  }
+}
+
+// (E) finish every tile that the chain resolved by maps
+template <typename P>
+__global__ __launch_bounds__(kT) void k_tile_apply(const P* __restrict__ p, int64_t n, const TileInfo* __restrict__ info,
+                                                   const double* __restrict__ tstart, double* __restrict__ cdf) {
+  __shared__ double tile[kTile];
+  __shared__ Map wave_tot[kT / 64];
+  const int64_t t = blockIdx.x;
+  const TileInfo ti = info[t];
+  if (ti.flag != 0) return;
+  const int64_t base = t * kTile;
+  stage_tile(p, n, base, tile);
+  Map f = {0, 0};
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) f = compose(f, elem_map(tile[threadIdx.x * kPer + k], ti.Eb));
+  const Map excl = block_scan_excl(f, wave_tot);
+  const double s0 = tstart[t];
+  const int64_t m0 = m_of(s0, ti.Eb);
+  double s = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), ti.Eb);
+  double out[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    s = s + tile[threadIdx.x * kPer + k];
+    out[k] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) tile[threadIdx.x * kPer + k] = out[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    if (i < n) cdf[i] = tile[k * kT + threadIdx.x];
+  }
+}
+
+struct CumsumWs {
+  double* tsum;
+  double* astart;
+  double* tstart;
+  TileInfo* info;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static size_t cumsum_ws_bytes(int64_t n) {
+  const size_t T = size_t((n + kTile - 1) / kTile);
+  return 3 * align256(T * sizeof(double)) + align256(T * sizeof(TileInfo));
+}
+
+template <typename P>
+static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipStream_t s) {
+  const P* p = reinterpret_cast<const P*>(probs);
+  const int64_t T = (n + kTile - 1) / kTile;
+  char* w = reinterpret_cast<char*>(ws);
+  CumsumWs c;
+  c.tsum = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  c.astart = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  c.tstart = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  c.info = reinterpret_cast<TileInfo*>(w);
+  hipLaunchKernelGGL(k_tile_sum<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.tsum);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_approx_prefix, dim3(1), dim3(1024), 0, s, c.tsum, T, c.astart);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(64), 0, s, p, n, T, c.info, c.tstart, cdf);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tile_apply<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.info, c.tstart, cdf);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// search, bits, histogram
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_search(const double* __restrict__ cdf, int64_t n,
+                                               const double* __restrict__ u, int64_t nshots,
+                                               int64_t* __restrict__ out) {
+  const int64_t k = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (k >= nshots) return;
+  const double total = cdf[n - 1];
+  const double v = u[k];
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (__ddiv_rn(cdf[mid], total) <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  out[k] = lo;
+}
+
+__global__ __launch_bounds__(kT) void k_to_binary(const int64_t* __restrict__ smp, int64_t nshots, int nbits,
+                                                  int64_t* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i >= nshots * nbits) return;
+  const int64_t s = i / nbits;
+  const int b = int(i % nbits);
+  out[i] = (smp[s] >> (nbits - 1 - b)) & 1;
+}
+
+__global__ __launch_bounds__(kT) void k_histogram(const int64_t* __restrict__ smp, int64_t nshots,
+                                                  unsigned long long* __restrict__ counts, int64_t nbins) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i >= nshots) return;
+  const int64_t v = smp[i];
+  if (v >= 0 && v < nbins) atomicAdd(&counts[v], 1ull);
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+extern "C" {
+
+size_t qsim_probabilities_workspace(int nqubits, int nmeasured) {
+  if (nqubits < 1 || nqubits > QSIM_MAX_QUBITS || nmeasured < 1 || nmeasured > nqubits) return 0;
+  if (nmeasured == nqubits) return 0;
+  uint64_t nchunks;
+  bool per_thread;
+  marg_plan(nqubits, nmeasured, &nchunks, &per_thread);
+  if (per_thread || nchunks == 1) return 0;
+  return size_t((uint64_t(1) << nmeasured) * nchunks * sizeof(double));
+}
+
+int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qubits, int nmeasured,
+                       void* probs, void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr && probs != nullptr, "qsim_probabilities: null pointer");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_probabilities: bad dtype");
+  QSIM_CHECK_ARG(nmeasured >= 1 && nmeasured <= nqubits, "qsim_probabilities: bad number of qubits");
+  if (!validate_qubits(nqubits, qubits, nmeasured, nullptr, 0, "qsim_probabilities")) return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return probabilities_impl<double>(state, nqubits, qubits, nmeasured, probs, workspace, workspace_bytes, s);
+  return probabilities_impl<float>(state, nqubits, qubits, nmeasured, probs, workspace, workspace_bytes, s);
+}
+
+size_t qsim_cumsum_workspace(int64_t n) { return n < 1 ? 0 : cumsum_ws_bytes(n); }
+
+int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(probs != nullptr && cdf != nullptr, "qsim_cumsum_exact: null pointer");
+  QSIM_CHECK_ARG(n >= 1 && n <= (int64_t(1) << QSIM_MAX_QUBITS), "qsim_cumsum_exact: bad length");
+  QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_cumsum_exact: bad dtype");
+  QSIM_CHECK_ARG(workspace != nullptr && workspace_bytes >= cumsum_ws_bytes(n),
+                 "qsim_cumsum_exact: workspace too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (prob_dtype == QSIM_F64) return cumsum_impl<double>(probs, n, cdf, workspace, s);
+  return cumsum_impl<float>(probs, n, cdf, workspace, s);
+}
+
+int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int64_t nshots, int64_t* out,
+                       void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(cdf != nullptr && uniforms != nullptr && out != nullptr, "qsim_search_sorted: null pointer");
+  QSIM_CHECK_ARG(n >= 1 && nshots >= 0, "qsim_search_sorted: bad sizes");
+  if (nshots == 0) return QSIM_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_search, dim3(ceil_div_i(uint64_t(nshots), kT)), dim3(kT), 0, s, cdf, n, uniforms, nshots, out);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_samples_to_binary(const int64_t* samples, int64_t nshots, int nbits, int64_t* out, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(samples != nullptr && out != nullptr, "qsim_samples_to_binary: null pointer");
+  QSIM_CHECK_ARG(nbits >= 1 && nbits <= 63 && nshots >= 0, "qsim_samples_to_binary: bad sizes");
+  if (nshots == 0) return QSIM_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const uint64_t total = uint64_t(nshots) * uint64_t(nbits);
+  hipLaunchKernelGGL(k_to_binary, dim3(ceil_div_i(total, kT)), dim3(kT), 0, s, samples, nshots, nbits, out);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_histogram(const int64_t* samples, int64_t nshots, int64_t* counts, int64_t nbins, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(samples != nullptr && counts != nullptr, "qsim_histogram: null pointer");
+  QSIM_CHECK_ARG(nbins >= 1 && nshots >= 0, "qsim_histogram: bad sizes");
+  if (nshots == 0) return QSIM_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_histogram, dim3(ceil_div_i(uint64_t(nshots), kT)), dim3(kT), 0, s, samples, nshots,
+                     reinterpret_cast<unsigned long long*>(counts), nbins);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+}  // extern "C"

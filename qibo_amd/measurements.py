@@ -1,0 +1,76 @@
+"""Per-register measurement results (``src/qibo/measurements.py:97-222``).
+
+A ``MeasurementResult`` belongs to one ``M`` gate. The circuit result fills it
+with the columns of the global sample matrix that belong to this register, or
+with per-register frequencies when only frequencies were sampled.
+"""
+
+import collections
+
+import numpy as np
+
+from .config import raise_error
+
+
+def frequencies_to_binary(frequencies, nqubits):
+    return collections.Counter({f"{k:0{nqubits}b}": v for k, v in frequencies.items()})
+
+
+class MeasurementResult:
+    def __init__(self, target_qubits):
+        self.target_qubits = tuple(target_qubits)
+        self.circuit = None
+        self.backend = None
+        self._samples = None
+        self._frequencies = None
+
+    def has_samples(self):
+        return self._samples is not None
+
+    def register_samples(self, samples):
+        self._samples = samples
+
+    def register_frequencies(self, frequencies):
+        self._frequencies = frequencies
+
+    def reset(self):
+        self._samples = None
+        self._frequencies = None
+
+    @property
+    def nshots(self):
+        if self.has_samples():
+            return len(self._samples)
+        if self._frequencies is not None:
+            return sum(self._frequencies.values())
+        return None
+
+    def samples(self, binary=True, registers=False, backend=None):
+        from .backends import _check_backend
+
+        backend = _check_backend(backend)
+        if self._samples is None:
+            if self.circuit is None:
+                raise_error(RuntimeError, "Cannot calculate samples if circuit is not provided.")
+            self.circuit.final_state.samples()
+        if binary:
+            return self._samples
+        return backend.samples_to_decimal(self._samples, len(self.target_qubits))
+
+    def frequencies(self, binary=True, registers=False, backend=None):
+        from .backends import _check_backend
+
+        backend = _check_backend(backend)
+        if self._frequencies is None:
+            self._frequencies = backend.calculate_frequencies(
+                self.samples(binary=False, backend=backend)
+            )
+        if binary:
+            return frequencies_to_binary(self._frequencies, len(self.target_qubits))
+        return self._frequencies
+
+
+def _to_numpy(x):
+    if hasattr(x, "detach"):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)

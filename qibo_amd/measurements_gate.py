@@ -1,0 +1,111 @@
+"""The measurement gate ``M``.
+
+Follows ``src/qibo/gates/measurements.py:11-204``: ``M(*q)`` records which
+qubits are measured (``target_qubits``), owns a per-register
+``MeasurementResult``, adds basis-rotation gates for non-Z bases, and on a
+state vector is a no-op unless ``collapse=True`` (measurements.py:189-197).
+Collapsing measurements need repeated (per-shot) execution and are not part
+of this backend's path: they raise ``NotImplementedError`` when executed.
+"""
+
+from typing import Optional
+
+from . import gates as _g
+from .config import raise_error
+from .measurements import MeasurementResult
+
+
+class M(_g.Gate):
+    def __init__(
+        self,
+        *q,
+        register_name: Optional[str] = None,
+        collapse: bool = False,
+        basis=None,
+        p0=None,
+        p1=None,
+    ):
+        super().__init__()
+        self.name = "measure"
+        self.draw_label = "M"
+        self.target_qubits = tuple(q)
+        self.register_name = register_name
+        self.collapse = collapse
+        self.result = MeasurementResult(self.target_qubits)
+        if basis is None:
+            basis = _g.Z
+        to_gate = lambda x: getattr(_g, x) if isinstance(x, str) else x
+        if not isinstance(basis, list):
+            self.basis_gates = len(q) * [to_gate(basis)]
+        elif len(basis) != len(self.target_qubits):
+            raise_error(
+                ValueError,
+                f"Given basis list has length {len(basis)} while we are measuring "
+                f"{len(self.target_qubits)} qubits.",
+            )
+        else:
+            self.basis_gates = [to_gate(b) for b in basis]
+        self.init_args = q
+        self.init_kwargs = {
+            "register_name": register_name,
+            "collapse": collapse,
+            "basis": [b.__name__ for b in self.basis_gates],
+            "p0": p0,
+            "p1": p1,
+        }
+        if collapse and (p0 is not None or p1 is not None):
+            raise_error(
+                NotImplementedError, "Bitflip measurement noise is not available when collapsing."
+            )
+        if p0 is not None or p1 is not None:
+            raise_error(NotImplementedError, "Bitflip measurement noise is not supported.")
+        self.bitflip_map = ({q_: 0 for q_ in self.target_qubits}, {q_: 0 for q_ in self.target_qubits})
+        # basis change gates added by the circuit before this measurement
+        self.basis = []
+        for qubit, cls in zip(self.target_qubits, self.basis_gates):
+            rot = _basis_rotation(cls, qubit)
+            if rot is not None:
+                self.basis.append(rot)
+
+    def has_bitflip_noise(self):
+        return False
+
+    def add(self, gate):
+        """Merge another measurement's qubits (used for the global measurement gate)."""
+        assert isinstance(gate, self.__class__)
+        self.target_qubits += gate.target_qubits
+        self.bitflip_map[0].update(gate.bitflip_map[0])
+        self.bitflip_map[1].update(gate.bitflip_map[1])
+
+    def controlled_by(self, *q):
+        raise_error(NotImplementedError, "Measurement gates cannot be controlled.")
+
+    def matrix(self, backend=None):
+        raise_error(NotImplementedError, "Measurement gates do not have a matrix representation.")
+
+    def apply(self, backend, state, nqubits):
+        self.result.backend = backend
+        if not self.collapse:
+            return state
+        raise_error(
+            NotImplementedError,
+            "Collapsing measurements (mid-circuit collapse) are not supported by this backend.",
+        )
+
+
+def _basis_rotation(cls, qubit):
+    """Gate rotating ``cls``'s eigenbasis onto Z (gates.py ``basis_rotation``)."""
+    name = cls.__name__
+    if name == "Z":
+        return None
+    if name == "X":
+        return _g.H(qubit)
+    if name == "Y":
+        # (Y + Z) / sqrt(2), as Y.basis_rotation (gates.py:235-241)
+        import math
+
+        import numpy as np
+
+        m = (np.array([[0j, -1j], [1j, 0j]]) + np.array([[1, 0], [0, -1]], dtype=complex)) / math.sqrt(2)
+        return _g.Unitary(m, qubit, trainable=False)
+    raise_error(NotImplementedError, f"Measurement basis {name} is not supported.")

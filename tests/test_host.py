@@ -1,0 +1,226 @@
+"""Host-side logic of the backend, checked on the CPU against the oracle.
+
+* gate matrices of qibo_amd.matrices are bitwise those of the oracle's
+  NumpyMatrices restatement (both dtypes);
+* the kernel plan chosen for a gate (controls peeled off, diagonal, swap,
+  dense) reproduces the dense product when emulated in numpy;
+* fusion grouping follows the structural goldens of
+  tests/test_models_circuit_fuse.py:24-54 of the reference;
+* the fused-matrix construction equals the oracle's sparse one;
+* circuit bookkeeping (registers, parameters, inversion) behaves like qibo's.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import OracleBackend
+from qibo_amd import Circuit, gates, hamiltonians, models
+from qibo_amd.backend import GatePlanner, plan_matrix
+
+GATE_FACTORIES = [
+    lambda: gates.H(0), lambda: gates.X(1), lambda: gates.Y(2), lambda: gates.Z(0),
+    lambda: gates.S(1), lambda: gates.SDG(1), lambda: gates.T(2), lambda: gates.TDG(2),
+    lambda: gates.SX(0), lambda: gates.SXDG(0), lambda: gates.I(1),
+    lambda: gates.RX(0, 0.123), lambda: gates.RY(1, 2.5), lambda: gates.RZ(2, -1.7),
+    lambda: gates.U1(0, 0.3), lambda: gates.U2(1, 0.4, -0.9), lambda: gates.U3(2, 0.1, 0.2, 0.3),
+    lambda: gates.CNOT(0, 2), lambda: gates.CY(1, 0), lambda: gates.CZ(2, 1), lambda: gates.CH(0, 1),
+    lambda: gates.CRX(0, 1, 0.7), lambda: gates.CRY(1, 2, -0.7), lambda: gates.CRZ(2, 0, 1.9),
+    lambda: gates.CU1(0, 2, 0.25), lambda: gates.SWAP(0, 2), lambda: gates.iSWAP(1, 2),
+    lambda: gates.FSWAP(0, 1), lambda: gates.fSim(0, 2, 0.4, 1.3), lambda: gates.RXX(0, 1, 0.3),
+    lambda: gates.RZZ(1, 2, 0.9), lambda: gates.TOFFOLI(0, 1, 2), lambda: gates.CCZ(2, 0, 1),
+]
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("make", GATE_FACTORIES)
+def test_matrices_bitwise_equal_oracle(make, dtype):
+    g = make()
+    mine = GatePlanner(dtype).matrix(g)
+    ref = OracleBackend(dtype).matrix(g)
+    assert mine.dtype == ref.dtype
+    np.testing.assert_array_equal(mine, ref)
+
+
+def emulate(op, state, n):
+    """numpy emulation of the kernel semantics of a planned op."""
+    psi = state.copy()
+    if op.kind == "noop":
+        return psi
+    idx = np.arange(2**n)
+    cmask = 0
+    for c in op.controls:
+        cmask |= 1 << (n - 1 - c)
+    active = (idx & cmask) == cmask
+    if op.kind == "diag":
+        r = np.zeros_like(idx)
+        for t in op.targets:
+            r = (r << 1) | ((idx >> (n - 1 - t)) & 1)
+        out = psi.copy()
+        out[active] = psi[active] * op.data[r[active]]
+        return out
+    if op.kind == "swap":
+        a, b = op.targets
+        ma, mb = 1 << (n - 1 - a), 1 << (n - 1 - b)
+        sel = active & ((idx & ma) != 0) & ((idx & mb) == 0)
+        out = psi.copy()
+        i1 = idx[sel]
+        i2 = (i1 & ~ma) | mb
+        out[i1], out[i2] = psi[i2], psi[i1]
+        return out
+    k = len(op.targets)
+    tmask = 0
+    for t in op.targets:
+        tmask |= 1 << (n - 1 - t)
+    out = psi.copy()
+    for base in idx[active & ((idx & tmask) == 0)]:
+        ids = []
+        for r in range(2**k):
+            i = base
+            for j, t in enumerate(op.targets):
+                if (r >> (k - 1 - j)) & 1:
+                    i |= 1 << (n - 1 - t)
+            ids.append(i)
+        out[ids] = op.data @ psi[ids]
+    return out
+
+
+@pytest.mark.parametrize("make", GATE_FACTORIES + [
+    lambda: gates.RX(1, 0.8).controlled_by(0, 2),
+    lambda: gates.SWAP(1, 2).controlled_by(0),
+    lambda: gates.U1(2, 0.4).controlled_by(0, 1),
+    lambda: gates.Unitary(np.linalg.qr(np.arange(16).reshape(4, 4) + 1j)[0], 2, 0),
+])
+def test_plan_reproduces_dense_product(make, rng):
+    n = 4
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    g = make()
+    op = GatePlanner().plan_gate(g)
+    want = OracleBackend().apply_gate(g, psi.copy(), n)
+    np.testing.assert_allclose(emulate(op, psi, n), want, atol=1e-14, rtol=0)
+
+
+def test_plan_kinds():
+    p = GatePlanner()
+    cz = p.plan_gate(gates.CZ(0, 1))
+    assert cz.kind == "diag" and cz.targets == () and set(cz.controls) == {0, 1}
+    cnot = p.plan_gate(gates.CNOT(2, 0))
+    assert cnot.kind == "dense" and cnot.targets == (0,) and cnot.controls == (2,)
+    assert p.plan_gate(gates.SWAP(0, 1)).kind == "swap"
+    assert p.plan_gate(gates.RZ(0, 0.3)).kind == "diag"
+    u1 = p.plan_gate(gates.U1(1, 0.2))
+    assert u1.kind == "diag" and u1.targets == () and u1.controls == (1,)
+    assert p.plan_gate(gates.I(0)).kind == "noop"
+    tof = p.plan_gate(gates.TOFFOLI(0, 1, 2))
+    assert tof.kind == "dense" and tof.targets == (2,) and set(tof.controls) == {0, 1}
+    assert plan_matrix(np.eye(16), (0, 1, 2, 3), ()).kind == "noop"
+
+
+def test_single_fusion_gate():
+    queue = [gates.H(0), gates.X(1), gates.CZ(0, 1)]
+    c = Circuit(2)
+    c.add(queue)
+    c = c.fuse()
+    assert len(c.queue) == 1
+    fg = c.queue[0]
+    assert fg.gates[0] == queue[1] and fg.gates[1] == queue[0] and fg.gates[2] == queue[2]
+
+
+def test_two_fusion_gate():
+    queue = [gates.X(0), gates.H(1), gates.RX(2, theta=0.1234).controlled_by(1), gates.H(2),
+             gates.Y(1), gates.H(0)]
+    c = Circuit(3)
+    c.add(queue)
+    c = c.fuse()
+    assert len(c.queue) == 2
+    f1, f2 = c.queue
+    assert f2.gates[0] == queue[0] and f2.gates[1] == queue[-1]
+    assert f1.gates == [queue[1], queue[2], queue[4], queue[3]]
+
+
+def test_fusedgate_matrix_calculation():
+    queue = [gates.H(0), gates.H(1), gates.CNOT(0, 1), gates.X(0), gates.X(1)]
+    c = Circuit(2)
+    c.add(queue)
+    c = c.fuse()
+    assert len(c.queue) == 1
+    x = np.array([[0, 1], [1, 0]])
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    cnot = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]])
+    np.testing.assert_allclose(GatePlanner().matrix_fused(c.queue[0]),
+                               np.kron(x, x) @ cnot @ np.kron(h, h), atol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_matrix_fused_equals_oracle(dtype):
+    c = models.random_layered_circuit(6, 4, seed=11, measure=False)
+    c.add(gates.RX(3, 0.2).controlled_by(1))
+    c.add(gates.TOFFOLI(0, 4, 5))
+    tol = 1e-14 if dtype == "complex128" else 2e-6
+    for k in (2, 3, 4, 5):
+        for g in c.fuse(max_qubits=k).queue:
+            if isinstance(g, gates.FusedGate):
+                np.testing.assert_allclose(GatePlanner(dtype).matrix_fused(g),
+                                           OracleBackend(dtype).matrix_fused(g), atol=tol)
+
+
+def test_fusion_keeps_measurements():
+    c = Circuit(3)
+    c.add(gates.X(i) for i in range(3))
+    c.add(gates.CNOT(0, 1))
+    c.add(gates.H(2))
+    c.add(gates.M(0, 1, 2))
+    f = c.fuse(max_qubits=2)
+    assert f.measurements == c.measurements
+    assert isinstance(f.queue[-1], gates.M)
+
+
+def test_circuit_bookkeeping():
+    c = Circuit(3)
+    c.add(gates.RX(0, 0.1))
+    c.add(gates.fSim(0, 2, 0.3, 0.4))
+    c.add(gates.H(1))
+    r = c.add(gates.M(0, 2, register_name="a"))
+    c.add(gates.M(1))
+    assert c.measurements[0].register_name == "a" and c.measurements[1].register_name == "register1"
+    assert r is c.measurements[0].result
+    c.set_parameters([0.5, (0.6, 0.7)])
+    assert c.queue[0].parameters == (0.5,) and c.queue[1].parameters == (0.6, 0.7)
+    c.set_parameters([1.0, 2.0, 3.0])
+    assert c.get_parameters() == [(1.0,), (2.0, 3.0)]
+    with pytest.raises(TypeError):  # a scalar for a two-parameter gate, as in qibo
+        c.set_parameters([1.0, 2.0])
+    with pytest.raises(ValueError):
+        c.set_parameters([1.0, 2.0, 3.0, 4.0])
+    with pytest.raises(ValueError):
+        Circuit(2).add(gates.H(5))
+    with pytest.raises(KeyError):
+        c.add(gates.M(1, register_name="a"))
+    inv = models.QFT(3).invert()
+    assert inv.ngates == models.QFT(3).ngates
+
+
+def test_invert_is_inverse(rng):
+    n = 4
+    c = models.random_layered_circuit(n, 2, seed=5, measure=False)
+    c.add(gates.CU1(0, 3, 0.3))
+    c.add(gates.U3(1, 0.1, 0.2, 0.3))
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    ob = OracleBackend()
+    back = ob.execute(c.invert(), ob.execute(c, psi))
+    np.testing.assert_allclose(back, psi, atol=1e-12)
+
+
+def test_layered_generator_counts():
+    c = models.random_layered_circuit(28, 20, seed=1)
+    names = c.gate_names
+    assert names["ry"] == 560 and names["rz"] == 560 and names["cz"] == 270 and names["measure"] == 1
+    h = models.hardware_efficient_ansatz(20, 10)
+    assert len(h.trainable_gates) == 400 and h.ngates == 590
+
+
+def test_tfim_terms():
+    ham = hamiltonians.TFIM(4, h=1.0, closed_boundary=False)
+    coeffs, strings, qubits = ham.simple_terms
+    assert sorted(zip(strings, qubits)) == sorted(
+        [("ZZ", (0, 1)), ("ZZ", (1, 2)), ("ZZ", (2, 3)), ("X", (0,)), ("X", (1,)), ("X", (2,)), ("X", (3,))])
+    assert all(c == -1.0 for c in coeffs)

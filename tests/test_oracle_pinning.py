@@ -1,0 +1,383 @@
+"""Pins of the CPU oracle (oracle/numpy_backend.py) -- CPU only.
+
+What pins each oracle function (none of these retypes the oracle's formula):
+
+* gate matrices      -- closed forms of the reference's tests/test_backends.py:109-188
+* apply_gate         -- brute-force 2^n x 2^n unitaries built bit by bit here;
+                        index conventions of tests/test_gates_gates.py:39-43, 549-564
+* controlled_by      -- the same brute force with control masks
+* circuit execution  -- QFT against the exact DFT matrix (tests/test_models_qft.py:11-77)
+* probabilities      -- brute-force marginals; numpy's complex-abs formula
+* sample_shots / sample_frequencies -- seeded goldens of
+                        tests/test_measurements_probabilistic.py:29-31, 71-74 and the
+                        choice == cumsum + searchsorted identity
+* exp_value_observable_symbolic -- <psi|H|psi> with a dense Kronecker H
+* parameter_shift / finite_differences -- closed-form gradient and the goldens
+                        of tests/test_derivative.py:31, 84
+* matrix_fused       -- tests/test_models_circuit_fuse.py:7-22 target matrices
+Planted mistakes (a transposed permutation, a conjugated matrix) are shown to
+fail the brute-force pins.
+"""
+
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import OracleBackend
+from qibo_amd import Circuit, gates, hamiltonians, models
+
+
+# ---------------------------------------------------------------------------
+# independent brute force
+# ---------------------------------------------------------------------------
+def brute_unitary(matrix, targets, controls, nqubits):
+    """Full unitary of ``matrix`` on ``targets`` (MSB first) under ``controls``."""
+    dim = 2**nqubits
+    u = np.zeros((dim, dim), dtype=complex)
+    k = len(targets)
+    for col in range(dim):
+        bits = [(col >> (nqubits - 1 - q)) & 1 for q in range(nqubits)]
+        if any(bits[c] == 0 for c in controls):
+            u[col, col] = 1
+            continue
+        r_in = 0
+        for t in targets:
+            r_in = (r_in << 1) | bits[t]
+        for r_out in range(2**k):
+            out_bits = list(bits)
+            for j, t in enumerate(targets):
+                out_bits[t] = (r_out >> (k - 1 - j)) & 1
+            row = 0
+            for b in out_bits:
+                row = (row << 1) | b
+            u[row, col] += matrix[r_out, r_in]
+    return u
+
+
+def random_state(n, rng, dtype=complex):
+    v = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    return (v / np.linalg.norm(v)).astype(dtype)
+
+
+def rx(t):
+    return np.array([[np.cos(t / 2), -1j * np.sin(t / 2)], [-1j * np.sin(t / 2), np.cos(t / 2)]])
+
+
+def ry(t):
+    return np.array([[np.cos(t / 2), -np.sin(t / 2)], [np.sin(t / 2), np.cos(t / 2)]])
+
+
+def rz(t):
+    return np.diag([np.exp(-1j * t / 2), np.exp(1j * t / 2)])
+
+
+H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+X = np.array([[0, 1], [1, 0]])
+CNOT = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]])
+CZ = np.diag([1, 1, 1, -1])
+SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
+TOFF = np.eye(8)
+TOFF[6:, 6:] = X
+
+
+def fsim(theta, phi):
+    return np.array([[1, 0, 0, 0], [0, np.cos(theta), -1j * np.sin(theta), 0],
+                     [0, -1j * np.sin(theta), np.cos(theta), 0], [0, 0, 0, np.exp(-1j * phi)]])
+
+
+CASES = [
+    (lambda: gates.H(3), H, (3,), ()),
+    (lambda: gates.RX(0, 0.3), rx(0.3), (0,), ()),
+    (lambda: gates.RY(4, 1.1), ry(1.1), (4,), ()),
+    (lambda: gates.RZ(2, -0.7), rz(-0.7), (2,), ()),
+    (lambda: gates.CNOT(3, 1), CNOT, (3, 1), ()),
+    (lambda: gates.CZ(0, 4), CZ, (0, 4), ()),
+    (lambda: gates.SWAP(4, 1), SWAP, (4, 1), ()),
+    (lambda: gates.fSim(2, 0, 0.4, 0.9), fsim(0.4, 0.9), (2, 0), ()),
+    (lambda: gates.TOFFOLI(4, 0, 2), TOFF, (4, 0, 2), ()),
+    (lambda: gates.CU1(1, 3, 0.25), np.diag([1, 1, 1, np.exp(0.25j)]), (1, 3), ()),
+    (lambda: gates.RX(2, 0.8).controlled_by(0, 4), rx(0.8), (2,), (0, 4)),
+    (lambda: gates.SWAP(1, 3).controlled_by(0), SWAP, (1, 3), (0,)),
+    (lambda: gates.H(1).controlled_by(3), H, (1,), (3,)),
+]
+
+
+@pytest.mark.parametrize("make,mat,targets,controls", CASES)
+def test_apply_gate_matches_brute_force(make, mat, targets, controls, rng):
+    n = 5
+    psi = random_state(n, rng)
+    got = OracleBackend().apply_gate(make(), psi.copy(), n)
+    want = brute_unitary(np.asarray(mat, dtype=complex), targets, controls, n) @ psi
+    np.testing.assert_allclose(got, want, atol=1e-13, rtol=0)
+
+
+def test_unitary_gate_random_3q(rng):
+    n = 5
+    a = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    u, _ = np.linalg.qr(a)
+    psi = random_state(n, rng)
+    got = OracleBackend().apply_gate(gates.Unitary(u, 3, 0, 4), psi.copy(), n)
+    want = brute_unitary(u, (3, 0, 4), (), n) @ psi
+    np.testing.assert_allclose(got, want, atol=1e-13)
+
+
+def test_planted_mistake_is_caught(rng, monkeypatch):
+    """A transposed axis permutation or a conjugated matrix fails the pin."""
+    n = 4
+    psi = random_state(n, rng)
+    gate = gates.CNOT(3, 1)
+    want = brute_unitary(CNOT, (3, 1), (), n) @ psi
+    ob = OracleBackend()
+    fwd = OracleBackend.permutations
+
+    def wrong_perm(qubits, nqubits):
+        f, i = fwd(tuple(reversed(qubits)), nqubits)
+        return f, i
+
+    monkeypatch.setattr(OracleBackend, "permutations", staticmethod(wrong_perm))
+    assert not np.allclose(ob.apply_gate(gate, psi.copy(), n), want, atol=1e-8)
+    monkeypatch.undo()
+    orig = OracleBackend.matrix
+    monkeypatch.setattr(OracleBackend, "matrix", lambda self, g: np.conj(orig(self, g)))
+    assert not np.allclose(ob.apply_gate(gates.RX(2, 0.3), psi.copy(), n),
+                           brute_unitary(rx(0.3), (2,), (), n) @ psi, atol=1e-8)
+
+
+def test_index_conventions(golden):
+    g = golden["index_conventions"]
+    ob = OracleBackend()
+    c = Circuit(2)
+    c.add(gates.X(0))
+    s = ob.execute(c)
+    assert int(np.argmax(np.abs(s))) == g["X0_on_2_qubits_index"]
+    c.add(gates.CNOT(0, 1))
+    s = ob.execute(c)
+    assert int(np.argmax(np.abs(s))) == g["X0_then_CNOT01_index"]
+
+
+REF_MATRICES = [
+    ("H", (0,), H),
+    ("X", (0,), X),
+    ("Y", (0,), np.array([[0, -1j], [1j, 0]])),
+    ("Z", (1,), np.array([[1, 0], [0, -1]])),
+    ("S", (2,), np.array([[1, 0], [0, 1j]])),
+    ("T", (2,), np.array([[1, 0], [0, np.exp(1j * np.pi / 4.0)]])),
+    ("CNOT", (0, 1), CNOT),
+    ("CZ", (1, 3), CZ),
+    ("SWAP", (2, 4), SWAP),
+    ("FSWAP", (2, 4), np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, -1]])),
+    ("TOFFOLI", (1, 2, 3), TOFF),
+]
+
+
+@pytest.mark.parametrize("name,qubits,target", REF_MATRICES)
+def test_oracle_fixed_matrices(name, qubits, target):
+    np.testing.assert_allclose(OracleBackend().matrix(getattr(gates, name)(*qubits)), target, atol=1e-15)
+
+
+@pytest.mark.parametrize("name,target", [("RX", rx), ("RY", ry), ("RZ", rz),
+                                         ("U1", lambda x: np.diag([1, np.exp(1j * x)])),
+                                         ("CU1", lambda x: np.diag([1, 1, 1, np.exp(1j * x)]))])
+def test_oracle_rotation_matrices(name, target):
+    theta = 0.1234
+    g = getattr(gates, name)(0, 1, theta) if name == "CU1" else getattr(gates, name)(0, theta)
+    np.testing.assert_allclose(OracleBackend().matrix(g), target(theta), atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [4, 5, 6])
+def test_oracle_qft_is_dft(n, rng, golden):
+    c = models.QFT(n)
+    assert c.depth == 2 * n and c.ngates == n * n // 2 + n
+    dim = 2**n
+    dft = np.exp(2j * np.pi * np.outer(np.arange(dim), np.arange(dim)) / dim) / np.sqrt(dim)
+    for psi in (np.eye(dim)[0].astype(complex), random_state(n, rng)):
+        np.testing.assert_allclose(OracleBackend().execute(c, psi), dft @ psi, atol=1e-12)
+
+
+def test_oracle_probabilities_brute(rng):
+    n = 5
+    psi = random_state(n, rng)
+    ob = OracleBackend()
+    full = ob.calculate_probabilities(psi, list(range(n)), n)
+    np.testing.assert_array_equal(full, np.abs(psi) ** 2)
+    qubits = [3, 0]
+    got = ob.calculate_probabilities(psi, qubits, n)
+    want = np.zeros(4)
+    for i in range(2**n):
+        b3, b0 = (i >> (n - 1 - 3)) & 1, (i >> (n - 1)) & 1
+        want[(b3 << 1) | b0] += abs(psi[i]) ** 2
+    np.testing.assert_allclose(got, want, atol=1e-15)
+
+
+def test_choice_equals_cumsum_searchsorted(rng):
+    """numpy's choice(p) is cumsum / cdf[-1] + searchsorted(side='right')."""
+    for n in (4, 4096, 2**16):
+        p = rng.random(n) ** 3
+        p /= p.sum()
+        np.random.seed(7)
+        a = np.random.choice(np.arange(n), size=3000, p=p)
+        np.random.seed(7)
+        cdf = p.cumsum()
+        cdf /= cdf[-1]
+        b = cdf.searchsorted(np.random.random_sample(3000), side="right")
+        np.testing.assert_array_equal(a, b)
+
+
+def test_cumsum_is_sequential(rng):
+    p = rng.random(5000)
+    seq = np.empty_like(p)
+    s = 0.0
+    for i, v in enumerate(p):
+        s = s + v
+        seq[i] = s
+    np.testing.assert_array_equal(np.cumsum(p), seq)
+
+
+def _fma_exact(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _np_abs_formula(z):
+    re, im = abs(float(z.real)), abs(float(z.imag))
+    lg, sm = max(re, im), min(re, im)
+    r = 0.0 if lg == 0 else sm / lg
+    return math.sqrt(_fma_exact(r, r, 1.0)) * lg
+
+
+def test_numpy_complex_abs_formula(rng):
+    """np.abs(z)**2 == (l*sqrt(fma(r, r, 1)))**2 bitwise on this host (c128)."""
+    z = (rng.normal(size=3000) + 1j * rng.normal(size=3000)) * 10.0 ** rng.integers(-8, 3, size=3000)
+    z[:3] = [0, 1j, -2.5]
+    want = np.array([_np_abs_formula(v) ** 2 for v in z])
+    np.testing.assert_array_equal(np.abs(z) ** 2, want)
+
+
+def test_numpy_complex64_abs_formula(rng):
+    z = (rng.normal(size=2000) + 1j * rng.normal(size=2000)).astype(np.complex64)
+    got = np.abs(z) ** 2
+    want = []
+    for v in z:
+        re, im = np.float32(abs(v.real)), np.float32(abs(v.imag))
+        lg, sm = max(re, im), min(re, im)
+        r = np.float32(0) if lg == 0 else np.float32(sm / lg)
+        f = np.float32(float(Fraction(float(r)) * Fraction(float(r)) + 1))
+        h = np.float32(np.sqrt(f) * lg)
+        want.append(np.float32(h * h))
+    np.testing.assert_array_equal(got, np.array(want, dtype=np.float32))
+
+
+def _golden_freq(d):
+    return {int(k): v for k, v in d.items()}
+
+
+def test_seeded_frequencies_golden(golden):
+    ob = OracleBackend()
+    g = golden["probabilistic_measurement"]
+    c = Circuit(4)
+    c.add([gates.H(0), gates.H(1)])
+    psi = ob.execute(c)
+    probs = ob.calculate_probabilities(psi, [0, 1], 4)
+    np.random.seed(g["seed"])
+    assert dict(ob.sample_frequencies(probs, g["nshots"])) == _golden_freq(g["frequencies"])
+    np.random.seed(g["seed"])
+    samples = ob.sample_shots(probs, g["nshots"])
+    assert dict(ob.calculate_frequencies(samples)) == _golden_freq(g["frequencies"])
+
+    g = golden["unbalanced_probabilistic_measurement"]
+    psi = np.array(g["initial_state_unnormalised"]) / g["norm"]
+    probs = ob.calculate_probabilities(psi.astype(complex), [0, 1], 2)
+    np.random.seed(g["seed"])
+    assert dict(ob.sample_frequencies(probs, g["nshots"])) == _golden_freq(g["frequencies"])
+    np.random.seed(g["seed"])
+    assert dict(ob.calculate_frequencies(ob.sample_shots(probs, g["nshots"]))) == _golden_freq(
+        g["frequencies"])
+
+
+def test_samples_to_binary_roundtrip(rng):
+    s = rng.integers(0, 2**7, size=100)
+    b = OracleBackend.samples_to_binary(s, 7)
+    assert b.shape == (100, 7)
+    np.testing.assert_array_equal(OracleBackend.samples_to_decimal(b, 7), s)
+    assert list(b[0]) == [int(x) for x in format(int(s[0]), "07b")]
+
+
+def _dense_paulis(terms, n):
+    P = {"X": X.astype(complex), "Y": np.array([[0, -1j], [1j, 0]]), "Z": np.diag([1.0, -1.0]).astype(complex)}
+    h = np.zeros((2**n, 2**n), dtype=complex)
+    for c, s, qs in terms:
+        ops = [np.eye(2)] * n
+        for f, q in zip(s, qs):
+            ops[q] = P[f]
+        m = ops[0]
+        for o in ops[1:]:
+            m = np.kron(m, o)
+        h += c * m
+    return h
+
+
+def test_oracle_expectation_brute(rng):
+    n = 6
+    ham = hamiltonians.TFIM(n, h=1.0, closed_boundary=False)
+    terms = ham.terms + [(0.37, "XYZ", (1, 3, 5)), (-0.2, "YY", (0, 2))]
+    psi = random_state(n, rng)
+    coeffs = [t[0] for t in terms]
+    strings = [t[1] for t in terms]
+    qubits = [t[2] for t in terms]
+    got = OracleBackend().exp_value_observable_symbolic(psi, strings, qubits, coeffs, n)
+    want = np.real(np.conj(psi) @ _dense_paulis(terms, n) @ psi)
+    assert abs(got - want) < 1e-12
+
+
+def _dummy_circuit(params):
+    c = Circuit(1)
+    c.add(gates.RX(0, params[0]))
+    c.add(gates.RY(0, params[1]))
+    c.add(gates.RZ(0, params[2]))
+    c.add(gates.M(0))
+    return c
+
+
+def test_oracle_gradients_golden(golden):
+    g = golden["derivative_dummy_circuit"]
+    ob = OracleBackend()
+    terms = ([1.0], ["Z"], [(0,)])
+    a, b, c = g["params"]
+    closed = [-np.sin(a) * np.cos(b), -np.cos(a) * np.sin(b), 0.0]
+    circ = _dummy_circuit(g["params"])
+    for k in range(3):
+        psr = ob.parameter_shift(circ, terms, k)
+        fd = ob.finite_differences(circ, terms, k, step_size=1e-6)
+        assert abs(psr - g["grads_scale_1"][k]) < 1e-8
+        assert abs(psr - closed[k]) < 1e-12
+        assert abs(fd - closed[k]) < 1e-6
+    circ = _dummy_circuit(g["params_scale_half"])
+    for k in range(3):
+        psr = ob.parameter_shift(circ, terms, k, scale_factor=0.5)
+        assert abs(psr - g["grads_scale_half_times_half"][k]) < 1e-8
+
+
+@pytest.mark.parametrize("nqubits", [2, 3])
+def test_oracle_matrix_fused(nqubits):
+    """tests/test_models_circuit_fuse.py:7-22."""
+    fg = gates.FusedGate(0, 1)
+    fg.append(gates.H(0))
+    fg.append(gates.H(1))
+    fg.append(gates.CZ(0, 1))
+    target = CZ @ np.kron(H, H)
+    if nqubits > 2:
+        fg.append(gates.TOFFOLI(0, 1, 2))
+        target = TOFF @ np.kron(target, np.eye(2))
+    np.testing.assert_allclose(OracleBackend().matrix_fused(fg), target, atol=1e-15)
+
+
+def test_oracle_fused_equals_unfused(rng):
+    n = 5
+    c = models.random_layered_circuit(n, 3, seed=3, measure=False)
+    c.add(gates.RX(2, 0.4).controlled_by(0))
+    psi = random_state(n, rng)
+    ob = OracleBackend()
+    want = ob.execute(c, psi)
+    for k in (2, 3, 4, 5):
+        np.testing.assert_allclose(ob.execute(c.fuse(max_qubits=k), psi), want, atol=1e-12)
