@@ -4,7 +4,10 @@
 // This translation unit is compiled with -ffp-contract=off: every floating
 // point operation below must round exactly like numpy's, so no multiply may be
 // fused into a following add behind our back. Fused multiply-adds appear only
-// where numpy itself uses one (its complex abs).
+// where numpy itself uses one (its complex abs). Division and square root are
+// the compiler's correctly rounded IEEE operations (plain `/` and
+// __builtin_sqrt[f]); HIP's __fsqrt_rn maps to the approximate native square
+// root unless OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used.
 //
 // Exact cumulative sum. numpy's cumsum (inside np.random.choice) is the
 // sequential chain s_i = fl(s_{i-1} + p_i). A tree scan rounds differently and
@@ -46,9 +49,9 @@ __device__ __forceinline__ double np_abs2(c128 z) {
   if (im != im) re = im;
   const double l = fmax(re, im);
   const double sm = fmin(im, re);
-  const double r = (l == 0.0 || sm == inf) ? 0.0 : __ddiv_rn(sm, l);
-  const double h = __dmul_rn(__dsqrt_rn(__fma_rn(r, r, 1.0)), l);
-  return __dmul_rn(h, h);
+  const double r = (l == 0.0 || sm == inf) ? 0.0 : sm / l;
+  const double h = __builtin_sqrt(__builtin_fma(r, r, 1.0)) * l;
+  return h * h;
 }
 
 __device__ __forceinline__ float np_abs2(c64 z) {
@@ -60,9 +63,9 @@ __device__ __forceinline__ float np_abs2(c64 z) {
   if (im != im) re = im;
   const float l = fmaxf(re, im);
   const float sm = fminf(im, re);
-  const float r = (l == 0.0f || sm == inf) ? 0.0f : __fdiv_rn(sm, l);
-  const float h = __fmul_rn(__fsqrt_rn(__fmaf_rn(r, r, 1.0f)), l);
-  return __fmul_rn(h, h);
+  const float r = (l == 0.0f || sm == inf) ? 0.0f : sm / l;
+  const float h = __builtin_sqrtf(__builtin_fmaf(r, r, 1.0f)) * l;
+  return h * h;
 }
 
 constexpr int kT = 256;
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(kT) void k_search(const double* __restrict__ cdf, i
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     const int64_t mid = lo + ((hi - lo) >> 1);
-    if (__ddiv_rn(cdf[mid], total) <= v)
+    if (cdf[mid] / total <= v)
       lo = mid + 1;
     else
       hi = mid;
