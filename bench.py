@@ -129,6 +129,23 @@ def main():
         samples = result.samples()
         return result, samples
 
+    # copy calibration: a plain device-to-device copy of one state-sized buffer
+    # (torch's copy kernel), the reference point for the HBM counter scaling
+    src = torch.empty(2**n, dtype=torch.complex128, device=backend.torch_device)
+    dst = torch.empty_like(src)
+    src.zero_()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    copy_ms = []
+    for _ in range(3):
+        ev0.record()
+        dst.copy_(src)
+        ev1.record()
+        ev1.synchronize()
+        copy_ms.append(ev0.elapsed_time(ev1))
+    copy_bytes = 2 * src.numel() * itemsize
+    del src, dst
+    torch.cuda.empty_cache()
+
     for _ in range(args.warmup):
         r, s = step()
         del r, s
@@ -183,6 +200,11 @@ def main():
             "traffic": None,
             "launches_timed": len(kernel_ms),
             "avg_launch_ms": avg_dense_ms,
+        },
+        "copy_calibration": {
+            "bytes": copy_bytes,
+            "best_ms": min(copy_ms),
+            "GBs": copy_bytes / (min(copy_ms) * 1e-3) / 1e9,
         },
     }
     if not args.no_cpu:
