@@ -91,6 +91,23 @@ int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const i
 int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, int ntargets,
                      const void* matrix_dev, void* stream);
 
+/* Many gates in one pass over the state: the gate loop of
+ * Backend._execute_circuit (abstract.py:3321-3322, each step an apply_gate,
+ * abstract.py:2322-2361) for a run of consecutive gates that the host has
+ * grouped so that their non-diagonal targets fit in 12 "tile bits".
+ * `tile_bits` = 12 ascending amplitude-index bit positions (not qubit
+ * numbers); `program_dev` = DEVICE array of `ninstr` instruction records of
+ * qsim_blocked_record_bytes(dtype) bytes each (layout: u32 op, u32 arg,
+ * u64 control mask, u32 nt, u8 bits[8], u32 pad, then 64 complex entries;
+ * ops: 0 diagonal on up to 5 global bits, 1 one-qubit gate on a register bit
+ * of the current mapping, 2 change of register mapping, 3 dense 2^k block on
+ * k <= 3 local tile bits). The result is bitwise that of applying the same
+ * gates one by one with qsim_apply_dense / qsim_apply_diag / qsim_apply_swap.
+ * Requires nqubits >= 12. */
+size_t qsim_blocked_record_bytes(int dtype);
+int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
+                       int ninstr, void* stream);
+
 /* Probabilities of measuring `qubits` (m = nmeasured, in the given order, the
  * first being the most significant bit of the output index):
  *   p = abs(state)**2 with numpy's complex abs, summed over unmeasured qubits.

@@ -24,6 +24,7 @@ SOURCES = {
     "fused.hip": [],
     "measure.hip": ["-ffp-contract=off"],
     "expval.hip": [],
+    "blocked.hip": [],
 }
 
 COMMON = [

@@ -44,6 +44,8 @@ SIGNATURES = {
     "qsim_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "qsim_apply_pauli_sum": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "qsim_inner_matrix": (_i, [_vp, _vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp, _vp, _sz, _vp]),
+    "qsim_blocked_record_bytes": (_sz, [_i]),
+    "qsim_apply_blocked": (_i, [_vp, _i, _i, _c_int_p, _vp, _i, _vp]),
 }
 
 _lock = threading.Lock()

@@ -36,7 +36,9 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import blocking
 from . import config
+from . import gates as gates_mod
 from .config import SHOT_BATCH_SIZE, log, raise_error
 from .matrices import GateMatrices
 
@@ -220,6 +222,11 @@ class MI355XBackend(GatePlanner):
         self.set_dtype(dtype)
         self._ws = {}  # workspace cache: (device, purpose, stream) -> tensor
         self._timer = None  # (selector, [(start, end) events]) while kernel timing is on
+        self.blocking = True  # group gates into blocked passes in run_queue
+        self.block_stats = []  # gates per blocked pass (diagnostics)
+        rec = self.lib.qsim_blocked_record_bytes(_lib.C128)
+        if rec != blocking.HEADER_BYTES + blocking.NMAT * 16:
+            raise ImportError(f"qibo_amd: blocked-pass record size {rec} does not match the host layout")
 
     # -- qibo Backend attributes -----------------------------------------
     @property
@@ -360,6 +367,7 @@ class MI355XBackend(GatePlanner):
         "dense1": lambda op: op.kind == "dense" and len(op.targets) == 1 and not op.controls,
         "diag1": lambda op: op.kind == "diag" and len(op.targets) == 1 and not op.controls,
         "fused": lambda op: op.kind == "fused",
+        "block": lambda op: op.kind == "block",
         "any": lambda op: op.kind != "noop",
     }
 
@@ -465,13 +473,74 @@ class MI355XBackend(GatePlanner):
 
         nqubits = circuit.nqubits
         state = self.zero_state(nqubits) if initial_state is None else initial_state
-        for gate in circuit.queue:
-            state = gate.apply(self, state, nqubits)
+        state = self.run_queue(circuit.queue, state, nqubits)
         if circuit.measurements:
             circuit._final_state = CircuitResult(state, circuit.measurements, backend=self, nshots=nshots)
         else:
             circuit._final_state = QuantumState(state, backend=self)
         return circuit._final_state
+
+    def run_queue(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        """Apply the gates of ``queue`` in order (the loop of abstract.py:3321-3322).
+
+        With ``self.blocking`` (the default) consecutive gates are grouped into
+        blocked passes (qibo_amd.blocking) that read and write the state once
+        each; the result is bitwise that of applying the gates one by one.
+        Measurement gates only register the backend (non-collapsing ``M`` is a
+        no-op on the state); other special gates run in order between passes.
+        """
+        from .measurements_gate import M
+        from .special import FusedGate
+
+        if not self.blocking or nqubits < blocking.TILE_BITS:
+            for gate in queue:
+                state = gate.apply(self, state, nqubits)
+            return state
+        segment = []
+        for gate in queue:
+            if isinstance(gate, M) and not gate.collapse:
+                gate.result.backend = self
+                continue
+            if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
+                state = self._run_blocked(segment, state, nqubits)
+                segment = []
+                state = gate.apply(self, state, nqubits)
+                continue
+            segment.append(self.plan_gate(gate))
+        return self._run_blocked(segment, state, nqubits)
+
+    def _run_blocked(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        if not ops:
+            return state
+        code = self._dtype_code(state)
+        cdtype = np.complex128 if code == _lib.C128 else np.complex64
+        low = 3 if code == _lib.C128 else 4
+        plan, program = blocking.compile_circuit(ops, nqubits, cdtype, low)
+        prog = None
+        if program:
+            host = torch.frombuffer(bytearray(program), dtype=torch.uint8)
+            prog = host.to(state.device, non_blocking=False)
+        rec = blocking.HEADER_BYTES + blocking.NMAT * np.dtype(cdtype).itemsize
+        stream = self._stream(state)
+        for entry in plan:
+            if entry[0] == "single":
+                self.apply_op(entry[1], state, nqubits)
+                continue
+            _, tile_bits, first, count, ngates = entry
+            timer = self._timer
+            if timer is not None and timer[0](_BLOCK_OP):
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(torch.cuda.current_stream(state.device))
+            rc = self.lib.qsim_apply_blocked(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits),
+                                             prog.data_ptr() + first * rec, count, stream)
+            _lib.check(rc, self.lib)
+            if timer is not None and timer[0](_BLOCK_OP):
+                ev[1].record(torch.cuda.current_stream(state.device))
+                timer[1].append(ev)
+            self.block_stats.append(ngates)
+        if prog is not None:
+            _keep_alive(prog, state)
+        return state
 
     def execute_circuits(self, circuits, initial_states=None, nshots=1000, processes=None):
         if initial_states is None:
@@ -647,6 +716,7 @@ class MI355XBackend(GatePlanner):
 
 
 _MAX_QUBITS = 36
+_BLOCK_OP = _Op("block", (), (), None)
 
 
 def ctypes_stream(stream) -> int:

@@ -8,8 +8,10 @@
 // target position: for a high target the two halves of a pair are two
 // contiguous streams, for a low target a lane's pair shares a 128-byte line
 // with its neighbours' pairs and both loads of the line are issued back to
-// back, so each line crosses HBM once. Each thread handles kItems bases with
-// all loads issued before the arithmetic, to keep enough bytes in flight.
+// back, so each line crosses HBM once; targets inside one 128-byte line go
+// through an LDS-staged tile instead. Each lane handles one base index with
+// streaming (nontemporal) loads and stores, and workgroups are remapped so
+// that each XCD walks a contiguous part of the state.
 //
 // Reference semantics: qibo's Backend.apply_gate (abstract.py:2322-2361) and
 // _apply_gate_controlled_by (abstract.py:3176-3197); the matrix row/column
@@ -77,17 +79,26 @@ static inline uint64_t qmask(int nqubits, int q) { return uint64_t(1) << (nqubit
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_zero_state(cplx<R>* __restrict__ psi, uint64_t n) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t i = xcd_block() * 256 + threadIdx.x;
   if (i < n) {
     cplx<R> v;
     v.re = (i == 0) ? R(1) : R(0);
     v.im = R(0);
-    psi[i] = v;
+    st_nt(psi, i, v);
   }
 }
 
+constexpr int kThreads = 256;
+
+// Amplitudes per 128-byte line: a target bit below log2 of this pairs
+// amplitudes inside one line.
+template <typename R>
+constexpr int line_bits() {
+  return sizeof(R) == 8 ? 3 : 4;
+}
+
 // ---------------------------------------------------------------------------
-// dense 2^NT x 2^NT on targets, under controls
+// dense 2^NT x 2^NT on targets, under controls: one base index per lane
 // ---------------------------------------------------------------------------
 template <typename R, int NT>
 struct DenseArgs {
@@ -96,71 +107,102 @@ struct DenseArgs {
   uint64_t off[1 << NT];       // offset of matrix index r from the base
   cplx<R> m[(1 << NT) * (1 << NT)];
   int nfixed;                  // controls + targets
+  int tbit;                    // bit of the single target (NT == 1, LDS variant)
   uint8_t pos[kMaxFixed];      // their bit positions, ascending
 };
 
-constexpr int kThreads = 256;
+template <typename R, int D>
+__device__ __forceinline__ cplx<R> row_times(const cplx<R>* m, const cplx<R>* v) {
+  cplx<R> acc = cmul(m[0], v[0]);
+#pragma clang loop unroll(full)
+  for (int c = 1; c < D; ++c) acc = cfma(m[c], v[c], acc);
+  return acc;
+}
 
-template <typename R, int NT, int kItems>
+template <typename R, int NT>
 __global__ __launch_bounds__(kThreads) void k_dense(const DenseArgs<R, NT> a, cplx<R>* __restrict__ psi) {
   constexpr int D = 1 << NT;
-  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
-  uint64_t base[kItems];
-  cplx<R> v[kItems][D];
+  const uint64_t j = xcd_block() * kThreads + threadIdx.x;
+  if (j >= a.nwork) return;
+  const uint64_t base = insert_zeros(j, a.pos, a.nfixed) | a.cmask;
+  cplx<R> v[D];
 #pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t j = first + uint64_t(it) * kThreads;
-    base[it] = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
+  for (int r = 0; r < D; ++r) v[r] = ld_nt(psi, base | a.off[r]);
 #pragma clang loop unroll(full)
-    for (int r = 0; r < D; ++r) v[it][r] = ld(psi, base[it] | a.off[r]);
-  }
+  for (int r = 0; r < D; ++r) st_nt(psi, base | a.off[r], row_times<R, D>(a.m + r * D, v));
+}
+
+// One-qubit gate whose target pairs amplitudes inside a 128-byte line: the
+// block stages a contiguous tile through LDS with coalesced streaming loads,
+// updates the pairs there (same arithmetic as k_dense) and streams it back.
+constexpr int kLdsTileE = 8;                       // amplitudes per lane
+constexpr int kLdsTile = kThreads * kLdsTileE;     // 2048 amplitudes per tile
+constexpr int kLdsTileBits = 11;
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_dense1_lds(const DenseArgs<R, 1> a, cplx<R>* __restrict__ psi) {
+  __shared__ cplx<R> t[kLdsTile];
+  const uint64_t base = xcd_block() * kLdsTile;
 #pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t j = first + uint64_t(it) * kThreads;
-    if (j >= a.nwork) continue;
+  for (int e = 0; e < kLdsTileE; ++e) t[e * kThreads + threadIdx.x] = ld_nt(psi, base + e * kThreads + threadIdx.x);
+  __syncthreads();
+  const uint32_t lowmask = (1u << a.tbit) - 1;
 #pragma clang loop unroll(full)
-    for (int r = 0; r < D; ++r) {
-      cplx<R> acc = cmul(a.m[r * D], v[it][0]);
-#pragma clang loop unroll(full)
-      for (int c = 1; c < D; ++c) acc = cfma(a.m[r * D + c], v[it][c], acc);
-      st(psi, base[it] | a.off[r], acc);
+  for (int e = 0; e < kLdsTileE / 2; ++e) {
+    const uint32_t jj = e * kThreads + threadIdx.x;
+    const uint32_t low = jj & lowmask;
+    const uint32_t i0 = low | ((jj ^ low) << 1), i1 = i0 | (1u << a.tbit);
+    if (((base + i0) & a.cmask) == a.cmask) {
+      cplx<R> v[2] = {t[i0], t[i1]};
+      t[i0] = row_times<R, 2>(a.m, v);
+      t[i1] = row_times<R, 2>(a.m + 2, v);
     }
   }
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int e = 0; e < kLdsTileE; ++e) st_nt(psi, base + e * kThreads + threadIdx.x, t[e * kThreads + threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
-// diagonal on targets, under controls: one amplitude per work item
+// diagonal on targets, under controls
 // ---------------------------------------------------------------------------
 template <typename R>
 struct DiagArgs {
-  uint64_t nwork;          // 2^(n - ncontrols)
+  uint64_t nwork;          // amplitudes walked
   uint64_t cmask;
   cplx<R> d[32];
-  int nfixed;              // controls only
+  int nfixed;              // controls only (subspace walk)
   int nt;                  // number of targets (0..5)
   uint8_t tbit[5];         // bit positions of targets, target 0 = MSB of diag index
   uint8_t pos[kMaxFixed];
 };
 
-template <typename R, int kItems>
+template <typename R>
+__device__ __forceinline__ int diag_index(const DiagArgs<R>& a, uint64_t idx) {
+  int r = 0;
+  for (int t = 0; t < a.nt; ++t) r = (r << 1) | int((idx >> a.tbit[t]) & 1);
+  return r;
+}
+
+// Walks only the controlled subspace: used when every control bit is at or
+// above the line size, so the walked amplitudes form whole lines.
+template <typename R>
 __global__ __launch_bounds__(kThreads) void k_diag(const DiagArgs<R> a, cplx<R>* __restrict__ psi) {
-  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
-  uint64_t idx[kItems];
-  cplx<R> v[kItems];
-#pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t j = first + uint64_t(it) * kThreads;
-    idx[it] = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
-    v[it] = ld(psi, idx[it]);
-  }
-#pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t j = first + uint64_t(it) * kThreads;
-    if (j >= a.nwork) continue;
-    int r = 0;
-    for (int t = 0; t < a.nt; ++t) r = (r << 1) | int((idx[it] >> a.tbit[t]) & 1);
-    st(psi, idx[it], cmul(a.d[r], v[it]));
-  }
+  const uint64_t j = xcd_block() * kThreads + threadIdx.x;
+  if (j >= a.nwork) return;
+  const uint64_t idx = insert_zeros(j, a.pos, a.nfixed) | a.cmask;
+  st_nt(psi, idx, cmul(a.d[diag_index(a, idx)], ld_nt(psi, idx)));
+}
+
+// Walks every amplitude contiguously and updates the controlled ones: used
+// when a control bit lies inside a line, where a subspace walk would fetch
+// each line several times through strided accesses.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_diag_full(const DiagArgs<R> a, cplx<R>* __restrict__ psi) {
+  const uint64_t idx = xcd_block() * kThreads + threadIdx.x;
+  if (idx >= a.nwork) return;
+  const cplx<R> v = ld_nt(psi, idx);
+  if ((idx & a.cmask) == a.cmask) st_nt(psi, idx, cmul(a.d[diag_index(a, idx)], v));
 }
 
 // ---------------------------------------------------------------------------
@@ -174,34 +216,19 @@ struct SwapArgs {
   uint8_t pos[kMaxFixed];
 };
 
-template <typename R, int kItems>
+template <typename R>
 __global__ __launch_bounds__(kThreads) void k_swap(const SwapArgs a, cplx<R>* __restrict__ psi) {
-  const uint64_t first = uint64_t(blockIdx.x) * (kItems * kThreads) + threadIdx.x;
-  uint64_t ia[kItems], ib[kItems];
-  cplx<R> x[kItems], y[kItems];
-#pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    const uint64_t j = first + uint64_t(it) * kThreads;
-    const uint64_t base = insert_zeros(j < a.nwork ? j : 0, a.pos, a.nfixed) | a.cmask;
-    ia[it] = base | a.m0;
-    ib[it] = base | a.m1;
-    x[it] = ld(psi, ia[it]);
-    y[it] = ld(psi, ib[it]);
-  }
-#pragma clang loop unroll(full)
-  for (int it = 0; it < kItems; ++it) {
-    if (first + uint64_t(it) * kThreads < a.nwork) {
-      st(psi, ia[it], y[it]);
-      st(psi, ib[it], x[it]);
-    }
-  }
+  const uint64_t j = xcd_block() * kThreads + threadIdx.x;
+  if (j >= a.nwork) return;
+  const uint64_t base = insert_zeros(j, a.pos, a.nfixed) | a.cmask;
+  const cplx<R> x = ld_nt(psi, base | a.m0), y = ld_nt(psi, base | a.m1);
+  st_nt(psi, base | a.m0, y);
+  st_nt(psi, base | a.m1, x);
 }
 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-constexpr int kItemsDefault = 4;
-
 template <typename R, int NT>
 static int launch_dense(void* state, int n, const int* targets, const int* controls, int nc,
                         const void* matrix, hipStream_t s) {
@@ -219,9 +246,17 @@ static int launch_dense(void* state, int n, const int* targets, const int* contr
     a.off[r] = off;
   }
   std::memcpy(a.m, matrix, sizeof(a.m));
-  const int items = kItemsDefault;
-  const int blocks = ceil_div_i(a.nwork, uint64_t(items) * kThreads);
-  hipLaunchKernelGGL((k_dense<R, NT, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
+  a.tbit = n - 1 - targets[0];
+  if constexpr (NT == 1) {
+    if (a.tbit < line_bits<R>() && n >= kLdsTileBits) {
+      const int blocks = int((uint64_t(1) << n) / kLdsTile);
+      hipLaunchKernelGGL((k_dense1_lds<R>), dim3(blocks), dim3(kThreads), 0, s, a,
+                         reinterpret_cast<cplx<R>*>(state));
+      QSIM_CHECK_LAUNCH();
+      return QSIM_OK;
+    }
+  }
+  hipLaunchKernelGGL((k_dense<R, NT>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                      reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -233,15 +268,21 @@ static int launch_diag(void* state, int n, const int* targets, int nt, const int
   DiagArgs<R> a;
   std::memset(&a, 0, sizeof(a));
   a.nfixed = sorted_positions(n, controls, nc, nullptr, 0, a.pos);
-  a.nwork = uint64_t(1) << (n - a.nfixed);
   a.cmask = 0;
   for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
   a.nt = nt;
   for (int t = 0; t < nt; ++t) a.tbit[t] = uint8_t(n - 1 - targets[t]);
   std::memcpy(a.d, diag, sizeof(cplx<R>) * (size_t(1) << nt));
-  const int blocks = ceil_div_i(a.nwork, uint64_t(kItemsDefault) * kThreads);
-  hipLaunchKernelGGL((k_diag<R, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
-                     reinterpret_cast<cplx<R>*>(state));
+  const bool control_in_line = a.nfixed > 0 && a.pos[0] < line_bits<R>();
+  if (control_in_line) {
+    a.nwork = uint64_t(1) << n;
+    hipLaunchKernelGGL((k_diag_full<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+                       reinterpret_cast<cplx<R>*>(state));
+  } else {
+    a.nwork = uint64_t(1) << (n - a.nfixed);
+    hipLaunchKernelGGL((k_diag<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+                       reinterpret_cast<cplx<R>*>(state));
+  }
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -257,8 +298,7 @@ static int launch_swap(void* state, int n, int q0, int q1, const int* controls, 
   for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
   a.m0 = qmask(n, q1);  // |..q0=0..q1=1..>
   a.m1 = qmask(n, q0);  // |..q0=1..q1=0..>
-  const int blocks = ceil_div_i(a.nwork, uint64_t(kItemsDefault) * kThreads);
-  hipLaunchKernelGGL((k_swap<R, kItemsDefault>), dim3(blocks), dim3(kThreads), 0, s, a,
+  hipLaunchKernelGGL((k_swap<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                      reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -280,12 +320,12 @@ int qsim_zero_state(void* state, int nqubits, int dtype, void* stream) {
   QSIM_CHECK_ARG(nqubits >= 1 && nqubits <= QSIM_MAX_QUBITS, "qsim_zero_state: nqubits out of range");
   QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_zero_state: bad dtype");
   const uint64_t n = uint64_t(1) << nqubits;
-  const int blocks = ceil_div_i(n, kThreads);
+  const int blocks = ceil_div_i(n, 256);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128)
-    hipLaunchKernelGGL(k_zero_state<double>, dim3(blocks), dim3(kThreads), 0, s, (c128*)state, n);
+    hipLaunchKernelGGL(k_zero_state<double>, dim3(blocks), dim3(256), 0, s, (c128*)state, n);
   else
-    hipLaunchKernelGGL(k_zero_state<float>, dim3(blocks), dim3(kThreads), 0, s, (c64*)state, n);
+    hipLaunchKernelGGL(k_zero_state<float>, dim3(blocks), dim3(256), 0, s, (c64*)state, n);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }

