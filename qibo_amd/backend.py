@@ -492,10 +492,12 @@ class MI355XBackend(GatePlanner):
         from .measurements_gate import M
         from .special import FusedGate
 
-        if not self.blocking or nqubits < blocking.TILE_BITS:
+        if not self.blocking:
             for gate in queue:
                 state = gate.apply(self, state, nqubits)
             return state
+        if nqubits < blocking.TILE_BITS:
+            return self._run_small(queue, state, nqubits)
         segment = []
         for gate in queue:
             if isinstance(gate, M) and not gate.collapse:
@@ -509,6 +511,37 @@ class MI355XBackend(GatePlanner):
             segment.append(self.plan_gate(gate))
         return self._run_blocked(segment, state, nqubits)
 
+    def _run_small(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        """Circuits on fewer than 12 qubits: run as one blocked pass on a
+        12-qubit buffer whose extra (most significant) qubits stay |0>.
+
+        The first 2^n amplitudes of the padded state are the n-qubit state, and
+        qubit q becomes qubit q + p of the padded register, so the whole circuit
+        is a single kernel launch. Same arithmetic as the per-gate kernels."""
+        from .measurements_gate import M
+        from .special import FusedGate
+
+        pad = blocking.TILE_BITS - nqubits
+        ops = []
+        for gate in queue:
+            if isinstance(gate, M) and not gate.collapse:
+                gate.result.backend = self
+                continue
+            if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
+                for g in queue:  # callbacks need the real state between gates
+                    state = g.apply(self, state, nqubits)
+                return state
+            op = self.plan_gate(gate)
+            ops.append(_Op(op.kind, tuple(q + pad for q in op.targets),
+                           tuple(q + pad for q in op.controls), op.data))
+        if not ops:
+            return state
+        big = torch.zeros(2**blocking.TILE_BITS, dtype=state.dtype, device=state.device)
+        big[: state.numel()].copy_(state)
+        big = self._run_blocked(ops, big, blocking.TILE_BITS)
+        state.copy_(big[: state.numel()])
+        return state
+
     def _run_blocked(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
         if not ops:
             return state
@@ -518,8 +551,8 @@ class MI355XBackend(GatePlanner):
         plan, program = blocking.compile_circuit(ops, nqubits, cdtype, low)
         prog = None
         if program:
-            host = torch.frombuffer(bytearray(program), dtype=torch.uint8)
-            prog = host.to(state.device, non_blocking=False)
+            host = torch.frombuffer(bytearray(program), dtype=torch.uint8).pin_memory()
+            prog = host.to(state.device, non_blocking=True)
         rec = blocking.HEADER_BYTES + blocking.NMAT * np.dtype(cdtype).itemsize
         stream = self._stream(state)
         for entry in plan:
@@ -608,7 +641,8 @@ class MI355XBackend(GatePlanner):
         out = torch.empty(k, dtype=torch.int64, device=cdf.device)
         if k == 0:
             return out
-        u = torch.as_tensor(np.ascontiguousarray(uniforms, dtype=np.float64), device=cdf.device)
+        u = torch.from_numpy(np.ascontiguousarray(uniforms, dtype=np.float64)).pin_memory()
+        u = u.to(cdf.device, non_blocking=True)
         rc = self.lib.qsim_search_sorted(cdf.data_ptr(), cdf.numel(), u.data_ptr(), k, out.data_ptr(),
                                          self._stream(cdf))
         _lib.check(rc, self.lib)
@@ -704,6 +738,35 @@ class MI355XBackend(GatePlanner):
         rc = self.lib.qsim_expval_pauli(state.data_ptr(), nqubits, self._dtype_code(state), nterms,
                                         xm.ctypes.data, zm.ctypes.data, ny.ctypes.data, c.ctypes.data,
                                         out.data_ptr(), ws.data_ptr(), wsb, self._stream(state))
+        _lib.check(rc, self.lib)
+        return out[0]
+
+    def apply_pauli_sum(self, state: torch.Tensor, nqubits: int, xm, zm, ny, coeffs) -> torch.Tensor:
+        """New state ``sum_t c_t P_t |psi>`` (the Hamiltonian-vector product)."""
+        out = torch.empty_like(state)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        xm = np.ascontiguousarray(xm, dtype=np.uint64)
+        zm = np.ascontiguousarray(zm, dtype=np.uint64)
+        ny = np.ascontiguousarray(ny, dtype=np.int32)
+        rc = self.lib.qsim_apply_pauli_sum(state.data_ptr(), nqubits, self._dtype_code(state), len(c),
+                                           xm.ctypes.data, zm.ctypes.data, ny.ctypes.data, c.ctypes.data,
+                                           out.data_ptr(), self._stream(state))
+        _lib.check(rc, self.lib)
+        return out
+
+    def inner_matrix(self, bra: torch.Tensor, ket: torch.Tensor, nqubits: int, targets, controls,
+                     matrix) -> torch.Tensor:
+        """Device float64 scalar ``Re <bra| M_targets |ket>`` over the controlled subspace."""
+        code = self._dtype_code(ket)
+        npdt = np.complex128 if code == _lib.C128 else np.complex64
+        m = np.ascontiguousarray(matrix, dtype=npdt)
+        out = torch.empty(1, dtype=torch.float64, device=ket.device)
+        wsb = self.lib.qsim_expval_workspace(nqubits, 1)
+        ws = self._workspace(wsb, "inner", ket.device)
+        rc = self.lib.qsim_inner_matrix(bra.data_ptr(), ket.data_ptr(), nqubits, code,
+                                        _lib.int_array(targets), len(targets), _lib.int_array(controls),
+                                        len(controls), m.ctypes.data, out.data_ptr(), ws.data_ptr(), wsb,
+                                        self._stream(ket))
         _lib.check(rc, self.lib)
         return out[0]
 

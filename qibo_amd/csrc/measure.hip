@@ -20,10 +20,11 @@
 // integers, and composing two such maps gives another one:
 //     (g o f)[b] = f[b] + g[(b + f[b]) & 1].
 // Composition is associative, so tiles of 4096 elements reduce to one map in
-// parallel, a single wavefront chains the tile maps (scanning 64 at a time),
-// and each tile then re-derives every thread's exact start value and finishes
-// its 16 elements with ordinary IEEE additions, which are now exact replicas of
-// numpy's. A tile is only handled by maps if an approximate prefix puts it well
+// parallel, groups of 64 tiles reduce to one map, a single wavefront chains
+// the group maps (64 groups = 2^24 elements per step), the tile starts inside
+// each group follow from a scan of its tile maps, and each tile then
+// re-derives every thread's exact start value and finishes its 16 elements
+// with ordinary IEEE additions, which are now exact replicas of numpy's. A tile is only handled by maps if an approximate prefix puts it well
 // inside one binade, and the chain re-checks that exactly (start in the binade,
 // end below 2^(e+1)); the few tiles where s crosses a power of two are summed
 // element by element in order.
@@ -428,62 +429,150 @@ __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_
   }
 }
 
-// (D) exact chain over tiles: one wavefront
+// (D1) maps of groups of 64 tiles: usable when all 64 tiles are usable on
+// the same grid (flag 1), else flag 0.
+__global__ __launch_bounds__(64) void k_group_map(const TileInfo* __restrict__ info, int64_t ntiles,
+                                                  TileInfo* __restrict__ ginfo) {
+  const int lane = threadIdx.x;
+  const int64_t t = int64_t(blockIdx.x) * 64 + lane;
+  TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
+  const int Eb0 = __shfl(ti.Eb, 0, 64);
+  if (t >= ntiles) ti.Eb = Eb0;
+  const bool uniform = __all(ti.flag == 1 && ti.Eb == Eb0);
+  const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
+  if (lane == 63) {
+    const bool ok = uniform && incl.c0 < kTwo53 && incl.c1 < kTwo53;
+    ginfo[blockIdx.x] = {incl.c0, incl.c1, Eb0, ok ? 1 : 0};
+  }
+}
+
+// Tile-level chain of one group (64 tiles) from the exact start s; every lane
+// computes the same scalar chain. Writes tile starts, tile flags (0 = finish
+// by maps, 1 = summed here) and the cdf of tiles summed here. Returns s.
 template <typename P>
-__global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n, int64_t ntiles,
-                                              TileInfo* __restrict__ info, double* __restrict__ tstart,
+__device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t ntiles, int64_t g,
+                                    TileInfo* __restrict__ info, double* __restrict__ tstart,
+                                    double* __restrict__ cdf, double s) {
+  const int lane = threadIdx.x;
+  const int64_t t = g * 64 + lane;
+  TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
+  const int Eb0 = __shfl(ti.Eb, 0, 64);
+  if (t >= ntiles) ti.Eb = Eb0;
+  if (__all(ti.flag == 1 && ti.Eb == Eb0) && in_binade(s, Eb0)) {
+    const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
+    const Map tot = shfl_map(incl, 63);
+    const int64_t m0 = m_of(s, Eb0);
+    const int64_t mend = m0 + ((m0 & 1) ? tot.c1 : tot.c0);
+    if (mend < kTwo53) {
+      Map excl = shfl_up_map(incl, 1);
+      if (lane == 0) excl = {0, 0};
+      if (t < ntiles) {
+        tstart[t] = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), Eb0);
+        info[t].flag = 0;
+      }
+      return from_m(mend, Eb0);
+    }
+  }
+  const int64_t gend = g * 64 + 64 < ntiles ? g * 64 + 64 : ntiles;
+  for (int64_t u = g * 64; u < gend; ++u) {
+    const TileInfo tu = info[u];
+    if (lane == 0) tstart[u] = s;
+    bool ok = tu.flag == 1 && in_binade(s, tu.Eb);
+    int64_t m2 = 0;
+    if (ok) {
+      const int64_t m = m_of(s, tu.Eb);
+      m2 = m + ((m & 1) ? tu.c1 : tu.c0);
+      ok = m2 < kTwo53;
+    }
+    if (ok) {
+      s = from_m(m2, tu.Eb);
+      if (lane == 0) info[u].flag = 0;
+    } else {
+      const int64_t lo = u * kTile;
+      const int64_t hi = lo + kTile < n ? lo + kTile : n;
+#pragma unroll 8
+      for (int64_t i = lo; i < hi; ++i) {
+        s = s + load_p(p, i);
+        if (lane == 0) cdf[i] = s;
+      }
+      if (lane == 0) info[u].flag = 1;
+    }
+  }
+  return s;
+}
+
+// (D2) exact chain over groups: one wavefront, 64 groups (262144 elements)
+// per step when they are all resolvable by their maps; group flags become
+// 0 (start known, tiles finished by k_tile_starts) or 1 (chained tile by tile).
+template <typename P>
+__global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n, int64_t ntiles, int64_t ngroups,
+                                              TileInfo* __restrict__ info, TileInfo* __restrict__ ginfo,
+                                              double* __restrict__ gstart, double* __restrict__ tstart,
                                               double* __restrict__ cdf) {
   const int lane = threadIdx.x;
   double s = 0.0;
-  for (int64_t g = 0; g < ntiles; g += 64) {
-    const int64_t t = g + lane;
-    TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
-    const int Eb0 = __shfl(ti.Eb, 0, 64);
-    if (t >= ntiles) ti.Eb = Eb0;
-    const bool mine = ti.flag == 1 && ti.Eb == Eb0;
-    const bool uniform = __all(mine) && in_binade(s, Eb0);
-    if (uniform) {
-      const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
+  for (int64_t k = 0; k < ngroups; k += 64) {
+    const int64_t gg = k + lane;
+    TileInfo gi = gg < ngroups ? ginfo[gg] : TileInfo{0, 0, 0, 1};
+    const int Eb0 = __shfl(gi.Eb, 0, 64);
+    if (gg >= ngroups) gi.Eb = Eb0;
+    if (__all(gi.flag == 1 && gi.Eb == Eb0) && in_binade(s, Eb0)) {
+      const Map incl = wave_scan_incl(Map{gi.c0, gi.c1});
       const Map tot = shfl_map(incl, 63);
       const int64_t m0 = m_of(s, Eb0);
       const int64_t mend = m0 + ((m0 & 1) ? tot.c1 : tot.c0);
       if (mend < kTwo53) {
         Map excl = shfl_up_map(incl, 1);
         if (lane == 0) excl = {0, 0};
-        if (t < ntiles) {
-          const int64_t ml = m0 + ((m0 & 1) ? excl.c1 : excl.c0);
-          tstart[t] = from_m(ml, Eb0);
-          info[t].flag = 0;
+        if (gg < ngroups) {
+          gstart[gg] = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), Eb0);
+          ginfo[gg].flag = 0;
         }
         s = from_m(mend, Eb0);
         continue;
       }
     }
-    // slow path: tile by tile, every lane computing the same scalar chain
-    const int64_t gend = g + 64 < ntiles ? g + 64 : ntiles;
-    for (int64_t u = g; u < gend; ++u) {
-      const TileInfo tu = info[u];
-      if (lane == 0) tstart[u] = s;
-      bool ok = tu.flag == 1 && in_binade(s, tu.Eb);
+    const int64_t kend = k + 64 < ngroups ? k + 64 : ngroups;
+    for (int64_t g = k; g < kend; ++g) {
+      const TileInfo gu = ginfo[g];
+      bool ok = gu.flag == 1 && in_binade(s, gu.Eb);
       int64_t m2 = 0;
       if (ok) {
-        const int64_t m = m_of(s, tu.Eb);
-        m2 = m + ((m & 1) ? tu.c1 : tu.c0);
+        const int64_t m = m_of(s, gu.Eb);
+        m2 = m + ((m & 1) ? gu.c1 : gu.c0);
         ok = m2 < kTwo53;
       }
       if (ok) {
-        s = from_m(m2, tu.Eb);
-        if (lane == 0) info[u].flag = 0;
-      } else {
-        const int64_t lo = u * kTile;
-        const int64_t hi = lo + kTile < n ? lo + kTile : n;
-        for (int64_t i = lo; i < hi; ++i) {
-          s = s + load_p(p, i);
-          if (lane == 0) cdf[i] = s;
+        if (lane == 0) {
+          gstart[g] = s;
+          ginfo[g].flag = 0;
         }
-        if (lane == 0) info[u].flag = 1;
+        s = from_m(m2, gu.Eb);
+      } else {
+        if (lane == 0) ginfo[g].flag = 1;
+        s = chain_group_tiles(p, n, ntiles, g, info, tstart, cdf, s);
       }
     }
+  }
+}
+
+// (D3) tile starts inside the groups the chain resolved by their maps
+__global__ __launch_bounds__(64) void k_tile_starts(TileInfo* __restrict__ info, int64_t ntiles,
+                                                    const TileInfo* __restrict__ ginfo, const double* __restrict__ gstart,
+                                                    double* __restrict__ tstart) {
+  const int64_t g = blockIdx.x;
+  if (ginfo[g].flag != 0) return;
+  const int lane = threadIdx.x;
+  const int64_t t = g * 64 + lane;
+  const TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
+  const int Eb = ginfo[g].Eb;
+  const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
+  Map excl = shfl_up_map(incl, 1);
+  if (lane == 0) excl = {0, 0};
+  if (t < ntiles) {
+    const int64_t m0 = m_of(gstart[g], Eb);
+    tstart[t] = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), Eb);
+    info[t].flag = 0;
   }
 }
 
@@ -527,19 +616,24 @@ struct CumsumWs {
   double* astart;
   double* tstart;
   TileInfo* info;
+  TileInfo* ginfo;
+  double* gstart;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 static size_t cumsum_ws_bytes(int64_t n) {
   const size_t T = size_t((n + kTile - 1) / kTile);
-  return 3 * align256(T * sizeof(double)) + align256(T * sizeof(TileInfo));
+  const size_t G = (T + 63) / 64;
+  return 3 * align256(T * sizeof(double)) + align256(T * sizeof(TileInfo)) + align256(G * sizeof(TileInfo)) +
+         align256(G * sizeof(double));
 }
 
 template <typename P>
 static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipStream_t s) {
   const P* p = reinterpret_cast<const P*>(probs);
   const int64_t T = (n + kTile - 1) / kTile;
+  const int64_t G = (T + 63) / 64;
   char* w = reinterpret_cast<char*>(ws);
   CumsumWs c;
   c.tsum = reinterpret_cast<double*>(w);
@@ -549,13 +643,21 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
   c.tstart = reinterpret_cast<double*>(w);
   w += align256(T * sizeof(double));
   c.info = reinterpret_cast<TileInfo*>(w);
+  w += align256(T * sizeof(TileInfo));
+  c.ginfo = reinterpret_cast<TileInfo*>(w);
+  w += align256(G * sizeof(TileInfo));
+  c.gstart = reinterpret_cast<double*>(w);
   hipLaunchKernelGGL(k_tile_sum<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.tsum);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_approx_prefix, dim3(1), dim3(1024), 0, s, c.tsum, T, c.astart);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(64), 0, s, p, n, T, c.info, c.tstart, cdf);
+  hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(64), 0, s, p, n, T, G, c.info, c.ginfo, c.gstart, c.tstart, cdf);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tile_starts, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo, c.gstart, c.tstart);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_tile_apply<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.info, c.tstart, cdf);
   QSIM_CHECK_LAUNCH();
