@@ -15,7 +15,19 @@ stream inside the timed region, and a CPU baseline: the numpy oracle of
 qibo's NumpyBackend timed on a bounded sample of the same circuit's gates on
 this host.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--nqubits 30] [--no-cpu]
+Beside the headline line the same JSON object carries the other parts of
+the metric, each on its own bounded workload:
+* "fused_c64": the layered generator at 32 qubits in complex64 run unfused,
+  fused with circuit.fuse(max_qubits=5) (5-qubit blocks on MFMA), and in
+  complex128 (64 GiB state); infidelity and max-abs amplitude error of the
+  fused complex64 state against complex128 at 30 qubits;
+* "serving": a Poisson stream of layered circuits (n ~ U[10, 30], depth ~
+  U[10, 50], half complex64, half <sum Z> / shots) through the continuous-
+  batching server: circuits/s, p50 and p99 latency;
+* "adjoint": adjoint gradients of the TFIM energy for the 20-qubit, 10-layer
+  RY/RZ + CNOT-ladder ansatz over 64 parameter sets, gradients/s.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--nqubits 30] [--no-cpu] [--no-extras]
 """
 
 import argparse
@@ -83,6 +95,110 @@ def cpu_baseline(nqubits, nlayers, seed, budget_s):
     }
 
 
+def _timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fn()
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t0
+
+
+def bench_fused(seed, n_time=32, n_fid=30, nlayers=20):
+    """Fused complex64 (MFMA blocks) vs unfused complex64 vs complex128."""
+    import torch
+
+    from qibo_amd import models
+    from qibo_amd.backend import MI355XBackend
+
+    b64 = MI355XBackend(dtype="complex64")
+    b128 = MI355XBackend(dtype="complex128")
+    circ = models.random_layered_circuit(n_time, nlayers, seed=seed, measure=False)
+    fused = circ.fuse(max_qubits=5)
+    nblocks = sum(1 for g in fused.queue if g.__class__.__name__ == "FusedGate")
+    res = {"nqubits": n_time, "layers": nlayers, "fused_gates": len(fused.queue), "blocks_5q": nblocks}
+    for key, backend, c in (("unfused_c64_s", b64, circ), ("fused_c64_s", b64, fused),
+                            ("unfused_c128_s", b128, circ)):
+        c._final_state = None
+        r, dt = _timed(lambda: backend.execute_circuit(c))
+        res[key] = dt
+        del r
+        c._final_state = None
+        torch.cuda.empty_cache()
+    res["speedup_fused_c64_vs_c128"] = res["unfused_c128_s"] / res["fused_c64_s"]
+    res["speedup_fused_c64_vs_unfused_c64"] = res["unfused_c64_s"] / res["fused_c64_s"]
+    # accuracy at n_fid: fused complex64 against complex128
+    circ = models.random_layered_circuit(n_fid, nlayers, seed=seed, measure=False)
+    s64 = b64.execute_circuit(circ.fuse(max_qubits=5)).state()
+    circ._final_state = None
+    s128 = b128.execute_circuit(circ).state()
+    a = s64.to(torch.complex128)
+    overlap = torch.vdot(a, s128)
+    res["nqubits_accuracy"] = n_fid
+    res["infidelity"] = float(1.0 - abs(complex(overlap.item())) ** 2)
+    res["max_abs_err"] = float(torch.max(torch.abs(a - s128)).item())
+    del s64, s128, a
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_serving(seed, nrequests=48, qps=16.0):
+    import torch
+
+    from qibo_amd.backend import MI355XBackend
+    from qibo_amd.serving import CircuitServer, GPUExecutor, poisson_stream
+
+    backends = {"complex128": MI355XBackend(dtype="complex128"), "complex64": MI355XBackend(dtype="complex64")}
+    ex = GPUExecutor(backends, nstreams=8)
+    budget = int(0.8 * torch.cuda.mem_get_info()[0])
+    reqs = poisson_stream(nrequests, qps=qps, seed=seed)
+    stats = CircuitServer(ex, budget, max_inflight=8).run(reqs, realtime=True)
+    out = stats.summary()
+    out.update({"qps_offered": qps, "requests": nrequests, "nqubits_range": [10, 30],
+                "depth_range": [10, 50], "hbm_budget_bytes": budget})
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_adjoint(seed, n=20, nlayers=10, nsets=64):
+    import numpy as np
+    import torch
+
+    from qibo_amd import hamiltonians, models
+    from qibo_amd.adjoint import batched_expectation_and_gradient
+    from qibo_amd.backend import MI355XBackend
+
+    rng = np.random.default_rng(seed)
+    params = rng.uniform(0, 2 * np.pi, (nsets, 2 * n * nlayers))
+    out = {"nqubits": n, "layers": nlayers, "parameters": 2 * n * nlayers, "parameter_sets": nsets}
+    for dtype in ("complex128", "complex64"):
+        backend = MI355XBackend(dtype=dtype)
+        c = models.hardware_efficient_ansatz(n, nlayers)
+        ham = hamiltonians.TFIM(n, h=1.0, closed_boundary=False, backend=backend)
+        batched_expectation_and_gradient(backend, c, ham, params[:1])  # warm-up
+        (e, g), dt = _timed(lambda: batched_expectation_and_gradient(backend, c, ham, params))
+        out[f"{dtype}_s"] = dt
+        out[f"{dtype}_gradients_per_s"] = nsets / dt
+        if dtype == "complex128":
+            # central differences (eps = 1e-4) on four parameters of set 0
+            c.set_parameters(params[0])
+            errs = []
+            for k in rng.choice(params.shape[1], 4, replace=False):
+                p = params[0].copy()
+                p[k] += 1e-4
+                c.set_parameters(p)
+                fp = ham.expectation(c)
+                p[k] -= 2e-4
+                c.set_parameters(p)
+                fm = ham.expectation(c)
+                errs.append(abs(float(g[0, k]) - (fp - fm) / 2e-4))
+            out["max_abs_err_vs_central_fd"] = max(errs)
+        del e, g
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +210,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -207,6 +324,10 @@ def main():
             "GBs": copy_bytes / (min(copy_ms) * 1e-3) / 1e9,
         },
     }
+    if not args.no_extras:
+        out["fused_c64"] = bench_fused(args.seed)
+        out["serving"] = bench_serving(args.seed)
+        out["adjoint"] = bench_adjoint(args.seed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.nlayers, args.seed, args.cpu_budget)
     print(json.dumps(out))

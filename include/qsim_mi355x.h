@@ -86,8 +86,10 @@ int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const i
 
 /* Dense 2^k x 2^k block on k = ntargets (1..5) targets with the matrix in
  * DEVICE memory: the application of a FusedGate (abstract.py:2680-2717 builds
- * the matrix; apply_gate applies it). Arithmetic in the state's precision
- * (FP64 FMA for complex128, FP32 for complex64). */
+ * the matrix; apply_gate applies it). complex128: FP64 FMA per column;
+ * complex64 blocks of 4-5 qubits (nqubits >= 12): v_mfma_f32_32x32x2f32 on the
+ * real embedding [[Re, -Im], [Im, Re]] with FP32 accumulation; smaller
+ * complex64 blocks: FP32 FMA. */
 int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, int ntargets,
                      const void* matrix_dev, void* stream);
 
@@ -130,6 +132,18 @@ int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qub
 size_t qsim_cumsum_workspace(int64_t n);
 int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* numpy's np.sum of a 1-D array of n = 2^m probabilities (F64 or F32, the
+ * result in the same dtype, one device value), bit for bit: pairwise sums of
+ * consecutive 8192-element buffers accumulated in order from 0. The
+ * normalisation `probabilities / sum(probabilities)` of
+ * Backend.sample_frequencies (abstract.py:2762) uses it, followed by
+ * qsim_divide_by (out[i] = probs[i] / divisor[0], correctly rounded).
+ * `workspace` must hold qsim_np_sum_workspace(n) bytes. */
+size_t qsim_np_sum_workspace(int64_t n);
+int qsim_np_sum(const void* probs, int prob_dtype, int64_t n, void* out, void* workspace, size_t workspace_bytes,
+                void* stream);
+int qsim_divide_by(const void* probs, int prob_dtype, int64_t n, const void* divisor, void* out, void* stream);
 
 /* Inverse-CDF draw: out[k] = number of i with fl(cdf[i] / cdf[n-1]) <= u[k],
  * i.e. numpy's `(cdf / cdf[-1]).searchsorted(u, side="right")` on the

@@ -37,6 +37,9 @@ SIGNATURES = {
     "qsim_probabilities": (_i, [_vp, _i, _i, _c_int_p, _i, _vp, _vp, _sz, _vp]),
     "qsim_cumsum_workspace": (_sz, [_i64]),
     "qsim_cumsum_exact": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "qsim_np_sum_workspace": (_sz, [_i64]),
+    "qsim_np_sum": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "qsim_divide_by": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "qsim_search_sorted": (_i, [_vp, _i64, _vp, _i64, _vp, _vp]),
     "qsim_samples_to_binary": (_i, [_vp, _i64, _i, _vp, _vp]),
     "qsim_histogram": (_i, [_vp, _i64, _vp, _i64, _vp]),

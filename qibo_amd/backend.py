@@ -656,9 +656,35 @@ class MI355XBackend(GatePlanner):
         u = np.random.random_sample(int(nshots))
         return self.search_cdf(cdf, u)
 
+    def np_sum(self, p: torch.Tensor) -> torch.Tensor:
+        """numpy's ``np.sum`` of a power-of-two-long probability vector, bit for bit
+        (one device value of p's dtype)."""
+        n = p.numel()
+        out = torch.empty(1, dtype=p.dtype, device=p.device)
+        wsb = self.lib.qsim_np_sum_workspace(n)
+        ws = self._workspace(wsb, "npsum", p.device)
+        pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
+        _lib.check(self.lib.qsim_np_sum(p.data_ptr(), pcode, n, out.data_ptr(), ws.data_ptr(), wsb,
+                                        self._stream(p)), self.lib)
+        return out
+
     def sample_frequencies(self, probabilities, nshots: int):
+        """Frequencies of ``nshots`` draws (abstract.py:2760-2772).
+
+        The normalisation ``p / sum(p)`` uses numpy's own summation order, and
+        the draws the same batches of SHOT_BATCH_SIZE uniforms from numpy's
+        global generator, so the counts equal the reference's for the same
+        seed and probabilities."""
         p = self._as_prob_tensor(probabilities)
-        nprobs = p / p.sum()
+        n = p.numel()
+        if n & (n - 1) == 0:
+            total = self.np_sum(p)
+            nprobs = torch.empty_like(p)
+            pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
+            _lib.check(self.lib.qsim_divide_by(p.data_ptr(), pcode, n, total.data_ptr(), nprobs.data_ptr(),
+                                               self._stream(p)), self.lib)
+        else:
+            nprobs = p / p.sum()
         self._validate_probabilities(nprobs)
         cdf = self.exact_cdf(nprobs)
         freqs = torch.zeros(nprobs.numel(), dtype=torch.int64, device=p.device)

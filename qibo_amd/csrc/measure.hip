@@ -702,6 +702,80 @@ __global__ __launch_bounds__(kT) void k_histogram(const int64_t* __restrict__ sm
   if (v >= 0 && v < nbins) atomicAdd(&counts[v], 1ull);
 }
 
+
+// ---------------------------------------------------------------------------
+// numpy's np.sum of a 1-D array, bit for bit
+// ---------------------------------------------------------------------------
+// np.sum reduces a contiguous array buffer by buffer (8192 elements per
+// buffer, numpy's default ufunc buffer size), adding each buffer's pairwise
+// sum to a running total that starts at 0. The pairwise sum of a buffer
+// splits it in halves (rounded down to a multiple of 8) down to blocks of at
+// most 128 elements; a block of fewer than 8 elements is summed in order from
+// 0, a larger block with 8 interleaved accumulators combined as
+// ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) plus a sequential tail.
+// For power-of-two lengths every split is exact, so a buffer of 2^b >= 256
+// elements is a balanced tree over 128-element leaves. Sample frequencies
+// normalise by this sum (Backend.sample_frequencies, abstract.py:2762).
+constexpr int kNpBuf = 8192;
+constexpr int kNpLeaf = 128;
+
+template <typename F>
+__device__ F np_leaf_sum(const F* a, int n) {
+  if (n < 8) {
+    F res = F(0);
+    for (int i = 0; i < n; ++i) res = res + a[i];
+    return res;
+  }
+  F r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = r[j] + a[i + j];
+  }
+  F res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res = res + a[i];
+  return res;
+}
+
+// One block of 64 threads per 8192-element buffer (power-of-two length n).
+template <typename F>
+__global__ __launch_bounds__(64) void k_np_buffer_sums(const F* __restrict__ p, int64_t n, F* __restrict__ bsum) {
+  __shared__ F leaf[kNpBuf / kNpLeaf];
+  const int64_t lo = int64_t(blockIdx.x) * kNpBuf;
+  const int64_t len = (n - lo) < kNpBuf ? (n - lo) : kNpBuf;
+  if (len <= kNpLeaf) {
+    if (threadIdx.x == 0) bsum[blockIdx.x] = np_leaf_sum(p + lo, int(len));
+    return;
+  }
+  const int nleaf = int(len / kNpLeaf);
+  if (threadIdx.x < nleaf) leaf[threadIdx.x] = np_leaf_sum(p + lo + int64_t(threadIdx.x) * kNpLeaf, kNpLeaf);
+  __syncthreads();
+  for (int w = nleaf; w > 1; w >>= 1) {
+    F v = F(0);
+    if (threadIdx.x < w / 2) v = leaf[2 * threadIdx.x] + leaf[2 * threadIdx.x + 1];
+    __syncthreads();
+    if (threadIdx.x < w / 2) leaf[threadIdx.x] = v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = leaf[0];
+}
+
+template <typename F>
+__global__ void k_np_total(const F* __restrict__ bsum, int64_t nbuf, F* __restrict__ out) {
+  F acc = F(0);
+  for (int64_t b = 0; b < nbuf; ++b) acc = acc + bsum[b];
+  out[0] = acc;
+}
+
+template <typename F>
+__global__ __launch_bounds__(kT) void k_div_by(const F* __restrict__ p, int64_t n, const F* __restrict__ d,
+                                               F* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i < n) out[i] = p[i] / d[0];
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -744,6 +818,50 @@ int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (prob_dtype == QSIM_F64) return cumsum_impl<double>(probs, n, cdf, workspace, s);
   return cumsum_impl<float>(probs, n, cdf, workspace, s);
+}
+
+size_t qsim_np_sum_workspace(int64_t n) {
+  return n < 1 ? 0 : size_t((n + kNpBuf - 1) / kNpBuf) * sizeof(double);
+}
+
+int qsim_np_sum(const void* probs, int prob_dtype, int64_t n, void* out, void* workspace, size_t workspace_bytes,
+                void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(probs != nullptr && out != nullptr && workspace != nullptr, "qsim_np_sum: null pointer");
+  QSIM_CHECK_ARG(n >= 1 && (n & (n - 1)) == 0, "qsim_np_sum: length must be a power of two");
+  QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_np_sum: bad dtype");
+  const int64_t nbuf = (n + kNpBuf - 1) / kNpBuf;
+  QSIM_CHECK_ARG(workspace_bytes >= size_t(nbuf) * sizeof(double), "qsim_np_sum: workspace too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (prob_dtype == QSIM_F64) {
+    hipLaunchKernelGGL(k_np_buffer_sums<double>, dim3(unsigned(nbuf)), dim3(64), 0, s, (const double*)probs, n,
+                       (double*)workspace);
+    QSIM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_np_total<double>, dim3(1), dim3(1), 0, s, (const double*)workspace, nbuf, (double*)out);
+  } else {
+    hipLaunchKernelGGL(k_np_buffer_sums<float>, dim3(unsigned(nbuf)), dim3(64), 0, s, (const float*)probs, n,
+                       (float*)workspace);
+    QSIM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_np_total<float>, dim3(1), dim3(1), 0, s, (const float*)workspace, nbuf, (float*)out);
+  }
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_divide_by(const void* probs, int prob_dtype, int64_t n, const void* divisor, void* out, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(probs != nullptr && divisor != nullptr && out != nullptr, "qsim_divide_by: null pointer");
+  QSIM_CHECK_ARG(n >= 1, "qsim_divide_by: bad length");
+  QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_divide_by: bad dtype");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (prob_dtype == QSIM_F64)
+    hipLaunchKernelGGL(k_div_by<double>, dim3(ceil_div_i(uint64_t(n), kT)), dim3(kT), 0, s, (const double*)probs, n,
+                       (const double*)divisor, (double*)out);
+  else
+    hipLaunchKernelGGL(k_div_by<float>, dim3(ceil_div_i(uint64_t(n), kT)), dim3(kT), 0, s, (const float*)probs, n,
+                       (const float*)divisor, (float*)out);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
 }
 
 int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int64_t nshots, int64_t* out,

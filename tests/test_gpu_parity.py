@@ -109,13 +109,28 @@ def test_fused_circuit_c128(backend, maxq):
     assert np.max(np.abs(got - want)) <= TOL128
 
 
+@pytest.mark.parametrize("n", [11, 14])
 @pytest.mark.parametrize("maxq", [4, 5])
-def test_fused_circuit_c64(backend64, maxq):
-    n = 11
+def test_fused_circuit_c64(backend64, maxq, n):
+    """n = 14 runs the 4- and 5-qubit blocks on MFMA (FP32 accumulation)."""
     c = models.random_layered_circuit(n, 4, seed=8, measure=False)
     want = OracleBackend().execute(c)
     got = to_np(backend64.execute_circuit(c.fuse(max_qubits=maxq)).state())
     assert np.max(np.abs(got - want)) <= 2e-5
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_mfma_block_random_unitary(backend64, k, rng):
+    """A random dense block on scattered targets (including the lowest bit), MFMA path."""
+    n = 15
+    a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+    u, _ = np.linalg.qr(a)
+    targets = [14, 3, 9, 0, 7][:k]
+    psi = rand_state(n, rng)
+    g = gates.Unitary(u, *targets)
+    got = to_np(backend64.apply_gate(g, backend64.cast(psi.astype(np.complex64)), n))
+    want = OracleBackend().apply_gate(g, psi.copy(), n)
+    assert np.max(np.abs(got - want)) <= 2e-6
 
 
 def _formula_abs2(z):
@@ -327,3 +342,26 @@ def test_errors(backend):
         config.set_hbm_budget(None)
     with pytest.raises(ValueError, match="sum to 1"):
         backend.sample_shots(np.array([0.5, 0.6]), 10)
+
+
+@pytest.mark.parametrize("m", [0, 3, 7, 8, 13, 14, 20, 22])
+def test_np_sum_bitwise(backend, rng, m):
+    p = rng.random(2**m) * 10.0 ** rng.integers(-6, 6, size=2**m)
+    got = float(backend.np_sum(torch.as_tensor(p, device=backend.torch_device)).item())
+    assert got == float(np.sum(p))
+    p32 = p.astype(np.float32)
+    got32 = backend.np_sum(torch.as_tensor(p32, device=backend.torch_device)).cpu().numpy()[0]
+    assert got32 == np.sum(p32)
+
+
+@pytest.mark.parametrize("m,nshots", [(10, 3 * 2**18 + 17), (20, 2**18 + 5)])
+def test_sample_frequencies_bit_exact(backend, rng, m, nshots):
+    """Counts equal the oracle's (numpy) for the same seed: normalisation by
+    numpy's summation order, the same batching of uniforms, exact cumsum."""
+    p = rng.random(2**m) ** 4
+    p /= p.sum()
+    np.random.seed(77)
+    want = OracleBackend().sample_frequencies(p, nshots)
+    np.random.seed(77)
+    got = backend.sample_frequencies(torch.as_tensor(p, device=backend.torch_device), nshots)
+    assert dict(got) == dict(want)

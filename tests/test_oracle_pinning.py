@@ -381,3 +381,39 @@ def test_oracle_fused_equals_unfused(rng):
     want = ob.execute(c, psi)
     for k in (2, 3, 4, 5):
         np.testing.assert_allclose(ob.execute(c.fuse(max_qubits=k), psi), want, atol=1e-12)
+
+
+def _np_pairwise(a, lo, hi):
+    n = hi - lo
+    if n < 8:
+        res = 0.0
+        for i in range(lo, hi):
+            res = res + a[i]
+        return res
+    if n <= 128:
+        r = [a[lo + j] for j in range(8)]
+        i = 8
+        while i < n - (n % 8):
+            for j in range(8):
+                r[j] = r[j] + a[lo + i + j]
+            i += 8
+        res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]))
+        while i < n:
+            res = res + a[lo + i]
+            i += 1
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return _np_pairwise(a, lo, lo + n2) + _np_pairwise(a, lo + n2, hi)
+
+
+@pytest.mark.parametrize("n", [1, 4, 64, 128, 256, 4096, 8192, 2**15, 2**17])
+def test_numpy_sum_order(n, rng):
+    """np.sum(1-D float64) = sum over 8192-element buffers of their pairwise
+    sums, accumulated from 0 -- the order the GPU's qsim_np_sum reproduces."""
+    a = rng.random(n) * 10.0 ** rng.integers(-6, 6, n)
+    al = [float(x) for x in a]
+    total = 0.0
+    for lo in range(0, n, 8192):
+        total = total + _np_pairwise(al, lo, min(lo + 8192, n))
+    assert total == float(np.sum(a))
