@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 1
+#define QSIM_ABI_VERSION 2
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -98,17 +98,22 @@ int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, in
  * abstract.py:2322-2361) for a run of consecutive gates that the host has
  * grouped so that their non-diagonal targets fit in 12 "tile bits".
  * `tile_bits` = 12 ascending amplitude-index bit positions (not qubit
- * numbers); `program_dev` = DEVICE array of `ninstr` instruction records of
- * qsim_blocked_record_bytes(dtype) bytes each (layout: u32 op, u32 arg,
- * u64 control mask, u32 nt, u8 bits[8], u32 pad, then 64 complex entries;
- * ops: 0 diagonal on up to 5 global bits, 1 one-qubit gate on a register bit
- * of the current mapping, 2 change of register mapping, 3 dense 2^k block on
- * k <= 3 local tile bits). The result is bitwise that of applying the same
- * gates one by one with qsim_apply_dense / qsim_apply_diag / qsim_apply_swap.
- * Requires nqubits >= 12. */
-size_t qsim_blocked_record_bytes(int dtype);
+ * numbers). `program_dev` = DEVICE buffer (16-byte aligned) of
+ * `program_bytes` bytes holding `ninstr` variable-length records: a 32-byte
+ * header (u8 op, u8 arg, u8 nt, u8 register-condition mask, u32 number of
+ * complex entries, u64 condition mask, u8 bits[8], u64 zero) followed by the
+ * entries, padded to 16 bytes. Ops: 0 diagonal on up to 5 bits, 1 complex 2x2
+ * on a register bit of the current mapping, 2 change of register mapping,
+ * 3 dense 2^k block on k <= 3 local tile bits, 4 real 2x2 on a register bit,
+ * 5 16-entry register phase table, 6 per-lane diagonal factor, 7 sign flip.
+ * The kernel stops at the first record that would extend past program_bytes
+ * or carries fewer entries than its op needs. A program of ops 0-3 only gives
+ * bitwise the state of applying the same gates one by one with
+ * qsim_apply_dense / qsim_apply_diag / qsim_apply_swap; ops 4-7 reorder
+ * roundings. Requires nqubits >= 12. */
+size_t qsim_blocked_header_bytes(void);
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
-                       int ninstr, void* stream);
+                       size_t program_bytes, int ninstr, void* stream);
 
 /* Probabilities of measuring `qubits` (m = nmeasured, in the given order, the
  * first being the most significant bit of the output index):

@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
@@ -47,8 +47,8 @@ SIGNATURES = {
     "qsim_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "qsim_apply_pauli_sum": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "qsim_inner_matrix": (_i, [_vp, _vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp, _vp, _sz, _vp]),
-    "qsim_blocked_record_bytes": (_sz, [_i]),
-    "qsim_apply_blocked": (_i, [_vp, _i, _i, _c_int_p, _vp, _i, _vp]),
+    "qsim_blocked_header_bytes": (_sz, []),
+    "qsim_apply_blocked": (_i, [_vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
 }
 
 _lock = threading.Lock()

@@ -223,10 +223,13 @@ class MI355XBackend(GatePlanner):
         self._ws = {}  # workspace cache: (device, purpose, stream) -> tensor
         self._timer = None  # (selector, [(start, end) events]) while kernel timing is on
         self.blocking = True  # group gates into blocked passes in run_queue
-        self.block_stats = []  # gates per blocked pass (diagnostics)
-        rec = self.lib.qsim_blocked_record_bytes(_lib.C128)
-        if rec != blocking.HEADER_BYTES + blocking.NMAT * 16:
-            raise ImportError(f"qibo_amd: blocked-pass record size {rec} does not match the host layout")
+        # keep circuit order inside blocked passes (bitwise equal to the per-gate
+        # path); the default lets commuting gates move ahead, giving fewer passes
+        self.blocking_ordered = False
+        self.block_stats = collections.deque(maxlen=1 << 16)  # gates per blocked pass (diagnostics)
+        rec = self.lib.qsim_blocked_header_bytes()
+        if rec != blocking.HEADER_BYTES:
+            raise ImportError(f"qibo_amd: blocked-pass record header of {rec} bytes does not match the host layout")
 
     # -- qibo Backend attributes -----------------------------------------
     @property
@@ -485,7 +488,10 @@ class MI355XBackend(GatePlanner):
 
         With ``self.blocking`` (the default) consecutive gates are grouped into
         blocked passes (qibo_amd.blocking) that read and write the state once
-        each; the result is bitwise that of applying the gates one by one.
+        each. Commuting gates may move ahead of each other, which changes the
+        rounding of the result by a few units in the last place; with
+        ``self.blocking_ordered`` circuit order is kept and the result is bitwise
+        that of applying the gates one by one.
         Measurement gates only register the backend (non-collapsing ``M`` is a
         no-op on the state); other special gates run in order between passes.
         """
@@ -548,12 +554,11 @@ class MI355XBackend(GatePlanner):
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         low = 3 if code == _lib.C128 else 4
-        plan, program = blocking.compile_circuit(ops, nqubits, cdtype, low)
+        plan, program = blocking.compile_circuit(ops, nqubits, cdtype, low, self.blocking_ordered)
         prog = None
         if program:
             host = torch.frombuffer(bytearray(program), dtype=torch.uint8).pin_memory()
             prog = host.to(state.device, non_blocking=True)
-        rec = blocking.HEADER_BYTES + blocking.NMAT * np.dtype(cdtype).itemsize
         stream = self._stream(state)
         for entry in plan:
             if entry[0] == "single":
@@ -565,7 +570,7 @@ class MI355XBackend(GatePlanner):
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(torch.cuda.current_stream(state.device))
             rc = self.lib.qsim_apply_blocked(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits),
-                                             prog.data_ptr() + first * rec, count, stream)
+                                             prog.data_ptr() + first, len(program) - first, count, stream)
             _lib.check(rc, self.lib)
             if timer is not None and timer[0](_BLOCK_OP):
                 ev[1].record(torch.cuda.current_stream(state.device))

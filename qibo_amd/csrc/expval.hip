@@ -183,7 +183,7 @@ struct PauliApplyArgs {
 template <typename R>
 __global__ __launch_bounds__(kET) void k_pauli_apply(const cplx<R>* __restrict__ psi, cplx<R>* __restrict__ out,
                                                      const PauliApplyArgs a, int accumulate) {
-  const uint64_t i = uint64_t(blockIdx.x) * kET + threadIdx.x;
+  const uint64_t i = block_id() * kET + threadIdx.x;
   if (i >= a.n) return;
   double re = 0.0, im = 0.0;
   if (accumulate) {
@@ -327,10 +327,10 @@ int qsim_apply_pauli_sum(const void* state, int nqubits, int dtype, int nterms, 
     }
     const int acc = t0 > 0 ? 1 : 0;
     if (dtype == QSIM_C128)
-      hipLaunchKernelGGL(k_pauli_apply<double>, dim3(ceil_div_i(N, kET)), dim3(kET), 0, s, (const c128*)state,
+      hipLaunchKernelGGL(k_pauli_apply<double>, launch_grid(ceil_div_u(N, kET)), dim3(kET), 0, s, (const c128*)state,
                          (c128*)out, a, acc);
     else
-      hipLaunchKernelGGL(k_pauli_apply<float>, dim3(ceil_div_i(N, kET)), dim3(kET), 0, s, (const c64*)state,
+      hipLaunchKernelGGL(k_pauli_apply<float>, launch_grid(ceil_div_u(N, kET)), dim3(kET), 0, s, (const c64*)state,
                          (c64*)out, a, acc);
     QSIM_CHECK_LAUNCH();
   }

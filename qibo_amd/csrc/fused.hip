@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_fp(const FusedArgs a, c
   __shared__ cplx<R> sm[D * D];
   for (int i = threadIdx.x; i < D * D; i += kFusedThreads) sm[i] = mat[i];
   __syncthreads();
-  const uint64_t j = uint64_t(blockIdx.x) * kFusedThreads + threadIdx.x;
+  const uint64_t j = block_id() * kFusedThreads + threadIdx.x;
   if (j >= a.nwork) return;
   const uint64_t base = insert_zeros(j, a.pos, a.nfixed);
   cplx<R> v[D];
@@ -62,8 +62,7 @@ static int launch_fused_fp(void* state, int n, const int* targets, const void* m
       if ((r >> (K - 1 - t)) & 1) off |= uint64_t(1) << (n - 1 - targets[t]);
     a.off[r] = off;
   }
-  const int blocks = ceil_div_i(a.nwork, kFusedThreads);
-  hipLaunchKernelGGL((k_fused_fp<R, K>), dim3(blocks), dim3(kFusedThreads), 0, s, a,
+  hipLaunchKernelGGL((k_fused_fp<R, K>), launch_grid(ceil_div_u(a.nwork, kFusedThreads)), dim3(kFusedThreads), 0, s, a,
                      reinterpret_cast<const cplx<R>*>(mat), reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -226,7 +225,7 @@ static int launch_fused_mfma(void* state, int n, const int* targets, const void*
     for (int i = 0; i < kMfmaTileBits; ++i)
       if (bits[i] == gb) a.rowbit[t] = uint8_t(i);
   }
-  hipLaunchKernelGGL((k_fused_mfma_c64<K>), dim3(unsigned(a.ntiles)), dim3(256), 0, s, a,
+  hipLaunchKernelGGL((k_fused_mfma_c64<K>), launch_grid(a.ntiles), dim3(256), 0, s, a,
                      reinterpret_cast<const c64*>(mat), reinterpret_cast<c64*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;

@@ -249,14 +249,13 @@ static int launch_dense(void* state, int n, const int* targets, const int* contr
   a.tbit = n - 1 - targets[0];
   if constexpr (NT == 1) {
     if (a.tbit < line_bits<R>() && n >= kLdsTileBits) {
-      const int blocks = int((uint64_t(1) << n) / kLdsTile);
-      hipLaunchKernelGGL((k_dense1_lds<R>), dim3(blocks), dim3(kThreads), 0, s, a,
+      hipLaunchKernelGGL((k_dense1_lds<R>), launch_grid((uint64_t(1) << n) / kLdsTile), dim3(kThreads), 0, s, a,
                          reinterpret_cast<cplx<R>*>(state));
       QSIM_CHECK_LAUNCH();
       return QSIM_OK;
     }
   }
-  hipLaunchKernelGGL((k_dense<R, NT>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+  hipLaunchKernelGGL((k_dense<R, NT>), launch_grid(ceil_div_u(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                      reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -276,11 +275,11 @@ static int launch_diag(void* state, int n, const int* targets, int nt, const int
   const bool control_in_line = a.nfixed > 0 && a.pos[0] < line_bits<R>();
   if (control_in_line) {
     a.nwork = uint64_t(1) << n;
-    hipLaunchKernelGGL((k_diag_full<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+    hipLaunchKernelGGL((k_diag_full<R>), launch_grid(ceil_div_u(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                        reinterpret_cast<cplx<R>*>(state));
   } else {
     a.nwork = uint64_t(1) << (n - a.nfixed);
-    hipLaunchKernelGGL((k_diag<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+    hipLaunchKernelGGL((k_diag<R>), launch_grid(ceil_div_u(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                        reinterpret_cast<cplx<R>*>(state));
   }
   QSIM_CHECK_LAUNCH();
@@ -298,7 +297,7 @@ static int launch_swap(void* state, int n, int q0, int q1, const int* controls, 
   for (int i = 0; i < nc; ++i) a.cmask |= qmask(n, controls[i]);
   a.m0 = qmask(n, q1);  // |..q0=0..q1=1..>
   a.m1 = qmask(n, q0);  // |..q0=1..q1=0..>
-  hipLaunchKernelGGL((k_swap<R>), dim3(ceil_div_i(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
+  hipLaunchKernelGGL((k_swap<R>), launch_grid(ceil_div_u(a.nwork, kThreads)), dim3(kThreads), 0, s, a,
                      reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -320,12 +319,12 @@ int qsim_zero_state(void* state, int nqubits, int dtype, void* stream) {
   QSIM_CHECK_ARG(nqubits >= 1 && nqubits <= QSIM_MAX_QUBITS, "qsim_zero_state: nqubits out of range");
   QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_zero_state: bad dtype");
   const uint64_t n = uint64_t(1) << nqubits;
-  const int blocks = ceil_div_i(n, 256);
+  const dim3 blocks = launch_grid(ceil_div_u(n, 256));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128)
-    hipLaunchKernelGGL(k_zero_state<double>, dim3(blocks), dim3(256), 0, s, (c128*)state, n);
+    hipLaunchKernelGGL(k_zero_state<double>, blocks, dim3(256), 0, s, (c128*)state, n);
   else
-    hipLaunchKernelGGL(k_zero_state<float>, dim3(blocks), dim3(256), 0, s, (c64*)state, n);
+    hipLaunchKernelGGL(k_zero_state<float>, blocks, dim3(256), 0, s, (c64*)state, n);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }

@@ -90,7 +90,7 @@ __device__ double block_sum_fixed(double v, double* lds) {
 template <typename R>
 __global__ __launch_bounds__(kT) void k_probs_all(const cplx<R>* __restrict__ psi, R* __restrict__ out,
                                                   uint64_t n) {
-  const uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  const uint64_t i = block_id() * kT + threadIdx.x;
   if (i < n) out[i] = np_abs2(psi[i]);
 }
 
@@ -104,7 +104,7 @@ struct PermArgs {
 template <typename R>
 __global__ __launch_bounds__(kT) void k_probs_perm(const cplx<R>* __restrict__ psi, R* __restrict__ out,
                                                    const PermArgs a) {
-  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  const uint64_t o = block_id() * kT + threadIdx.x;
   if (o >= a.nout) return;
   uint64_t i = 0;
   for (int k = 0; k < a.m; ++k) i |= ((o >> (a.m - 1 - k)) & 1) << a.ibit[k];
@@ -131,7 +131,7 @@ __device__ __forceinline__ uint64_t scatter_out(uint64_t o, const MargArgs& a) {
 template <typename R>
 __global__ __launch_bounds__(kT) void k_marg_thread(const cplx<R>* __restrict__ psi, R* __restrict__ out,
                                                     const MargArgs a) {
-  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  const uint64_t o = block_id() * kT + threadIdx.x;
   if (o >= a.nout) return;
   const uint64_t ob = scatter_out(o, a);
   double acc = 0.0;
@@ -144,7 +144,7 @@ template <typename R>
 __global__ __launch_bounds__(kT) void k_marg_block(const cplx<R>* __restrict__ psi, R* __restrict__ out,
                                                    double* __restrict__ partial, const MargArgs a) {
   __shared__ double lds[kT];
-  const uint64_t b = blockIdx.x;
+  const uint64_t b = block_id();
   const uint64_t o = b / a.nchunks, c = b % a.nchunks;
   const uint64_t len = a.nin / a.nchunks;
   const uint64_t ob = scatter_out(o, a);
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kT) void k_marg_block(const cplx<R>* __restrict__ p
 template <typename R>
 __global__ __launch_bounds__(kT) void k_marg_final(const double* __restrict__ partial, R* __restrict__ out,
                                                    uint64_t nout, uint64_t nchunks) {
-  const uint64_t o = uint64_t(blockIdx.x) * kT + threadIdx.x;
+  const uint64_t o = block_id() * kT + threadIdx.x;
   if (o >= nout) return;
   double acc = 0.0;
   for (uint64_t c = 0; c < nchunks; ++c) acc += partial[o * nchunks + c];
@@ -192,7 +192,7 @@ static int probabilities_impl(const void* state, int n, const int* qubits, int m
   for (int k = 0; identity && k < m; ++k) identity = (qubits[k] == k);
   if (identity) {
     const uint64_t N = uint64_t(1) << n;
-    hipLaunchKernelGGL(k_probs_all<R>, dim3(ceil_div_i(N, kT)), dim3(kT), 0, s, psi, out, N);
+    hipLaunchKernelGGL(k_probs_all<R>, launch_grid(ceil_div_u(N, kT)), dim3(kT), 0, s, psi, out, N);
     QSIM_CHECK_LAUNCH();
     return QSIM_OK;
   }
@@ -202,7 +202,7 @@ static int probabilities_impl(const void* state, int n, const int* qubits, int m
     a.nout = uint64_t(1) << m;
     a.m = m;
     for (int k = 0; k < m; ++k) a.ibit[k] = uint8_t(n - 1 - qubits[k]);
-    hipLaunchKernelGGL(k_probs_perm<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    hipLaunchKernelGGL(k_probs_perm<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
     QSIM_CHECK_LAUNCH();
     return QSIM_OK;
   }
@@ -217,17 +217,17 @@ static int probabilities_impl(const void* state, int n, const int* qubits, int m
   bool per_thread;
   marg_plan(n, m, &a.nchunks, &per_thread);
   if (per_thread) {
-    hipLaunchKernelGGL(k_marg_thread<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    hipLaunchKernelGGL(k_marg_thread<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
     QSIM_CHECK_LAUNCH();
     return QSIM_OK;
   }
   const uint64_t need = (a.nchunks > 1) ? a.nout * a.nchunks * sizeof(double) : 0;
   QSIM_CHECK_ARG(need <= ws_bytes && (need == 0 || ws != nullptr), "qsim_probabilities: workspace too small");
   double* partial = reinterpret_cast<double*>(ws);
-  hipLaunchKernelGGL(k_marg_block<R>, dim3(unsigned(a.nout * a.nchunks)), dim3(kT), 0, s, psi, out, partial, a);
+  hipLaunchKernelGGL(k_marg_block<R>, launch_grid(a.nout * a.nchunks), dim3(kT), 0, s, psi, out, partial, a);
   QSIM_CHECK_LAUNCH();
   if (a.nchunks > 1) {
-    hipLaunchKernelGGL(k_marg_final<R>, dim3(ceil_div_i(a.nout, kT)), dim3(kT), 0, s, partial, out, a.nout,
+    hipLaunchKernelGGL(k_marg_final<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, partial, out, a.nout,
                        a.nchunks);
     QSIM_CHECK_LAUNCH();
   }
@@ -670,7 +670,7 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
 __global__ __launch_bounds__(kT) void k_search(const double* __restrict__ cdf, int64_t n,
                                                const double* __restrict__ u, int64_t nshots,
                                                int64_t* __restrict__ out) {
-  const int64_t k = int64_t(blockIdx.x) * kT + threadIdx.x;
+  const int64_t k = int64_t(block_id()) * kT + threadIdx.x;
   if (k >= nshots) return;
   const double total = cdf[n - 1];
   const double v = u[k];
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(kT) void k_search(const double* __restrict__ cdf, i
 
 __global__ __launch_bounds__(kT) void k_to_binary(const int64_t* __restrict__ smp, int64_t nshots, int nbits,
                                                   int64_t* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  const int64_t i = int64_t(block_id()) * kT + threadIdx.x;
   if (i >= nshots * nbits) return;
   const int64_t s = i / nbits;
   const int b = int(i % nbits);
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(kT) void k_to_binary(const int64_t* __restrict__ sm
 
 __global__ __launch_bounds__(kT) void k_histogram(const int64_t* __restrict__ smp, int64_t nshots,
                                                   unsigned long long* __restrict__ counts, int64_t nbins) {
-  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  const int64_t i = int64_t(block_id()) * kT + threadIdx.x;
   if (i >= nshots) return;
   const int64_t v = smp[i];
   if (v >= 0 && v < nbins) atomicAdd(&counts[v], 1ull);
@@ -772,7 +772,7 @@ __global__ void k_np_total(const F* __restrict__ bsum, int64_t nbuf, F* __restri
 template <typename F>
 __global__ __launch_bounds__(kT) void k_div_by(const F* __restrict__ p, int64_t n, const F* __restrict__ d,
                                                F* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  const int64_t i = int64_t(block_id()) * kT + threadIdx.x;
   if (i < n) out[i] = p[i] / d[0];
 }
 
@@ -811,7 +811,9 @@ int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
                       size_t workspace_bytes, void* stream) {
   clear_error();
   QSIM_CHECK_ARG(probs != nullptr && cdf != nullptr, "qsim_cumsum_exact: null pointer");
-  QSIM_CHECK_ARG(n >= 1 && n <= (int64_t(1) << QSIM_MAX_QUBITS), "qsim_cumsum_exact: bad length");
+  // one 1-D grid of 256-thread workgroups per 4096-element tile: n <= 2^35
+  // keeps it below HIP's 2^32 work-items per grid dimension
+  QSIM_CHECK_ARG(n >= 1 && n <= (int64_t(1) << 35), "qsim_cumsum_exact: length must be in [1, 2^35]");
   QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_cumsum_exact: bad dtype");
   QSIM_CHECK_ARG(workspace != nullptr && workspace_bytes >= cumsum_ws_bytes(n),
                  "qsim_cumsum_exact: workspace too small");
@@ -855,10 +857,10 @@ int qsim_divide_by(const void* probs, int prob_dtype, int64_t n, const void* div
   QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_divide_by: bad dtype");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (prob_dtype == QSIM_F64)
-    hipLaunchKernelGGL(k_div_by<double>, dim3(ceil_div_i(uint64_t(n), kT)), dim3(kT), 0, s, (const double*)probs, n,
+    hipLaunchKernelGGL(k_div_by<double>, launch_grid(ceil_div_u(uint64_t(n), kT)), dim3(kT), 0, s, (const double*)probs, n,
                        (const double*)divisor, (double*)out);
   else
-    hipLaunchKernelGGL(k_div_by<float>, dim3(ceil_div_i(uint64_t(n), kT)), dim3(kT), 0, s, (const float*)probs, n,
+    hipLaunchKernelGGL(k_div_by<float>, launch_grid(ceil_div_u(uint64_t(n), kT)), dim3(kT), 0, s, (const float*)probs, n,
                        (const float*)divisor, (float*)out);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -871,7 +873,7 @@ int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int
   QSIM_CHECK_ARG(n >= 1 && nshots >= 0, "qsim_search_sorted: bad sizes");
   if (nshots == 0) return QSIM_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_search, dim3(ceil_div_i(uint64_t(nshots), kT)), dim3(kT), 0, s, cdf, n, uniforms, nshots, out);
+  hipLaunchKernelGGL(k_search, launch_grid(ceil_div_u(uint64_t(nshots), kT)), dim3(kT), 0, s, cdf, n, uniforms, nshots, out);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -883,7 +885,7 @@ int qsim_samples_to_binary(const int64_t* samples, int64_t nshots, int nbits, in
   if (nshots == 0) return QSIM_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const uint64_t total = uint64_t(nshots) * uint64_t(nbits);
-  hipLaunchKernelGGL(k_to_binary, dim3(ceil_div_i(total, kT)), dim3(kT), 0, s, samples, nshots, nbits, out);
+  hipLaunchKernelGGL(k_to_binary, launch_grid(ceil_div_u(total, kT)), dim3(kT), 0, s, samples, nshots, nbits, out);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -894,7 +896,7 @@ int qsim_histogram(const int64_t* samples, int64_t nshots, int64_t* counts, int6
   QSIM_CHECK_ARG(nbins >= 1 && nshots >= 0, "qsim_histogram: bad sizes");
   if (nshots == 0) return QSIM_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_histogram, dim3(ceil_div_i(uint64_t(nshots), kT)), dim3(kT), 0, s, samples, nshots,
+  hipLaunchKernelGGL(k_histogram, launch_grid(ceil_div_u(uint64_t(nshots), kT)), dim3(kT), 0, s, samples, nshots,
                      reinterpret_cast<unsigned long long*>(counts), nbins);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;

@@ -18,17 +18,20 @@ GROUPS = {0: (8, 9, 10, 11), 1: (4, 5, 6, 7), 2: (0, 1, 2, 3)}
 
 
 def decode(program: bytes, first: int, count: int, itemsize: int):
-    rec = blocking.HEADER_BYTES + blocking.NMAT * itemsize
+    """Parse ``count`` variable-length records starting at byte ``first``."""
     cdt = np.complex128 if itemsize == 16 else np.complex64
     out = []
-    for k in range(first, first + count):
-        raw = program[k * rec:(k + 1) * rec]
-        op, arg = np.frombuffer(raw[0:8], dtype="<u4")
-        cmask = int(np.frombuffer(raw[8:16], dtype="<u8")[0])
-        nt = int(np.frombuffer(raw[16:20], dtype="<u4")[0])
-        bits = [int(b) for b in np.frombuffer(raw[20:28], dtype=np.uint8)]
-        mat = np.frombuffer(raw[32:], dtype=cdt)
-        out.append((int(op), int(arg), cmask, nt, bits, mat))
+    off = first
+    for _ in range(count):
+        op, arg, nt, rreq = (int(x) for x in program[off: off + 4])
+        nent = int(np.frombuffer(program[off + 4: off + 8], dtype="<u4")[0])
+        cmask = int(np.frombuffer(program[off + 8: off + 16], dtype="<u8")[0])
+        bits = [int(b) for b in program[off + 16: off + 24]]
+        assert program[off + 24: off + 32] == bytes(8)
+        size = blocking.record_bytes(nent, itemsize)
+        mat = np.frombuffer(program[off + 32: off + 32 + nent * itemsize], dtype=cdt)
+        out.append((op, arg, cmask, nt, bits, rreq, mat))
+        off += size
     return out
 
 
@@ -47,18 +50,44 @@ def run_pass(state, n, tile_bits, instrs):
     gidx = base[:, None] | loc[None, :]          # (ntiles, 4096) global indices
     tile = state[gidx].copy()
     cur = 0
-    for op, arg, cmask, nt, bits, mat in instrs:
-        active = (gidx & cmask) == cmask
-        if op == blocking.OP_DIAG:
+    for op, arg, cmask, nt, bits, rreq, mat in instrs:
+        rb = [tile_bits[lb] for lb in GROUPS[cur]]     # global bit of register bit j
+        regmask = sum(1 << b for b in rb)
+        if op != blocking.OP_DENSE_LDS:
+            # the kernel's split: lane-fixed condition bits in cmask, register bits in rreq
+            assert cmask & regmask == 0
+            active = (gidx & cmask) == cmask
+            for j in range(4):
+                if (rreq >> j) & 1:
+                    active &= ((gidx >> rb[j]) & 1) == 1
+        else:
+            assert rreq == 0
+            active = (gidx & cmask) == cmask
+        if op in (blocking.OP_DIAG, blocking.OP_LANE_PHASE):
             di = np.zeros_like(gidx)
             for tb in bits[:nt]:
+                if tb & blocking.REG_FLAG:
+                    assert op == blocking.OP_DIAG
+                    tb = rb[tb & 3]
+                elif op == blocking.OP_LANE_PHASE:
+                    assert tb not in rb  # per-lane factor: no register bit may be involved
                 di = (di << 1) | ((gidx >> tb) & 1)
             tile = np.where(active, tile * mat[di], tile)
+        elif op == blocking.OP_NEG:
+            tile = np.where(active, -tile, tile)
+        elif op == blocking.OP_REG_PHASE:
+            assert cmask == 0 and rreq == 0
+            r = np.zeros_like(gidx)
+            for j in range(4):
+                r |= ((gidx >> rb[j]) & 1) << j
+            tile = tile * mat[r]
         elif op == blocking.OP_MAP:
             cur = arg
-        elif op == blocking.OP_DENSE1:
+        elif op in (blocking.OP_DENSE1, blocking.OP_DENSE1_REAL):
             lb = GROUPS[cur][arg]
             m = mat[:4].reshape(2, 2)
+            if op == blocking.OP_DENSE1_REAL:
+                assert not np.any(np.imag(m))
             i0 = u[(u >> lb) & 1 == 0]
             i1 = i0 | (1 << lb)
             x0, x1 = tile[:, i0], tile[:, i1]
@@ -66,6 +95,7 @@ def run_pass(state, n, tile_bits, instrs):
             tile[:, i0] = np.where(act, m[0, 0] * x0 + m[0, 1] * x1, x0)
             tile[:, i1] = np.where(act, m[1, 0] * x0 + m[1, 1] * x1, x1)
         else:
+            assert op == blocking.OP_DENSE_LDS
             k = arg
             lbits = bits[:k]
             d = 2**k
@@ -89,8 +119,8 @@ def run_pass(state, n, tile_bits, instrs):
     return out
 
 
-def run_plan(ops, n, psi, cdtype=np.complex128):
-    plan, program = blocking.compile_circuit(ops, n, cdtype, 3 if cdtype == np.complex128 else 4)
+def run_plan(ops, n, psi, cdtype=np.complex128, ordered=False):
+    plan, program = blocking.compile_circuit(ops, n, cdtype, 3 if cdtype == np.complex128 else 4, ordered)
     itemsize = np.dtype(cdtype).itemsize
     ob = OracleBackend()
     state = psi.copy()
@@ -175,3 +205,69 @@ def test_layered_30q_pass_count():
         if kind == "block":
             bits, pops = payload
             assert len(bits) == blocking.TILE_BITS and bits[:3] == [0, 1, 2]
+
+
+def test_ordered_partition_keeps_circuit_order():
+    n = 16
+    c = _circuits(n)[2]
+    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    order = []
+    for kind, payload in blocking.partition(ops, n, 3, ordered=True):
+        order.extend([payload] if kind == "single" else payload[1])
+    assert [id(o) for o in order] == [id(o) for o in ops if o.kind != "noop"]
+    for kind, payload in blocking.partition(ops, n, 3, ordered=True):
+        if kind != "block":
+            continue
+        bits, pops = payload
+        code, count, _ = blocking.compile_pass(bits, pops, n, np.complex128, ordered=True)
+        kinds = [r[0] for r in decode(code, 0, count, 16)]
+        assert set(kinds) <= {blocking.OP_DIAG, blocking.OP_DENSE1, blocking.OP_MAP, blocking.OP_DENSE_LDS}
+        gates_emitted = [k for k in kinds if k != blocking.OP_MAP]
+        assert len(gates_emitted) == len(pops)
+        for got, o in zip(gates_emitted, pops):  # same sequence of gate kinds
+            assert (got == blocking.OP_DIAG) == (o.kind == "diag")
+
+
+def test_ordered_mode_reproduces_oracle(rng):
+    n = 14
+    c = _circuits(n)[2]
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi /= np.linalg.norm(psi)
+    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    got, _ = run_plan(ops, n, psi, ordered=True)
+    want = OracleBackend().execute(c, psi)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_fast_forms_used_on_layered_circuit():
+    n = 30
+    c = models.random_layered_circuit(n, 20, seed=1, measure=False)
+    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    plan, program = blocking.compile_circuit(ops, n, np.complex128, 3)
+    kinds = {}
+    for entry in plan:
+        if entry[0] != "block":
+            continue
+        for rec in decode(program, entry[2], entry[3], 16):
+            kinds[rec[0]] = kinds.get(rec[0], 0) + 1
+    # RY is real: every register one-qubit gate uses the real form
+    assert kinds.get(blocking.OP_DENSE1, 0) == 0 and kinds[blocking.OP_DENSE1_REAL] > 0
+    # RZ / CZ fold into register tables (several gates per record), sign flips
+    # or per-lane factors; no general per-amplitude diagonal is left
+    assert kinds[blocking.OP_REG_PHASE] > 0 and kinds.get(blocking.OP_DIAG, 0) == 0
+    ngates = sum(e[4] for e in plan if e[0] == "block")
+    folded = ngates - kinds[blocking.OP_DENSE1_REAL] - kinds.get(blocking.OP_NEG, 0) \
+        - kinds.get(blocking.OP_LANE_PHASE, 0)
+    assert kinds[blocking.OP_REG_PHASE] < folded / 2
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_compiled_passes_reproduce_oracle_c64(which, rng):
+    n = 13
+    c = _circuits(n)[which]
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi = (psi / np.linalg.norm(psi)).astype(np.complex64)
+    ops = [GatePlanner("complex64").plan_gate(g) for g in c.queue]
+    got, _ = run_plan(ops, n, psi, cdtype=np.complex64)
+    want = OracleBackend().execute(c, psi.astype(np.complex128))
+    assert np.max(np.abs(got - want)) < 5e-6

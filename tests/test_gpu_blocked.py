@@ -1,8 +1,11 @@
 """Blocked multi-gate passes on the GPU (needs an MI355X).
 
-A blocked pass must give bitwise the same state as the one-gate-per-pass
-kernels (same fused multiply-adds in the same order), and the oracle's state
-within 1e-12 (complex128) / 5e-6 (complex64).
+With circuit order kept (``blocking_ordered``) a blocked pass must give
+bitwise the same state as the one-gate-per-pass kernels (same fused
+multiply-adds in the same order). With commuting gates reordered (the
+default) the rounding changes: tolerance 1e-14 absolute (complex128) / 1e-6
+(complex64) against the per-gate state, on normalised random states. Both
+match the oracle within 1e-12 (complex128) / 5e-6 (complex64).
 """
 
 import numpy as np
@@ -43,12 +46,14 @@ def _random_circuit(n, ngates, seed):
     return c
 
 
-def _run(backend, circuit, psi, blocking):
+def _run(backend, circuit, psi, blocking, ordered=False):
     backend.blocking = blocking
+    backend.blocking_ordered = ordered
     try:
         out = backend.execute_circuit(circuit, initial_state=psi).state().detach().cpu().numpy()
     finally:
         backend.blocking = True
+        backend.blocking_ordered = False
     return out
 
 
@@ -67,8 +72,10 @@ def test_blocked_bitwise_equals_per_gate_c128(backend, name, make, n, rng):
     psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
     psi /= np.linalg.norm(psi)
     per_gate = _run(backend, c, psi, False)
+    ordered = _run(backend, c, psi, True, ordered=True)
+    np.testing.assert_array_equal(ordered, per_gate)
     blocked = _run(backend, c, psi, True)
-    np.testing.assert_array_equal(blocked, per_gate)
+    np.testing.assert_allclose(blocked, per_gate, rtol=0, atol=1e-14)
     if n <= 15:
         want = OracleBackend().execute(c, psi)
         assert np.max(np.abs(blocked - want)) <= 1e-12
@@ -81,8 +88,10 @@ def test_blocked_bitwise_equals_per_gate_c64(backend64, name, make, rng):
     psi = (rng.normal(size=2**n) + 1j * rng.normal(size=2**n))
     psi = (psi / np.linalg.norm(psi)).astype(np.complex64)
     per_gate = _run(backend64, c, psi, False)
+    ordered = _run(backend64, c, psi, True, ordered=True)
+    np.testing.assert_array_equal(ordered, per_gate)
     blocked = _run(backend64, c, psi, True)
-    np.testing.assert_array_equal(blocked, per_gate)
+    np.testing.assert_allclose(blocked, per_gate, rtol=0, atol=1e-6)
     want = OracleBackend().execute(c, psi.astype(np.complex128))
     assert np.max(np.abs(blocked - want)) <= 5e-6
 
