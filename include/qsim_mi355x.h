@@ -106,11 +106,14 @@ int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, in
  * on a register bit of the current mapping, 2 change of register mapping,
  * 3 dense 2^k block on k <= 3 local tile bits, 4 real 2x2 on a register bit,
  * 5 16-entry register phase table, 6 per-lane diagonal factor, 7 sign flip.
- * The kernel stops at the first record that would extend past program_bytes
- * or carries fewer entries than its op needs. A program of ops 0-3 only gives
+ * The buffer must stay readable for program_bytes + QSIM_BLOCKED_PAD bytes
+ * (the kernel reads one header ahead and trusts each record's entry count,
+ * so no read leaves that range); records are read while they start inside
+ * program_bytes. A program of ops 0-3 only gives
  * bitwise the state of applying the same gates one by one with
  * qsim_apply_dense / qsim_apply_diag / qsim_apply_swap; ops 4-7 reorder
  * roundings. Requires nqubits >= 12. */
+#define QSIM_BLOCKED_PAD 1056
 size_t qsim_blocked_header_bytes(void);
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                        size_t program_bytes, int ninstr, void* stream);

@@ -4,7 +4,10 @@ Run ``python -m qibo_amd._build`` (or ``__graft_entry__.build()``). Every HIP
 source is compiled to an object with ``--offload-arch=gfx950`` and linked into
 one shared library next to this file, where the ctypes loader looks for it.
 ``measure.hip`` is compiled with ``-ffp-contract=off`` because its arithmetic
-must round exactly like numpy's.
+must round exactly like numpy's. ``blocked.hip`` keeps uniform branches
+unstructurized (its instruction loop switches on a uniform opcode; structurizing
+that switch keeps every case's inputs live through the others and costs a copy
+of the lane's amplitude registers per instruction).
 """
 
 import os
@@ -24,7 +27,7 @@ SOURCES = {
     "fused.hip": [],
     "measure.hip": ["-ffp-contract=off"],
     "expval.hip": [],
-    "blocked.hip": [],
+    "blocked.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"],
 }
 
 COMMON = [

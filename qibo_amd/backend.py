@@ -96,6 +96,16 @@ def _peel_controls(m: np.ndarray, qubits: Sequence[int]) -> Tuple[np.ndarray, li
 def plan_matrix(matrix: np.ndarray, targets: Sequence[int], controls: Sequence[int]) -> _Op:
     """Cheapest exact kernel form of ``matrix`` on ``targets`` under ``controls``."""
     matrix = np.asarray(matrix)
+    if matrix.shape == (2, 2):  # one target: the same result as the general path, without it
+        a, b, c, d = matrix.ravel().tolist()
+        t = targets[0]
+        if b != 0 or c != 0:
+            return _Op("dense", (t,), controls, matrix)
+        if a == 1:  # [[1, 0], [0, d]]: the target becomes a control
+            if d == 1:
+                return _Op("noop", (), tuple(controls) + (t,), None)
+            return _Op("diag", (), tuple(controls) + (t,), np.array([d], dtype=matrix.dtype))
+        return _Op("diag", (t,), controls, np.array([a, d], dtype=matrix.dtype))
     m, tg, extra = _peel_controls(matrix, targets)
     ctrl = list(controls) + extra
     dim = m.shape[0]
@@ -549,35 +559,37 @@ class MI355XBackend(GatePlanner):
         return state
 
     def _run_blocked(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        """Partition ``ops`` into passes and launch each pass as soon as it is
+        compiled, so the host compiles pass k+1 while the GPU runs pass k."""
         if not ops:
             return state
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         low = 3 if code == _lib.C128 else 4
-        plan, program = blocking.compile_circuit(ops, nqubits, cdtype, low, self.blocking_ordered)
-        prog = None
-        if program:
-            host = torch.frombuffer(bytearray(program), dtype=torch.uint8).pin_memory()
-            prog = host.to(state.device, non_blocking=True)
+        ordered = self.blocking_ordered
         stream = self._stream(state)
-        for entry in plan:
-            if entry[0] == "single":
-                self.apply_op(entry[1], state, nqubits)
+        cur = torch.cuda.current_stream(state.device)
+        for kind, payload in blocking.partition(ops, nqubits, low, ordered):
+            if kind == "single":
+                self.apply_op(payload, state, nqubits)
                 continue
-            _, tile_bits, first, count, ngates = entry
+            tile_bits, pops = payload
+            program, count, _ = blocking.compile_pass(tile_bits, pops, nqubits, cdtype, ordered)
+            # zero padding: the kernel may read QSIM_BLOCKED_PAD bytes past the stream
+            host = torch.frombuffer(bytearray(program) + bytes(blocking.PAD_BYTES), dtype=torch.uint8).pin_memory()
+            prog = host.to(state.device, non_blocking=True)
             timer = self._timer
             if timer is not None and timer[0](_BLOCK_OP):
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-                ev[0].record(torch.cuda.current_stream(state.device))
+                ev[0].record(cur)
             rc = self.lib.qsim_apply_blocked(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits),
-                                             prog.data_ptr() + first, len(program) - first, count, stream)
+                                             prog.data_ptr(), len(program), count, stream)
             _lib.check(rc, self.lib)
             if timer is not None and timer[0](_BLOCK_OP):
-                ev[1].record(torch.cuda.current_stream(state.device))
+                ev[1].record(cur)
                 timer[1].append(ev)
-            self.block_stats.append(ngates)
-        if prog is not None:
             _keep_alive(prog, state)
+            self.block_stats.append(len(pops))
         return state
 
     def execute_circuits(self, circuits, initial_states=None, nshots=1000, processes=None):

@@ -47,13 +47,14 @@ lane, ``rreq`` the register bits (as a 4-bit mask over register positions);
 DENSE_LDS keeps all condition bits in ``cmask``.
 """
 
-from typing import List, Sequence, Tuple
+from typing import Iterator, Sequence, Tuple
 
 import numpy as np
 
 TILE_BITS = 12
 HEADER_BYTES = 32
 NMAT = 64  # largest number of complex entries of a record (8x8 block)
+PAD_BYTES = 1056  # readable bytes required past a record stream (QSIM_BLOCKED_PAD)
 
 OP_DIAG, OP_DENSE1, OP_MAP, OP_DENSE_LDS = 0, 1, 2, 3
 OP_DENSE1_REAL, OP_REG_PHASE, OP_LANE_PHASE, OP_NEG = 4, 5, 6, 7
@@ -85,19 +86,19 @@ def blockable(op) -> bool:
         op.kind != "diag" or len(op.targets) <= 5)
 
 
-def partition(ops: Sequence, nqubits: int, low_bits: int, ordered: bool = False) -> List[Tuple[str, object]]:
+def partition(ops: Sequence, nqubits: int, low_bits: int, ordered: bool = False) -> Iterator[Tuple[str, object]]:
     """Split ``ops`` (planned gates, circuit order) into passes.
 
-    Returns a list of ("single", op) and ("block", (tile_bits, [ops])) items
-    whose sequential execution equals the circuit. With ``ordered`` a pass
-    ends at the first gate that does not fit (no gate moves ahead).
+    Yields ("single", op) and ("block", (tile_bits, [ops])) items whose
+    sequential execution equals the circuit, one at a time (the caller can
+    launch a pass before the next is formed). With ``ordered`` a pass ends at
+    the first gate that does not fit (no gate moves ahead).
     """
     items = [_Item(op, nqubits, i) for i, op in enumerate(ops) if op.kind != "noop"]
-    out = []
     while items:
         first = items[0]
         if not blockable(first.op) or nqubits < TILE_BITS:
-            out.append(("single", first.op))
+            yield ("single", first.op)
             items = items[1:]
             continue
         tile = set(range(low_bits))
@@ -116,13 +117,12 @@ def partition(ops: Sequence, nqubits: int, low_bits: int, ordered: bool = False)
                 ad |= it.all_bits
         items = deferred
         if len(chosen) == 1:
-            out.append(("single", chosen[0].op))
+            yield ("single", chosen[0].op)
             continue
         bits = sorted(tile)
         extra = [b for b in range(nqubits) if b not in tile]
         bits = sorted(bits + extra[: TILE_BITS - len(bits)])
-        out.append(("block", (bits, [c.op for c in chosen])))
-    return out
+        yield ("block", (bits, [c.op for c in chosen]))
 
 
 def record_bytes(nent: int, itemsize: int) -> int:
@@ -152,6 +152,49 @@ def _pack(buf: bytearray, op: int, arg: int, cmask: int, nt: int, bits: Sequence
     buf.extend(rec)
 
 
+def _dependencies(items, ordered):
+    """Dependency graph of a pass: (predecessor counts, successor lists).
+
+    Two items conflict when some bit is a non-diagonal target of one and
+    touched by the other. Per bit it suffices to order a non-diagonal user
+    after the last non-diagonal user and after every diagonal user since, and
+    a diagonal user after the last non-diagonal user: the transitive closure
+    is the full conflict relation, so the topological orders are the same.
+    ``ordered`` chains the items in their given order."""
+    n = len(items)
+    succ = [[] for _ in range(n)]
+    npred = [0] * n
+    if ordered:
+        for i in range(n - 1):
+            succ[i].append(i + 1)
+            npred[i + 1] = 1
+        return npred, succ
+    last_nd = {}   # bit -> index of the last non-diagonal user
+    diag_since = {}  # bit -> diagonal users after it
+    for j, it in enumerate(items):
+        preds = set()
+        for b in it.all_bits:
+            if b in it.nd_targets:
+                preds.update(diag_since.get(b, ()))
+                if b in last_nd:
+                    preds.add(last_nd[b])
+            elif b in last_nd:
+                preds.add(last_nd[b])
+        for b in it.all_bits:
+            if b in it.nd_targets:
+                last_nd[b] = j
+                diag_since[b] = []
+            else:
+                diag_since.setdefault(b, []).append(j)
+        for i in preds:
+            succ[i].append(j)
+        npred[j] = len(preds)
+    return npred, succ
+
+
+_RVALS = np.array([[(r >> j) & 1 for r in range(16)] for j in range(4)], dtype=np.int64)  # [j, r]
+
+
 def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
                  ordered: bool = False) -> Tuple[bytes, int, int]:
     """Instruction records of one blocked pass; returns (bytes, count, map switches).
@@ -160,13 +203,7 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
     the exact per-gate forms (DIAG, DENSE1, DENSE_LDS, MAP)."""
     local = {b: i for i, b in enumerate(tile_bits)}
     items = [_Item(op, nqubits, i) for i, op in enumerate(ops)]
-    npred = [0] * len(items)
-    succ = [[] for _ in items]
-    for j, y in enumerate(items):
-        for i in range(j):
-            if ordered or not _commute(items[i], y):
-                npred[j] += 1
-                succ[i].append(j)
+    npred, succ = _dependencies(items, ordered)
     ready = [i for i in range(len(items)) if npred[i] == 0]
     buf = bytearray()
     count = 0
@@ -221,14 +258,13 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
                 return
             if all(b in rb for b in tb + cb):
                 table = np.ones(16, dtype=np.complex128) if pending is None else pending
-                for r in range(16):
-                    val = {b: (r >> j) & 1 for j, b in enumerate(rb)}
-                    if all(val[c] for c in cb):
-                        di = 0
-                        for b in tb:
-                            di = (di << 1) | val[b]
-                        table[r] = table[r] * d[di]
-                pending = table
+                ok = np.ones(16, dtype=bool)
+                for c in cb:
+                    ok &= _RVALS[rb.index(c)] == 1
+                di = np.zeros(16, dtype=np.int64)
+                for b in tb:
+                    di = (di << 1) | _RVALS[rb.index(b)]
+                pending = np.where(ok, table * d[di], table)
                 return
         cm, rreq = split_cond(op, cur)
         enc = [(REG_FLAG | rb.index(b)) if b in rb else b for b in tb]

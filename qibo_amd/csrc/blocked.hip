@@ -30,9 +30,11 @@
 //                   next MAP / DENSE_LDS / store (it commutes with everything
 //                   that stays inside the lane's registers)
 //   NEG (7)         v = -v (the -1 phase of CZ-like gates)
-// Conditions: `cmask` = condition bits fixed per lane (DENSE_LDS: all condition
-// bits); `rreq` = register-bit conditions as a 4-bit mask over register
-// positions. DIAG / DENSE1 / DENSE_LDS / MAP perform exactly the fused
+// Conditions: `tmask` = condition bits outside the tile (the same for the
+// whole tile: a record whose tmask fails is skipped by the workgroup);
+// `cmask` = condition bits inside the tile that select lanes (DENSE_LDS: all
+// condition bits); `rreq` = register-bit conditions as a 4-bit mask over
+// register positions. DIAG / DENSE1 / DENSE_LDS / MAP perform exactly the fused
 // multiply-adds of the one-gate kernels, so a pass built only from them is
 // bitwise equal to applying its gates one by one; the other forms change the
 // order of roundings.
@@ -72,10 +74,10 @@ struct BlkHdr {
   uint8_t nt;        // DIAG / LANE_PHASE: number of target bits (0..5)
   uint8_t rreq;      // register-bit conditions (bit j: register bit j must be 1)
   uint32_t nent;     // complex entries following the header
-  uint64_t cmask;    // condition bits fixed per lane (DENSE_LDS: all condition bits), all must be 1
+  uint64_t cmask;    // lane-selecting condition bits (DENSE_LDS: all condition bits), all must be 1
   uint8_t bits[8];   // DIAG / LANE_PHASE: target bits, t0 = MSB of the diagonal index
                      // DENSE_LDS: [0..k) local target bits in matrix order, [4..4+k) the same ascending
-  uint64_t reserved;
+  uint64_t tmask;    // condition bits outside the tile, all must be 1
 };
 static_assert(sizeof(BlkHdr) == 32, "record header is 32 bytes");
 // Entries: DIAG / LANE_PHASE 2^nt; DENSE1*: 2x2 row-major; REG_PHASE: 16;
@@ -88,18 +90,40 @@ struct BlkArgs {
   uint8_t tbits[kBTileBits];  // global bit of local tile bit i, ascending
 };
 
-// Complex entries a record of this header must carry.
-__device__ __forceinline__ uint32_t needed_entries(const BlkHdr& h) {
-  switch (h.op) {
-    case kOpDiag:
-    case kOpLanePhase: return 1u << (h.nt < 5 ? h.nt : 5);
-    case kOpDense1:
-    case kOpDense1Real: return 4;
-    case kOpRegPhase: return 16;
-    case kOpDenseLds: return h.arg <= 1 ? 4u : (h.arg == 2 ? 16u : 64u);
-    default: return 0;
-  }
+// A header decoded from its eight dwords. gfx950 has no scalar byte loads, so
+// the header is read as whole dwords (uniform address: scalar loads) and the
+// byte fields are extracted with scalar shifts.
+struct Hdr {
+  uint32_t op, arg, nt, rreq, nent;
+  uint64_t cmask, tmask;
+  uint32_t b0, b1;  // bits[0..3], bits[4..7]
+};
+
+__device__ __forceinline__ Hdr load_hdr(const uint8_t* p) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+  const uint32_t w0 = w[0];
+  Hdr h;
+  h.op = w0 & 255u;
+  h.arg = (w0 >> 8) & 255u;
+  h.nt = (w0 >> 16) & 255u;
+  h.rreq = w0 >> 24;
+  h.nent = w[1];
+  h.cmask = uint64_t(w[2]) | (uint64_t(w[3]) << 32);
+  h.b0 = w[4];
+  h.b1 = w[5];
+  h.tmask = uint64_t(w[6]) | (uint64_t(w[7]) << 32);
+  return h;
 }
+
+__device__ __forceinline__ uint32_t hbit(const Hdr& h, uint32_t t) {
+  return ((t < 4 ? h.b0 : h.b1) >> (8 * (t & 3))) & 255u;
+}
+
+// Readable bytes the caller guarantees past the end of the record stream: the
+// largest record (header + 64 complex128 entries) fits in it, so the
+// read-ahead of one header and the entry reads of a record that starts inside
+// the stream never leave the allocation.
+constexpr uint64_t kBlockedPad = 1056;
 
 __device__ __forceinline__ int reg_base(int map) { return map == 0 ? 8 : (map == 1 ? 4 : 0); }
 
@@ -122,16 +146,44 @@ __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
   return g;
 }
 
+// The workgroup's LDS buffer holds one real component of the tile (4096
+// values: 32 KB in complex128, 16 KB in complex64), so a transpose moves the
+// real parts, then the imaginary parts. Half the LDS of a whole complex tile
+// lets twice as many workgroups share a CU.
 template <typename R>
-__device__ __forceinline__ void to_lds(cplx<R>* lds, const cplx<R>* v, int map) {
+__device__ __forceinline__ void lds_put(R* lds, const R* x, int map) {
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) lds[swz(local_index(map, threadIdx.x, r))] = v[r];
+  for (int r = 0; r < kBReg; ++r) lds[swz(local_index(map, threadIdx.x, r))] = x[r];
 }
 
 template <typename R>
-__device__ __forceinline__ void from_lds(const cplx<R>* lds, cplx<R>* v, int map) {
+__device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) v[r] = lds[swz(local_index(map, threadIdx.x, r))];
+  for (int r = 0; r < kBReg; ++r) x[r] = lds[swz(local_index(map, threadIdx.x, r))];
+}
+
+// Change of register mapping: amplitude (lane, r) of mapping `from` moves to
+// its place under mapping `to`, through LDS, one component at a time.
+template <typename R>
+__device__ __forceinline__ void remap(R* lds, cplx<R>* v, int from, int to) {
+  R x[kBReg];
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) x[r] = v[r].re;
+  lds_put(lds, x, from);
+  __syncthreads();
+  lds_get(lds, x, to);
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    v[r].re = x[r];
+    x[r] = v[r].im;
+  }
+  lds_put(lds, x, from);
+  __syncthreads();
+  lds_get(lds, x, to);
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) v[r].im = x[r];
 }
 
 // Register r satisfies the register-bit conditions `rreq` (r is a constant
@@ -141,6 +193,53 @@ __device__ __forceinline__ bool reg_ok(int r, uint32_t rreq) { return (uint32_t(
 // One-qubit gate on register bit J of the current mapping: the arithmetic of
 // k_dense (cmul then cfma) for DENSE1; two real multiplies per component for
 // DENSE1_REAL.
+// In-place kernels of the two hot forms. The values are updated in their own
+// registers (tied asm operands), with the roundings of the C++ forms below
+// (v_mul then v_fma, exactly fmaR(b, x1, a * x0)); written this way because
+// the compiler otherwise routes the results through v_fmac temporaries and
+// copies the lane's amplitude array at the instruction loop's merge points.
+__device__ __forceinline__ void real2_inplace(double& x0, double& x1, double a, double b, double c, double d) {
+  double t0, t1;
+  asm("v_mul_f64 %[t0], %[a], %[x0]\n\t"
+      "v_mul_f64 %[t1], %[c], %[x0]\n\t"
+      "v_fma_f64 %[x0], %[b], %[x1], %[t0]\n\t"
+      "v_fma_f64 %[x1], %[d], %[x1], %[t1]"
+      : [x0] "+v"(x0), [x1] "+v"(x1), [t0] "=&v"(t0), [t1] "=&v"(t1)
+      : [a] "s"(a), [b] "s"(b), [c] "s"(c), [d] "s"(d));
+}
+
+__device__ __forceinline__ void real2_inplace(float& x0, float& x1, float a, float b, float c, float d) {
+  float t0, t1;
+  asm("v_mul_f32 %[t0], %[a], %[x0]\n\t"
+      "v_mul_f32 %[t1], %[c], %[x0]\n\t"
+      "v_fma_f32 %[x0], %[b], %[x1], %[t0]\n\t"
+      "v_fma_f32 %[x1], %[d], %[x1], %[t1]"
+      : [x0] "+v"(x0), [x1] "+v"(x1), [t0] "=&v"(t0), [t1] "=&v"(t1)
+      : [a] "s"(a), [b] "s"(b), [c] "s"(c), [d] "s"(d));
+}
+
+// x *= d with d uniform: exactly cmul(d, x) (re = fma(d.re, x.re, -(d.im x.im)),
+// im = fma(d.re, x.im, d.im x.re)).
+__device__ __forceinline__ void cmul_inplace(cplx<double>& x, cplx<double> d) {
+  double t1, t2;
+  asm("v_mul_f64 %[t1], %[di], %[xi]\n\t"
+      "v_mul_f64 %[t2], %[di], %[xr]\n\t"
+      "v_fma_f64 %[xr], %[dr], %[xr], -%[t1]\n\t"
+      "v_fma_f64 %[xi], %[dr], %[xi], %[t2]"
+      : [xr] "+v"(x.re), [xi] "+v"(x.im), [t1] "=&v"(t1), [t2] "=&v"(t2)
+      : [dr] "s"(d.re), [di] "s"(d.im));
+}
+
+__device__ __forceinline__ void cmul_inplace(cplx<float>& x, cplx<float> d) {
+  float t1, t2;
+  asm("v_mul_f32 %[t1], %[di], %[xi]\n\t"
+      "v_mul_f32 %[t2], %[di], %[xr]\n\t"
+      "v_fma_f32 %[xr], %[dr], %[xr], -%[t1]\n\t"
+      "v_fma_f32 %[xi], %[dr], %[xi], %[t2]"
+      : [xr] "+v"(x.re), [xi] "+v"(x.im), [t1] "=&v"(t1), [t2] "=&v"(t2)
+      : [dr] "s"(d.re), [di] "s"(d.im));
+}
+
 template <typename R, bool REAL>
 __device__ __forceinline__ void dense1_pair(cplx<R>& x0r, cplx<R>& x1r, cplx<R> m0, cplx<R> m1, cplx<R> m2,
                                             cplx<R> m3) {
@@ -156,33 +255,52 @@ __device__ __forceinline__ void dense1_pair(cplx<R>& x0r, cplx<R>& x1r, cplx<R> 
   }
 }
 
+template <typename R>
+__device__ __forceinline__ cplx<R> csel(bool c, cplx<R> a, cplx<R> b) {
+  cplx<R> r;
+  r.re = c ? a.re : b.re;
+  r.im = c ? a.im : b.im;
+  return r;
+}
+
+// Lane conditions are applied with selects, not branches: a divergent branch
+// around register updates makes the compiler copy the whole register array.
 template <typename R, int J, bool REAL>
-__device__ __forceinline__ void dense1_regs(cplx<R>* v, const BlkHdr& h, const cplx<R>* m, bool lane_ok) {
+__device__ __forceinline__ void dense1_regs(cplx<R>* v, const Hdr& h, const cplx<R>* m, bool lane_ok) {
   const cplx<R> m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
   const uint32_t rreq = h.rreq;
-  if (!lane_ok) return;
-  if (rreq == 0) {  // no register-bit conditions: straight-line code
+  if (rreq == 0 && h.cmask == 0) {  // unconditional: straight-line code
 #pragma clang loop unroll(full)
-    for (int r = 0; r < kBReg; ++r)
-      if (!((r >> J) & 1)) dense1_pair<R, REAL>(v[r], v[r | (1 << J)], m0, m1, m2, m3);
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      if constexpr (REAL) {
+        real2_inplace(v[r].re, v[r | (1 << J)].re, m0.re, m1.re, m2.re, m3.re);
+        real2_inplace(v[r].im, v[r | (1 << J)].im, m0.re, m1.re, m2.re, m3.re);
+      } else {
+        dense1_pair<R, REAL>(v[r], v[r | (1 << J)], m0, m1, m2, m3);
+      }
+    }
     return;
   }
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) {
     if ((r >> J) & 1) continue;
     if (!reg_ok(r, rreq)) continue;
-    dense1_pair<R, REAL>(v[r], v[r | (1 << J)], m0, m1, m2, m3);
+    cplx<R> a = v[r], b = v[r | (1 << J)];
+    dense1_pair<R, REAL>(a, b, m0, m1, m2, m3);
+    v[r] = csel(lane_ok, a, v[r]);
+    v[r | (1 << J)] = csel(lane_ok, b, v[r | (1 << J)]);
   }
 }
 
 // Diagonal index contributions of one DIAG / LANE_PHASE record: `dl` from the
 // lane's fixed bits, rm[j] from register bit j (uniform).
-__device__ __forceinline__ uint32_t diag_lane_part(const BlkHdr& h, uint64_t lane_idx, uint32_t* rm) {
+__device__ __forceinline__ uint32_t diag_lane_part(const Hdr& h, uint64_t lane_idx, uint32_t* rm) {
   const uint32_t nt = h.nt < 5 ? h.nt : 5;
   uint32_t dl = 0;
   rm[0] = rm[1] = rm[2] = rm[3] = 0;
   for (uint32_t t = 0; t < nt; ++t) {
-    const uint32_t b = h.bits[t];
+    const uint32_t b = hbit(h, t);
     const uint32_t w = 1u << (nt - 1 - t);
     if (b & kRegFlag) {
       rm[0] |= (b & 3) == 0 ? w : 0u;
@@ -197,18 +315,28 @@ __device__ __forceinline__ uint32_t diag_lane_part(const BlkHdr& h, uint64_t lan
 }
 
 template <typename R>
-__device__ __forceinline__ void diag_regs(cplx<R>* v, const BlkHdr& h, const cplx<R>* m, uint64_t lane_idx,
+__device__ __forceinline__ void diag_regs(cplx<R>* v, const Hdr& h, const cplx<R>* m, uint64_t lane_idx,
                                           bool lane_ok) {
   uint32_t rm[4];
   const uint32_t dl = diag_lane_part(h, lane_idx, rm);
   const uint32_t rreq = h.rreq;
-  if (!lane_ok) return;
+  if (h.nt <= 1) {  // entries are uniform: scalar loads and a per-register select
+    const cplx<R> d0 = m[0], d1 = m[h.nt];
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if (!reg_ok(r, rreq)) continue;
+      const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
+                          ((r & 8) ? rm[3] : 0u);
+      v[r] = csel(lane_ok, cmul(di ? d1 : d0, v[r]), v[r]);
+    }
+    return;
+  }
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) {
     if (!reg_ok(r, rreq)) continue;
     const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
                         ((r & 8) ? rm[3] : 0u);
-    v[r] = cmul(m[di], v[r]);
+    v[r] = csel(lane_ok, cmul(m[di], v[r]), v[r]);
   }
 }
 
@@ -222,52 +350,106 @@ __device__ __forceinline__ void flush_lane_phase(cplx<R>* v, cplx<R>& P, bool& p
   pdirty = false;
 }
 
-// Dense 2^k block (k = 1..3) on local bits, on the tile held in LDS.
+// Dense 2^k block (k = 1..3) on local bits. Each lane owns 16 / 2^k groups of
+// 2^k amplitudes (group g = lane + 256 i); the groups are gathered from the
+// tile through LDS one component at a time, multiplied in registers, and
+// scattered back the same way.
 template <typename R, int K>
-__device__ __forceinline__ void dense_lds(cplx<R>* lds, const BlkHdr& in, const cplx<R>* m, uint64_t tile_base,
-                                          const uint64_t* gbit) {
+__device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr& in, const cplx<R>* m,
+                                          uint64_t tile_base, const uint64_t* gbit) {
   constexpr int D = 1 << K;
-  constexpr int kGroups = kBTile >> K;
-  uint8_t srt[K];
+  constexpr int NG = (kBTile >> K) / kBT;  // groups per lane
+  uint32_t srt[K];
   uint32_t off[D];
 #pragma clang loop unroll(full)
-  for (int t = 0; t < K; ++t) srt[t] = in.bits[4 + t] < kBTileBits ? in.bits[4 + t] : kBTileBits - 1;
+  for (int t = 0; t < K; ++t) srt[t] = hbit(in, 4 + t) < kBTileBits ? hbit(in, 4 + t) : kBTileBits - 1;
 #pragma clang loop unroll(full)
   for (int r = 0; r < D; ++r) {
     uint32_t o = 0;
 #pragma clang loop unroll(full)
     for (int t = 0; t < K; ++t) {
-      const uint32_t b = in.bits[t] < kBTileBits ? in.bits[t] : kBTileBits - 1;
+      const uint32_t b = hbit(in, t) < kBTileBits ? hbit(in, t) : kBTileBits - 1;
       if ((r >> (K - 1 - t)) & 1) o |= 1u << b;
     }
     off[r] = o;
   }
-  const uint64_t cmask = in.cmask;
-  for (int g = threadIdx.x; g < kGroups; g += kBT) {
-    uint32_t base = g;
+  uint32_t base[NG];
+  bool act[NG];
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i) {
+    uint32_t b = threadIdx.x + i * kBT;
 #pragma clang loop unroll(full)
     for (int t = 0; t < K; ++t) {
-      const uint32_t low = base & ((1u << srt[t]) - 1);
-      base = low | ((base ^ low) << 1);
+      const uint32_t low = b & ((1u << srt[t]) - 1);
+      b = low | ((b ^ low) << 1);
     }
-    if (((tile_base | deposit(base, gbit)) & cmask) != cmask) continue;
-    cplx<R> x[D];
+    base[i] = b;
+    act[i] = ((tile_base | deposit(b, gbit)) & in.cmask) == in.cmask;
+  }
+  uint32_t slot[kBReg];  // the lane's amplitudes in the current mapping
 #pragma clang loop unroll(full)
-    for (int c = 0; c < D; ++c) x[c] = lds[swz(base | off[c])];
+  for (int r = 0; r < kBReg; ++r) slot[r] = swz(local_index(map, threadIdx.x, r));
+  cplx<R> g[kBReg];  // g[i * D + c]: input c of group i
+  // gather: real parts, then imaginary parts
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) lds[slot[r]] = v[r].re;
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i)
+#pragma clang loop unroll(full)
+    for (int c = 0; c < D; ++c) g[i * D + c].re = lds[swz(base[i] | off[c])];
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) lds[slot[r]] = v[r].im;
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i)
+#pragma clang loop unroll(full)
+    for (int c = 0; c < D; ++c) g[i * D + c].im = lds[swz(base[i] | off[c])];
+  __syncthreads();
+  // multiply (the arithmetic of the one-gate kernels: cmul, then cfma in column order)
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i) {
+    if (!act[i]) continue;
+    cplx<R> y[D];
 #pragma clang loop unroll(full)
     for (int r = 0; r < D; ++r) {
-      cplx<R> acc = cmul(m[r * D], x[0]);
+      cplx<R> acc = cmul(m[r * D], g[i * D]);
 #pragma clang loop unroll(full)
-      for (int c = 1; c < D; ++c) acc = cfma(m[r * D + c], x[c], acc);
-      lds[swz(base | off[r])] = acc;
+      for (int c = 1; c < D; ++c) acc = cfma(m[r * D + c], g[i * D + c], acc);
+      y[r] = acc;
     }
+#pragma clang loop unroll(full)
+    for (int r = 0; r < D; ++r) g[i * D + r] = y[r];
   }
+  // scatter back and reload the lane's amplitudes in the current mapping
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i)
+#pragma clang loop unroll(full)
+    for (int c = 0; c < D; ++c) lds[swz(base[i] | off[c])] = g[i * D + c].re;
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) v[r].re = lds[slot[r]];
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i)
+#pragma clang loop unroll(full)
+    for (int c = 0; c < D; ++c) lds[swz(base[i] | off[c])] = g[i * D + c].im;
+  __syncthreads();
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) v[r].im = lds[slot[r]];
+  __syncthreads();
 }
 
+#ifndef QSIM_BLOCKED_WAVES
+#define QSIM_BLOCKED_WAVES 3
+#endif
+
 template <typename R>
-__global__ __launch_bounds__(kBT) void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_BLOCKED_WAVES)))
+void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
                                                  cplx<R>* __restrict__ psi) {
-  __shared__ cplx<R> lds[kBTile];
+  __shared__ R lds[kBTile];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   uint64_t gbit[kBTileBits];
@@ -288,14 +470,26 @@ __global__ __launch_bounds__(kBT) void k_blocked(const BlkArgs a, const uint8_t*
   P.im = R(0);
   bool pdirty = false;
 
+  // The record stream is read ahead by one header. The buffer is readable for
+  // nbytes + kBlockedPad bytes, so a header or entry read that starts inside
+  // nbytes stays inside the allocation whatever the record claims.
+  const uint64_t nbytes = __builtin_amdgcn_readfirstlane(uint32_t(a.nbytes)) |
+                          (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(a.nbytes >> 32))) << 32);
+  const uint64_t tb_lo = __builtin_amdgcn_readfirstlane(uint32_t(tile_base));
+  const uint64_t tb_hi = __builtin_amdgcn_readfirstlane(uint32_t(tile_base >> 32));
+  const uint64_t tile_base_s = tb_lo | (tb_hi << 32);  // uniform copy (scalar registers)
   uint64_t off = 0;
-  for (int k = 0; k < a.ninstr; ++k) {
-    if (off + sizeof(BlkHdr) > a.nbytes) break;
-    const BlkHdr& h = *reinterpret_cast<const BlkHdr*>(prog + off);
+  Hdr h = {};
+  if (a.ninstr > 0) h = load_hdr(prog);
+  for (int k = 0; k < a.ninstr && off + sizeof(BlkHdr) <= nbytes; ++k) {
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
-    const uint32_t nent = h.nent;
-    off += sizeof(BlkHdr) + ((uint64_t(nent) * sizeof(cplx<R>) + 15) & ~uint64_t(15));
-    if (off > a.nbytes || nent < needed_entries(h)) break;  // malformed program: stop reading it
+    off += sizeof(BlkHdr) + ((uint64_t(h.nent) * sizeof(cplx<R>) + 15) & ~uint64_t(15));
+    const uint64_t next = off < nbytes ? off : nbytes;  // stays inside the padded buffer
+    const Hdr hn = load_hdr(prog + next);  // next header, in flight while this record runs
+    if ((tile_base_s & h.tmask) != h.tmask) {  // condition fails on the whole tile
+      h = hn;
+      continue;
+    }
     const bool lane_ok = (lane_idx & h.cmask) == h.cmask;
     switch (h.op) {
       case kOpDense1Real:
@@ -316,24 +510,29 @@ __global__ __launch_bounds__(kBT) void k_blocked(const BlkArgs a, const uint8_t*
         break;
       case kOpRegPhase:
 #pragma clang loop unroll(full)
-        for (int r = 0; r < kBReg; ++r) v[r] = cmul(m[r], v[r]);
+        for (int r = 0; r < kBReg; ++r) cmul_inplace(v[r], m[r]);
         break;
       case kOpLanePhase: {
         uint32_t rm[4];
         const uint32_t di = diag_lane_part(h, lane_idx, rm);  // no register bits: rm stays 0
-        if (lane_ok) P = cmul(m[di], P);
+        if (h.nt <= 1) {  // uniform entries: scalar loads and a per-lane select
+          const cplx<R> d0 = m[0], d1 = m[h.nt];
+          P = csel(lane_ok, cmul(di ? d1 : d0, P), P);
+        } else {
+          P = csel(lane_ok, cmul(m[di], P), P);
+        }
         pdirty = true;
         break;
       }
       case kOpNeg: {
         const uint32_t rreq = h.rreq;
-        if (lane_ok) {
 #pragma clang loop unroll(full)
-          for (int r = 0; r < kBReg; ++r) {
-            if (!reg_ok(r, rreq)) continue;
-            v[r].re = -v[r].re;
-            v[r].im = -v[r].im;
-          }
+        for (int r = 0; r < kBReg; ++r) {
+          if (!reg_ok(r, rreq)) continue;
+          cplx<R> nv;
+          nv.re = -v[r].re;
+          nv.im = -v[r].im;
+          v[r] = csel(lane_ok, nv, v[r]);
         }
         break;
       }
@@ -343,41 +542,32 @@ __global__ __launch_bounds__(kBT) void k_blocked(const BlkArgs a, const uint8_t*
       case kOpMap: {
         flush_lane_phase(v, P, pdirty);
         const int nmap = h.arg < 3 ? int(h.arg) : 0;
-        to_lds(lds, v, map);
-        __syncthreads();
+        remap(lds, v, map, nmap);
         map = nmap;
-        from_lds(lds, v, map);
-        __syncthreads();
         lane_idx = tile_base | deposit(local_index(map, threadIdx.x, 0), gbit);
         break;
       }
       case kOpDenseLds: {
         flush_lane_phase(v, P, pdirty);
-        to_lds(lds, v, map);
-        __syncthreads();
         const uint32_t kk = h.arg;
         if (kk <= 1)
-          dense_lds<R, 1>(lds, h, m, tile_base, gbit);
+          dense_lds<R, 1>(lds, v, map, h, m, tile_base, gbit);
         else if (kk == 2)
-          dense_lds<R, 2>(lds, h, m, tile_base, gbit);
+          dense_lds<R, 2>(lds, v, map, h, m, tile_base, gbit);
         else
-          dense_lds<R, 3>(lds, h, m, tile_base, gbit);
-        __syncthreads();
-        from_lds(lds, v, map);
-        __syncthreads();
+          dense_lds<R, 3>(lds, v, map, h, m, tile_base, gbit);
         break;
       }
       default:
         break;
     }
+    h = hn;
   }
 
   flush_lane_phase(v, P, pdirty);
   if (map != 0) {
-    to_lds(lds, v, map);
-    __syncthreads();
+    remap(lds, v, map, 0);
     map = 0;
-    from_lds(lds, v, map);
     lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
   }
 #pragma clang loop unroll(full)
@@ -415,6 +605,7 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
   QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, "qsim_apply_blocked: nqubits out of range");
   QSIM_CHECK_ARG(ninstr >= 0 && (ninstr == 0 || (program_dev != nullptr && program_bytes >= sizeof(BlkHdr))),
                  "qsim_apply_blocked: bad program");
+  static_assert(kBlockedPad == QSIM_BLOCKED_PAD, "padding contract of the C ABI");
   QSIM_CHECK_ARG((reinterpret_cast<uintptr_t>(program_dev) & 15) == 0, "qsim_apply_blocked: program not 16-byte aligned");
   for (int i = 0; i < kBTileBits; ++i) {
     QSIM_CHECK_ARG(tile_bits[i] >= 0 && tile_bits[i] < nqubits, "qsim_apply_blocked: tile bit out of range");

@@ -199,7 +199,7 @@ def test_partition_respects_order():
 def test_layered_30q_pass_count():
     c = models.random_layered_circuit(30, 20, seed=1, measure=False)
     ops = [GatePlanner().plan_gate(g) for g in c.queue]
-    parts = blocking.partition(ops, 30, 3)
+    parts = list(blocking.partition(ops, 30, 3))
     assert len(parts) <= 24  # 1490 gates in a couple of dozen state traversals
     for kind, payload in parts:
         if kind == "block":
@@ -271,3 +271,14 @@ def test_compiled_passes_reproduce_oracle_c64(which, rng):
     got, _ = run_plan(ops, n, psi, cdtype=np.complex64)
     want = OracleBackend().execute(c, psi.astype(np.complex128))
     assert np.max(np.abs(got - want)) < 5e-6
+
+
+def test_padding_contract_matches_header():
+    import os
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "qsim_mi355x.h")).read()
+    pad = int(re.search(r"#define QSIM_BLOCKED_PAD (\d+)", hdr).group(1))
+    assert pad == blocking.PAD_BYTES
+    # the largest record's entries, read from a record that starts inside the stream
+    assert pad >= blocking.record_bytes(blocking.NMAT, 16)
