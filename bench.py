@@ -18,10 +18,11 @@ sample of the same circuit's gates on this host.
 
 Beside the headline line the same JSON object carries the other parts of
 the metric, each on its own bounded workload:
-* "fused_c64": the layered generator at 32 qubits in complex64 run unfused,
-  fused with circuit.fuse(max_qubits=5) (5-qubit blocks on MFMA), and in
-  complex128 (64 GiB state); infidelity and max-abs amplitude error of the
-  fused complex64 state against complex128 at 30 qubits;
+* "fused_c64": the layered circuit at 32 qubits in complex64 and complex128
+  (64 GiB state) through blocked passes; 24 dense 5-qubit blocks at 32 qubits
+  (complex64 on MFMA, complex128 in FP64) with ms per block, HBM GB/s and
+  TFLOP/s; infidelity and max-abs amplitude error of the fused complex64 state
+  against complex128 at 30 qubits;
 * "serving": a Poisson stream of layered circuits (n ~ U[10, 30], depth ~
   U[10, 50], half complex64, half <sum Z> / shots) through the continuous-
   batching server: circuits/s, p50 and p99 latency;
@@ -106,8 +107,35 @@ def _timed(fn):
     return res, time.perf_counter() - t0
 
 
-def bench_fused(seed, n_time=32, n_fid=30, nlayers=20):
-    """Fused complex64 (MFMA blocks) vs unfused complex64 vs complex128."""
+def _block_circuit(n, nblocks, seed):
+    """Brickwork of Haar-random 5-qubit unitaries (dense 32 x 32 blocks, the
+    shape Circuit.fuse(max_qubits=5) produces on dense circuits)."""
+    import numpy as np
+
+    from qibo_amd import Circuit, gates
+
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    starts = list(range(0, n - 4, 5))
+    k = 0
+    while k < nblocks:
+        for s0 in starts:
+            if k == nblocks:
+                break
+            z = (rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32))) / np.sqrt(2)
+            q, r = np.linalg.qr(z)
+            u = q * (np.diagonal(r) / np.abs(np.diagonal(r)))
+            off = (k // len(starts)) % 5
+            qubits = [(s0 + off + i) % n for i in range(5)]
+            c.add(gates.Unitary(u, *qubits, check_unitary=False))
+            k += 1
+    return c
+
+
+def bench_fused(seed, n_time=32, n_fid=30, nlayers=20, nblocks=24):
+    """complex64 vs complex128: blocked passes on the layered circuit, and
+    dense 5-qubit blocks (complex64 on MFMA, complex128 FP64) with their
+    roofline; accuracy of fused complex64 against complex128."""
     import torch
 
     from qibo_amd import models
@@ -115,21 +143,37 @@ def bench_fused(seed, n_time=32, n_fid=30, nlayers=20):
 
     b64 = MI355XBackend(dtype="complex64")
     b128 = MI355XBackend(dtype="complex128")
+    res = {"nqubits": n_time, "layers": nlayers}
     circ = models.random_layered_circuit(n_time, nlayers, seed=seed, measure=False)
-    fused = circ.fuse(max_qubits=5)
-    sizes = [len(g.qubits) for g in fused.queue if g.__class__.__name__ == "FusedGate"]
-    res = {"nqubits": n_time, "layers": nlayers, "gates": circ.ngates, "fused_gates": len(fused.queue),
-           "fused_block_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))}}
-    for key, backend, c in (("unfused_c64_s", b64, circ), ("fused_c64_s", b64, fused),
-                            ("unfused_c128_s", b128, circ)):
-        c._final_state = None
-        r, dt = _timed(lambda: backend.execute_circuit(c))
+    res["gates"] = circ.ngates
+    for key, backend in (("layered_c64_s", b64), ("layered_c128_s", b128)):
+        circ._final_state = None
+        backend.execute_circuit(circ)  # warm-up (allocator, first launch)
+        circ._final_state = None
+        torch.cuda.empty_cache()
+        r, dt = _timed(lambda: backend.execute_circuit(circ))
         res[key] = dt
         del r
-        c._final_state = None
+        circ._final_state = None
         torch.cuda.empty_cache()
-    res["speedup_fused_c64_vs_c128"] = res["unfused_c128_s"] / res["fused_c64_s"]
-    res["speedup_fused_c64_vs_unfused_c64"] = res["unfused_c64_s"] / res["fused_c64_s"]
+    res["speedup_layered_c64_vs_c128"] = res["layered_c128_s"] / res["layered_c64_s"]
+    # dense 5-qubit blocks: one qsim_apply_fused launch each
+    blocks = _block_circuit(n_time, nblocks, seed)
+    for key, backend, isz in (("blocks5_c64", b64, 8), ("blocks5_c128", b128, 16)):
+        blocks._final_state = None
+        backend.execute_circuit(blocks)
+        blocks._final_state = None
+        torch.cuda.empty_cache()
+        r, dt = _timed(lambda: backend.execute_circuit(blocks))
+        del r
+        blocks._final_state = None
+        torch.cuda.empty_cache()
+        per = dt / nblocks
+        flops = 8 * 32 * 2**n_time  # 32 complex multiply-adds per amplitude
+        res[key] = {"s": dt, "blocks": nblocks, "ms_per_block": 1e3 * per,
+                    "hbm_GBs": 2 * 2**n_time * isz / per / 1e9, "TFLOPs": flops / per / 1e12}
+    res["blocks5_c64"]["mfma_peak_TFLOPs_fp32"] = 157.3
+    res["speedup_blocks5_c64_vs_c128"] = res["blocks5_c128"]["s"] / res["blocks5_c64"]["s"]
     # accuracy at n_fid: fused complex64 against complex128
     circ = models.random_layered_circuit(n_fid, nlayers, seed=seed, measure=False)
     s64 = b64.execute_circuit(circ.fuse(max_qubits=5)).state()
@@ -248,6 +292,10 @@ def main():
 
     def step():
         circuit._final_state = None
+        # a non-collapsing M keeps the samples of the previous run on the gate
+        # (qibo's MeasurementResult); clear them so every step samples afresh
+        for gate in circuit.measurements:
+            gate.result.reset()
         result = backend.execute_circuit(circuit, nshots=args.nshots)
         samples = result.samples()
         return result, samples
@@ -288,6 +336,15 @@ def main():
     passes = list(backend.block_stats)
 
     ms_per_step = 1000.0 * elapsed / args.steps
+    # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
+    # this same workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scales read off a
+    # plain copy of one state buffer); only quoted for the workload it measured
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "k_blocked_layered30_c128.json")
+    if n == 30 and args.nlayers == 20 and os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh)["traffic_bytes"]
+        traffic_src = "profiles/k_blocked_layered30_c128.json"
     value = ngates * args.steps / elapsed
     avg_dense_ms = float(np.mean(kernel_ms)) if kernel_ms else float("nan")
     dense_bytes = 2 * (2**n) * itemsize
@@ -322,7 +379,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "launches_timed": len(kernel_ms),
             "avg_launch_ms": avg_dense_ms,
             "gates_per_launch": (sum(passes) / len(passes)) if passes else None,

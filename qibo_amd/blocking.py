@@ -273,13 +273,30 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         else:
             emit(OP_DIAG, 0, cm, nt, enc, d, rreq)
 
+    def reach(mapping):
+        """Register one-qubit gates that could run in ``mapping`` from the
+        current state before another mapping change is needed (the mapping
+        choice looks ahead instead of counting only the ready gates)."""
+        left = list(npred)
+        frontier = [i for i in ready if runs_here(i, mapping)]
+        done = 0
+        while frontier:
+            i = frontier.pop()
+            if items[i].op.kind != "diag":
+                done += 1
+            for j in succ[i]:
+                left[j] -= 1
+                if left[j] == 0 and runs_here(j, mapping):
+                    frontier.append(j)
+        return done
+
     while ready:
         ready.sort()
         pick = next((i for i in ready if runs_here(i, cur)), None)
         if pick is None:
             dense1 = [i for i in ready if items[i].op.kind == "dense" and len(items[i].op.targets) == 1]
             if dense1:
-                counts = {m: sum(runs_here(i, m) for i in dense1) for m in (0, 1, 2)}
+                counts = {m: reach(m) for m in (0, 1, 2) if m != cur}
                 best = max(counts, key=lambda m: (counts[m], -m))
                 flush_reg()
                 emit(OP_MAP, best, 0, 0, (), None)

@@ -442,7 +442,7 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 }
 
 #ifndef QSIM_BLOCKED_WAVES
-#define QSIM_BLOCKED_WAVES 3
+#define QSIM_BLOCKED_WAVES 4
 #endif
 
 template <typename R>
