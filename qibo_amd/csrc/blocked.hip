@@ -133,9 +133,17 @@ __device__ __forceinline__ uint32_t local_index(int map, uint32_t lane, uint32_t
   return (lane & ((1u << r0) - 1)) | (r << r0) | ((lane >> r0) << (r0 + 4));
 }
 
-// XOR swizzle of LDS slots: spreads the strided accesses of mappings 1 and 2
-// over all banks. A bijection of [0, 4096).
-__device__ __forceinline__ uint32_t swz(uint32_t u) { return u ^ ((u >> 4) & 15); }
+// XOR swizzle of LDS slots (8- or 4-byte components): bits 0-3 ^= bits 4-7 and
+// bits 4-5 ^= bits 8-9. For every mapping the 64 lanes of a wave then hit each
+// of the 64 four-byte banks at most twice with 8-byte slots and once with
+// 4-byte slots (the minimum). A bijection of [0, 4096) (bits 6-11 are kept,
+// bits 4-5 are recovered from bits 8-9, then bits 0-3 from bits 4-7), and
+// linear over GF(2): swz(a ^ b) = swz(a) ^ swz(b), so a lane's slots are its
+// own part XOR a per-register constant.
+__device__ __forceinline__ uint32_t swz(uint32_t u) { return u ^ ((u >> 4) & 15u) ^ (((u >> 8) & 3u) << 4); }
+
+// Local index of register 0 of `lane` under mapping `map` (register bits zero).
+__device__ __forceinline__ uint32_t lane_local(int map, uint32_t lane) { return local_index(map, lane, 0); }
 
 // Global-index bits contributed by local index u.
 __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
@@ -152,14 +160,18 @@ __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
 // lets twice as many workgroups share a CU.
 template <typename R>
 __device__ __forceinline__ void lds_put(R* lds, const R* x, int map) {
+  const uint32_t s = swz(lane_local(map, threadIdx.x));
+  const int r0 = reg_base(map);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) lds[swz(local_index(map, threadIdx.x, r))] = x[r];
+  for (int r = 0; r < kBReg; ++r) lds[s ^ swz(uint32_t(r) << r0)] = x[r];
 }
 
 template <typename R>
 __device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
+  const uint32_t s = swz(lane_local(map, threadIdx.x));
+  const int r0 = reg_base(map);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) x[r] = lds[swz(local_index(map, threadIdx.x, r))];
+  for (int r = 0; r < kBReg; ++r) x[r] = lds[s ^ swz(uint32_t(r) << r0)];
 }
 
 // Change of register mapping: amplitude (lane, r) of mapping `from` moves to
@@ -253,6 +265,14 @@ __device__ __forceinline__ void dense1_pair(cplx<R>& x0r, cplx<R>& x1r, cplx<R> 
     x0r = cfma(m1, x1, cmul(m0, x0));
     x1r = cfma(m3, x1, cmul(m2, x0));
   }
+}
+
+__device__ __forceinline__ void flip_sign(double& x, uint32_t sign) {
+  x = __longlong_as_double(__double_as_longlong(x) ^ (uint64_t(sign) << 32));
+}
+
+__device__ __forceinline__ void flip_sign(float& x, uint32_t sign) {
+  x = __uint_as_float(__float_as_uint(x) ^ sign);
 }
 
 template <typename R>
@@ -387,8 +407,9 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
     act[i] = ((tile_base | deposit(b, gbit)) & in.cmask) == in.cmask;
   }
   uint32_t slot[kBReg];  // the lane's amplitudes in the current mapping
+  const uint32_t s0 = swz(lane_local(map, threadIdx.x));
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) slot[r] = swz(local_index(map, threadIdx.x, r));
+  for (int r = 0; r < kBReg; ++r) slot[r] = s0 ^ swz(uint32_t(r) << reg_base(map));
   cplx<R> g[kBReg];  // g[i * D + c]: input c of group i
   // gather: real parts, then imaginary parts
 #pragma clang loop unroll(full)
@@ -525,14 +546,14 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
       }
       case kOpNeg: {
+        // flip the sign bits where the lane condition holds: one XOR per component
         const uint32_t rreq = h.rreq;
+        const uint32_t sign = lane_ok ? 0x80000000u : 0u;
 #pragma clang loop unroll(full)
         for (int r = 0; r < kBReg; ++r) {
           if (!reg_ok(r, rreq)) continue;
-          cplx<R> nv;
-          nv.re = -v[r].re;
-          nv.im = -v[r].im;
-          v[r] = csel(lane_ok, nv, v[r]);
+          flip_sign(v[r].re, sign);
+          flip_sign(v[r].im, sign);
         }
         break;
       }

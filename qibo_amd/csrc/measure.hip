@@ -446,6 +446,76 @@ __global__ __launch_bounds__(64) void k_group_map(const TileInfo* __restrict__ i
   }
 }
 
+// Exact in-order sum of elements [lo, hi) (at most one tile) from the exact
+// start s, by the whole wavefront: lane c owns the 64-element chunk c. A
+// chunk whose partial sums stay in one binade advances the chain by its map
+// (and is finished by its lane from its exact start, in order); a chunk where
+// the sum crosses a power of two is added element by element. Writes
+// cdf[lo, hi) and returns the end value: bitwise the sequential chain.
+template <typename P>
+__device__ double chain_tile_chunks(const P* __restrict__ p, int64_t lo, int64_t hi, double s,
+                                    double* __restrict__ cdf) {
+  constexpr int C = 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t c_lo = lo + int64_t(lane) * C;
+  const int64_t c_hi = c_lo + C < hi ? c_lo + C : hi;  // empty when c_lo >= hi
+  double asum = 0.0;
+  for (int64_t i = c_lo; i < c_hi; ++i) asum += load_p(p, i);
+  double incl = asum;  // approximate inclusive prefix of the chunk sums
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  const double A = s + (incl - asum), Z = s + incl;
+  const int Eb = binade_of(A);
+  const double margin = 1.0 + 0x1p-20;
+  const double blo = ldexp(1.0, Eb - 1023), bhi = ldexp(1.0, Eb - 1022);
+  bool safe = c_lo < c_hi && (Eb == 1 || A >= blo * margin) && (Z * margin < bhi);
+  Map f = {0, 0};
+  if (safe) {
+    for (int64_t i = c_lo; i < c_hi; ++i) f = compose(f, elem_map(load_p(p, i), Eb));
+    safe = f.c0 < kTwo53 && f.c1 < kTwo53;
+  }
+  double start = 0.0;
+  bool resolved = false;
+  for (int c = 0; c < 64; ++c) {
+    const int64_t cl = lo + int64_t(c) * C;
+    if (cl >= hi) break;
+    const int64_t ch = cl + C < hi ? cl + C : hi;
+    const bool sf = __shfl(safe ? 1 : 0, c, 64) != 0;
+    const int cEb = __shfl(Eb, c, 64);
+    const int64_t fc0 = __shfl(f.c0, c, 64), fc1 = __shfl(f.c1, c, 64);
+    bool ok = sf && in_binade(s, cEb);
+    int64_t m2 = 0;
+    if (ok) {
+      const int64_t m = m_of(s, cEb);
+      m2 = m + ((m & 1) ? fc1 : fc0);
+      ok = m2 < kTwo53;
+    }
+    if (ok) {
+      if (lane == c) {
+        start = s;
+        resolved = true;
+      }
+      s = from_m(m2, cEb);
+    } else {
+      for (int64_t i = cl; i < ch; ++i) {
+        s = s + load_p(p, i);
+        if (lane == 0) cdf[i] = s;
+      }
+    }
+  }
+  if (resolved) {
+    double x = start;
+    for (int64_t i = c_lo; i < c_hi; ++i) {
+      x = x + load_p(p, i);
+      cdf[i] = x;
+    }
+  }
+  return s;
+}
+
 // Tile-level chain of one group (64 tiles) from the exact start s; every lane
 // computes the same scalar chain. Writes tile starts, tile flags (0 = finish
 // by maps, 1 = summed here) and the cdf of tiles summed here. Returns s.
@@ -490,11 +560,7 @@ __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t 
     } else {
       const int64_t lo = u * kTile;
       const int64_t hi = lo + kTile < n ? lo + kTile : n;
-#pragma unroll 8
-      for (int64_t i = lo; i < hi; ++i) {
-        s = s + load_p(p, i);
-        if (lane == 0) cdf[i] = s;
-      }
+      s = chain_tile_chunks(p, lo, hi, s, cdf);
       if (lane == 0) info[u].flag = 1;
     }
   }

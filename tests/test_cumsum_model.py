@@ -143,3 +143,59 @@ def test_composition_is_associative():
         a = compose(compose(fs[0], fs[1]), fs[2])
         b = compose(fs[0], compose(fs[1], fs[2]))
         assert a == b
+
+
+def chain_tile_chunks(s: float, ps, chunk: int = 64):
+    """Python restatement of the wave-parallel fallback for one tile whose
+    sum crosses a power of two (csrc/measure.hip chain_tile_chunks): chunks
+    whose partial sums stay in one binade advance by their maps and are
+    finished from their exact start; crossing chunks add element by element.
+    Returns (cdf, end)."""
+    n = len(ps)
+    chunks = [ps[i:i + chunk] for i in range(0, n, chunk)]
+    asums = [float(np.sum(c)) for c in chunks]
+    approx = np.cumsum([0.0] + asums)
+    maps = []
+    for c, (a, z) in zip(chunks, zip(s + approx[:-1], s + approx[1:])):
+        eb = binade_of(a)
+        lo, hi = math.ldexp(1.0, eb - 1023), math.ldexp(1.0, eb - 1022)
+        margin = 1.0 + 2.0**-20
+        safe = (eb == 1 or a >= lo * margin) and (z * margin < hi)
+        f = (0, 0)
+        if safe:
+            for p in c:
+                f = compose(f, elem_map(float(p), eb))
+            safe = f[0] < TWO53 and f[1] < TWO53
+        maps.append((safe, eb, f))
+    out = []
+    for c, (safe, eb, f) in zip(chunks, maps):
+        ok = safe and in_binade(s, eb)
+        if ok:
+            m = m_of(s, eb)
+            m2 = m + (f[1] if m & 1 else f[0])
+            ok = m2 < TWO53
+        if ok:
+            x = s
+            for p in c:  # the owning lane finishes from the exact start
+                x = x + float(p)
+                out.append(x)
+            s = from_m(m2, eb)
+            assert s == x  # the map's end equals the in-order sum
+        else:
+            for p in c:
+                s = s + float(p)
+                out.append(s)
+    return out, s
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_chunked_fallback_equals_sequential(seed):
+    rng = np.random.default_rng(seed)
+    # a tile that starts near zero and crosses many binades, plus exact ties
+    ps = rng.random(4096) * np.geomspace(1e-12, 1e-3, 4096)
+    ps[::97] = 2.0**-40
+    s0 = [0.0, 1e-300, 2.0**-30, 0.5 - 1e-9][seed]
+    got, end = chain_tile_chunks(s0, ps)
+    want = np.cumsum(np.concatenate([[s0], ps]))[1:]
+    assert np.array_equal(np.array(got), want)
+    assert end == want[-1]
