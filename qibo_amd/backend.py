@@ -106,6 +106,28 @@ def plan_matrix(matrix: np.ndarray, targets: Sequence[int], controls: Sequence[i
                 return _Op("noop", (), tuple(controls) + (t,), None)
             return _Op("diag", (), tuple(controls) + (t,), np.array([d], dtype=matrix.dtype))
         return _Op("diag", (t,), controls, np.array([a, d], dtype=matrix.dtype))
+    if matrix.ndim == 2 and not np.any(matrix - np.diag(np.diagonal(matrix))):
+        # diagonal: the same peeling on the diagonal vector (qubit t is a
+        # control when every entry with bit t = 0 is 1), without the blocks
+        d = np.diagonal(matrix).copy()
+        tg, extra = list(targets), []
+        changed = True
+        while changed and tg:
+            changed = False
+            k = len(tg)
+            cube = d.reshape((2,) * k)
+            for t in range(k):
+                if np.all(np.take(cube, 0, axis=t) == 1):
+                    d = np.take(cube, 1, axis=t).ravel().copy()
+                    extra.append(tg.pop(t))
+                    changed = True
+                    break
+        ctrl = list(controls) + extra
+        if np.all(d == 1):
+            return _Op("noop", tg, ctrl, None)
+        if len(tg) > 5:
+            raise_error(NotImplementedError, "diagonal gates on more than 5 targets")
+        return _Op("diag", tg, ctrl, d)
     m, tg, extra = _peel_controls(matrix, targets)
     ctrl = list(controls) + extra
     dim = m.shape[0]
@@ -514,18 +536,26 @@ class MI355XBackend(GatePlanner):
             return state
         if nqubits < blocking.TILE_BITS:
             return self._run_small(queue, state, nqubits)
-        segment = []
-        for gate in queue:
-            if isinstance(gate, M) and not gate.collapse:
-                gate.result.backend = self
-                continue
-            if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
-                state = self._run_blocked(segment, state, nqubits)
-                segment = []
-                state = gate.apply(self, state, nqubits)
-                continue
-            segment.append(self.plan_gate(gate))
-        return self._run_blocked(segment, state, nqubits)
+        gates_left = iter(queue)
+        stop = []  # the special gate that ended the current segment
+
+        def segment():
+            # planned gates up to the next special gate, produced lazily so
+            # planning overlaps the passes already launched
+            for gate in gates_left:
+                if isinstance(gate, M) and not gate.collapse:
+                    gate.result.backend = self
+                    continue
+                if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
+                    stop.append(gate)
+                    return
+                yield self.plan_gate(gate)
+
+        while True:
+            state = self._run_blocked(segment(), state, nqubits)
+            if not stop:
+                return state
+            state = stop.pop().apply(self, state, nqubits)
 
     def _run_small(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
         """Circuits on fewer than 12 qubits: run as one blocked pass on a
@@ -559,10 +589,9 @@ class MI355XBackend(GatePlanner):
         return state
 
     def _run_blocked(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
-        """Partition ``ops`` into passes and launch each pass as soon as it is
-        compiled, so the host compiles pass k+1 while the GPU runs pass k."""
-        if not ops:
-            return state
+        """Partition ``ops`` (an iterable of planned gates) into passes and
+        launch each pass as soon as it is compiled, so the host plans and
+        compiles pass k+1 while the GPU runs pass k."""
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         low = 3 if code == _lib.C128 else 4

@@ -47,7 +47,7 @@ lane, ``rreq`` the register bits (as a 4-bit mask over register positions);
 DENSE_LDS keeps all condition bits in ``cmask``.
 """
 
-from typing import Iterator, Sequence, Tuple
+from typing import Iterable, Iterator, Sequence, Tuple
 
 import numpy as np
 
@@ -86,20 +86,41 @@ def blockable(op) -> bool:
         op.kind != "diag" or len(op.targets) <= 5)
 
 
-def partition(ops: Sequence, nqubits: int, low_bits: int, ordered: bool = False) -> Iterator[Tuple[str, object]]:
-    """Split ``ops`` (planned gates, circuit order) into passes.
+WINDOW = 768  # planned gates a pass may look ahead over
+
+
+def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
+              window: int = WINDOW) -> Iterator[Tuple[str, object]]:
+    """Split ``ops`` (planned gates in circuit order, any iterable) into passes.
 
     Yields ("single", op) and ("block", (tile_bits, [ops])) items whose
     sequential execution equals the circuit, one at a time (the caller can
-    launch a pass before the next is formed). With ``ordered`` a pass ends at
-    the first gate that does not fit (no gate moves ahead).
+    launch a pass before the next is formed). A pass is chosen among the
+    first ``window`` pending gates, so ``ops`` is consumed lazily: gates
+    further on are planned while earlier passes run. With ``ordered`` a pass
+    ends at the first gate that does not fit (no gate moves ahead).
     """
-    items = [_Item(op, nqubits, i) for i, op in enumerate(ops) if op.kind != "noop"]
+    source = iter(ops)
+    items = []
+    counter = 0
+
+    def fill():
+        nonlocal counter
+        while len(items) < window:
+            op = next(source, None)
+            if op is None:
+                return
+            if op.kind != "noop":
+                items.append(_Item(op, nqubits, counter))
+                counter += 1
+
+    fill()
     while items:
         first = items[0]
         if not blockable(first.op) or nqubits < TILE_BITS:
             yield ("single", first.op)
-            items = items[1:]
+            del items[0]
+            fill()
             continue
         tile = set(range(low_bits))
         chosen, deferred = [], []
@@ -115,7 +136,8 @@ def partition(ops: Sequence, nqubits: int, low_bits: int, ordered: bool = False)
                 deferred.append(it)
                 td |= it.nd_targets
                 ad |= it.all_bits
-        items = deferred
+        items[:] = deferred
+        fill()
         if len(chosen) == 1:
             yield ("single", chosen[0].op)
             continue

@@ -224,3 +224,28 @@ def test_tfim_terms():
     assert sorted(zip(strings, qubits)) == sorted(
         [("ZZ", (0, 1)), ("ZZ", (1, 2)), ("ZZ", (2, 3)), ("X", (0,)), ("X", (1,)), ("X", (2,)), ("X", (3,))])
     assert all(c == -1.0 for c in coeffs)
+
+
+def test_diagonal_planning_matches_general_peeling():
+    """plan_matrix's diagonal path gives the op the general block peeling gives."""
+    from qibo_amd.backend import _peel_controls, plan_matrix
+
+    rng = np.random.default_rng(0)
+    for k in range(1, 5):
+        for _ in range(100):
+            cube = np.exp(1j * rng.normal(size=2**k)).reshape((2,) * k)
+            for t in range(k):
+                if rng.random() < 0.5:
+                    idx = [slice(None)] * k
+                    idx[t] = 0
+                    cube[tuple(idx)] = 1
+            d = cube.ravel()
+            m = np.diag(d)
+            targets = [int(x) for x in rng.choice(20, size=k, replace=False)]
+            got = plan_matrix(m, targets, [25])
+            red, tg, extra = _peel_controls(m, targets)
+            want_d = np.diagonal(red)
+            kind = "noop" if np.all(want_d == 1) else "diag"
+            assert (got.kind, got.targets, got.controls) == (kind, tuple(tg), tuple([25] + extra))
+            if kind == "diag":
+                assert np.array_equal(got.data, want_d)
