@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 2
+#define QSIM_ABI_VERSION 3
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -117,6 +117,34 @@ int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, in
 size_t qsim_blocked_header_bytes(void);
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                        size_t program_bytes, int ninstr, void* stream);
+
+/* The blocked pass for a batch of `nbatch` states stored one after the other
+ * (2^nqubits amplitudes apart), state b running the program at
+ * programs_dev + b * program_stride (all programs program_bytes long, same
+ * padding rule). Runs the batch in one launch (grid z = batch entry): the
+ * forward pass of the batched adjoint gradients. No reference counterpart
+ * (the reference runs one circuit per call). */
+int qsim_apply_blocked_batched(void* states, int nbatch, int nqubits, int dtype, const int* tile_bits,
+                               const void* programs_dev, size_t program_stride, size_t program_bytes, int ninstr,
+                               void* stream);
+
+/* Adjoint sweep pass for a batch: the records (ops 0-2, 4, 6) are applied to
+ * both psi and lam of every batch entry; op 8 (2x2 M on a register bit) and
+ * op 9 (diagonal M on up to 5 bits) add Re <lam| M |psi> of each tile into
+ * `partials` (qsim_adjoint_partials_count(nqubits, nbatch, nslots) doubles,
+ * nslots = INNER records per program, in program order). The adjoint method
+ * replaces the reference's parameter shift / finite differences
+ * (derivative.py:7-226), which re-simulate per parameter. */
+size_t qsim_adjoint_partials_count(int nqubits, int nbatch, int nslots);
+int qsim_adjoint_blocked(void* psi, void* lam, int nbatch, int nqubits, int dtype, const int* tile_bits,
+                         const void* programs_dev, size_t program_stride, size_t program_bytes, int ninstr,
+                         int nslots, double* partials, size_t partials_count, void* stream);
+
+/* grads[b * nparams + param_of[s]] = 2 * (sum of the partials of INNER
+ * record s of batch entry b), summed in a fixed order. `param_of_dev` is a
+ * DEVICE int array of nslots entries (-1: not stored). */
+int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nslots, const int* param_of_dev,
+                        int nparams, double* grads, void* stream);
 
 /* Probabilities of measuring `qubits` (m = nmeasured, in the given order, the
  * first being the most significant bit of the output index):

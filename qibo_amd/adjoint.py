@@ -30,8 +30,9 @@ from typing import List, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import gates
+from . import _lib, blocking, gates
 from .backend import plan_matrix
+from .blocking import BatchOp
 from .config import raise_error
 
 _X = np.array([[0, 1], [1, 0]], dtype=complex)
@@ -125,9 +126,21 @@ def expectation_and_gradient(backend, circuit, hamiltonian, initial_state=None):
 
 
 def batched_expectation_and_gradient(backend, circuit, hamiltonian, params) -> Tuple[torch.Tensor, torch.Tensor]:
-    """Energies [B] and gradients [B, P] for a batch of parameter sets [B, P]."""
+    """Energies [B] and gradients [B, P] for a batch of parameter sets [B, P].
+
+    Circuits of at least 12 qubits whose gates the batched blocked sweep
+    covers (one-qubit gates, controlled one-qubit gates, diagonal gates;
+    trainable gates RX/RY/RZ/CRX/CRY/CRZ/U1/CU1/RZZ) run all sets at once:
+    forward and reverse sweeps as batched blocked passes, one launch per pass
+    for the whole batch. Other circuits run set by set."""
     params = np.asarray(params.detach().cpu().numpy() if isinstance(params, torch.Tensor) else params,
                         dtype=np.float64)
+    if params.ndim == 1:
+        params = params[None, :]
+    try:
+        return blocked_expectation_and_gradient(backend, circuit, hamiltonian, params)
+    except _Unsupported:
+        pass
     energies, grads = [], []
     for row in params:
         circuit.set_parameters(row)
@@ -135,6 +148,224 @@ def batched_expectation_and_gradient(backend, circuit, hamiltonian, params) -> T
         energies.append(e)
         grads.append(g)
     return torch.stack(energies), torch.stack(grads)
+
+
+# ---------------------------------------------------------------------------
+# batched blocked sweep
+# ---------------------------------------------------------------------------
+class _Unsupported(Exception):
+    """The batched blocked sweep does not cover this circuit."""
+
+
+def _controlled(block: np.ndarray) -> np.ndarray:
+    """[[I, 0], [0, block]] for a batch of 2x2 blocks (one control, first)."""
+    out = np.zeros(block.shape[:-2] + (4, 4), dtype=block.dtype)
+    out[..., 0, 0] = out[..., 1, 1] = 1
+    out[..., 2:, 2:] = block
+    return out
+
+
+def _batched_rotation(name: str, th: np.ndarray) -> np.ndarray:
+    """Matrices of a parametrized gate for angles ``th`` [B] -> [B, d, d], by the
+    formulas of qibo_amd.matrices (npmatrices.py:80-135, 160-205, 269-272)."""
+    th = np.asarray(th, dtype=np.float64)
+    z = np.zeros_like(th) + 0j
+    if name in ("RX", "CRX"):
+        c, s = np.cos(th / 2.0) + 0j, -1j * np.sin(th / 2.0)
+        m = np.array([[c, s], [s, c]])
+    elif name in ("RY", "CRY"):
+        c, s = np.cos(th / 2.0) + 0j, np.sin(th / 2.0) + 0j
+        m = np.array([[c, -s], [s, c]])
+    elif name in ("RZ", "CRZ"):
+        ph = np.exp(0.5j * th)
+        m = np.array([[np.conj(ph), z], [z, ph]])
+    elif name in ("U1", "CU1"):
+        m = np.array([[z + 1, z], [z, np.exp(1j * th)]])
+    elif name == "RZZ":
+        ph = np.exp(0.5j * th)
+        cph = np.conj(ph)
+        m = np.array([[cph, z, z, z], [z, ph, z, z], [z, z, ph, z], [z, z, z, cph]])
+    else:
+        raise _Unsupported(name)
+    m = np.moveaxis(m, -1, 0)
+    return _controlled(m) if name.startswith("C") else m
+
+
+# M = dU/dtheta U^dagger of the trainable gates the sweep covers, on the
+# gate's target qubits (controls of the gate stay conditions)
+_HALF_I = -0.5j
+_INNER = {
+    "RX": ("inner1", _HALF_I * _X), "RY": ("inner1", _HALF_I * _Y), "RZ": ("innerd", _HALF_I * np.array([1, -1])),
+    "CRX": ("inner1", _HALF_I * _X), "CRY": ("inner1", _HALF_I * _Y),
+    "CRZ": ("innerd", _HALF_I * np.array([1, -1])),
+    "U1": ("innerd", np.array([0, 1j])), "CU1": ("innerd", np.array([0, 1j])),
+    "RZZ": ("innerd", _HALF_I * np.array([1, -1, -1, 1])),
+}
+
+
+def _plan_batch(mats: np.ndarray, qubits, controls):
+    """The planned operation shared by every set's matrix in ``mats`` [B, D, D],
+    with its per-set data; raises _Unsupported when the sets would plan
+    differently or the operation is not a one-qubit dense or diagonal one."""
+    op0 = plan_matrix(mats[0], qubits, controls)
+    nb = mats.shape[0]
+    if op0.kind == "noop":
+        if np.array_equal(mats, np.broadcast_to(mats[0], mats.shape)):
+            return None
+        raise _Unsupported("parameter-dependent identity")
+    if op0.kind not in ("dense", "diag") or (op0.kind == "dense" and len(op0.targets) != 1):
+        raise _Unsupported(op0.kind)
+    q = list(qubits)
+    L = len(q)
+    peeled = [c for c in op0.controls if c not in tuple(controls)]
+    k = len(op0.targets)
+    idx = []
+    for r in range(2**k):
+        i = 0
+        for j, t in enumerate(op0.targets):
+            if (r >> (k - 1 - j)) & 1:
+                i |= 1 << (L - 1 - q.index(t))
+        for c in peeled:
+            i |= 1 << (L - 1 - q.index(c))
+        idx.append(i)
+    idx = np.array(idx)
+    full = np.broadcast_to(np.eye(2**L, dtype=mats.dtype), mats.shape).copy()
+    if op0.kind == "dense":
+        data = mats[:, idx][:, :, idx]
+        full[:, idx[:, None], idx[None, :]] = data
+    else:
+        data = np.diagonal(mats, axis1=1, axis2=2)[:, idx]
+        full[:, idx, idx] = data
+    if not np.array_equal(full, mats):
+        raise _Unsupported("structure differs across parameter sets")
+    assert data.shape[0] == nb
+    return BatchOp(op0.kind, op0.targets, op0.controls, data)
+
+
+def _batched_ops(backend, circuit, params: np.ndarray):
+    """Forward and reverse BatchOp lists and the trainable-parameter count."""
+    queue = [g for g in circuit.queue if not isinstance(g, gates.M)]
+    nb = params.shape[0]
+    slot_of, k = {}, 0
+    for g in circuit.trainable_gates:
+        if g.nparams != 1:
+            raise _Unsupported("multi-parameter trainable gate")
+        slot_of[id(g)] = k
+        k += 1
+    if k != params.shape[1]:
+        raise _Unsupported("parameter count")
+    cdt = np.complex128 if backend.dtype == "complex128" else np.complex64
+    fwd, rev = [], []
+    for g in queue:
+        if not isinstance(g, gates.Gate) or isinstance(g, gates.SpecialGate):
+            raise _Unsupported("special gate")
+        targets, controls = _gate_qubits(g)
+        qubits = tuple(g.qubits) if not g.is_controlled_by else tuple(g.target_qubits)
+        name = g.__class__.__name__
+        if id(g) in slot_of and g.trainable:
+            mats = _batched_rotation(name, params[:, slot_of[id(g)]]).astype(cdt)
+        else:
+            m = np.asarray(g.matrix(backend), dtype=cdt)
+            mats = np.broadcast_to(m, (nb,) + m.shape)
+        op = _plan_batch(mats, qubits, controls)
+        dag = _plan_batch(np.conj(np.swapaxes(mats, 1, 2)), qubits, controls)
+        if op is not None:
+            fwd.append(op)
+        step = []
+        if id(g) in slot_of and g.trainable:
+            kind, mm = _INNER[name]
+            mm = np.asarray(mm, dtype=cdt).reshape(1, -1)
+            step.append(BatchOp(kind, g.target_qubits, g.control_qubits, np.broadcast_to(mm, (nb, mm.shape[1])),
+                                slot_of[id(g)]))
+        if dag is not None:
+            step.append(dag)
+        rev.append(step)
+    reverse = [op for step in reversed(rev) for op in step]
+    return fwd, reverse, k
+
+
+def _upload_programs(progs: np.ndarray, device):
+    nb, nbytes = progs.shape
+    stride = (nbytes + 15) // 16 * 16
+    buf = np.zeros(nb * stride + blocking.PAD_BYTES, dtype=np.uint8)
+    buf[: nb * stride].reshape(nb, stride)[:, :nbytes] = progs
+    dev = torch.from_numpy(buf).pin_memory().to(device, non_blocking=True)
+    return dev, stride, nbytes
+
+
+def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.ndarray):
+    """Energies [B] and gradients [B, P] of all parameter sets at once (see
+    batched_expectation_and_gradient); raises _Unsupported for other circuits."""
+    n = circuit.nqubits
+    if circuit.density_matrix or n < blocking.TILE_BITS:
+        raise _Unsupported("size")
+    nb_all = params.shape[0]
+    fwd, rev, nparams = _batched_ops(backend, circuit, params)
+    xm, zm, ny, coeffs = hamiltonian.pauli_tables()
+    dev = backend.torch_device
+    tdt = torch.complex128 if backend.dtype == "complex128" else torch.complex64
+    cdt = np.complex128 if backend.dtype == "complex128" else np.complex64
+    code = _lib.C128 if backend.dtype == "complex128" else _lib.C64
+    low = 3 if code == _lib.C128 else 4
+    lib = backend.lib
+    state_bytes = (2**n) * (16 if code == _lib.C128 else 8)
+    free = torch.cuda.mem_get_info(dev)[0]
+    chunk = max(1, min(nb_all, int(0.4 * free) // (2 * state_bytes)))
+    energies = torch.empty(nb_all, dtype=torch.float64, device=dev)
+    grads = torch.zeros((nb_all, nparams), dtype=torch.float64, device=dev)
+    stream = backend._stream()
+    c = np.ascontiguousarray(coeffs, dtype=np.float64)
+    xm = np.ascontiguousarray(xm, dtype=np.uint64)
+    zm = np.ascontiguousarray(zm, dtype=np.uint64)
+    ny = np.ascontiguousarray(ny, dtype=np.int32)
+    for b0 in range(0, nb_all, chunk):
+        b1 = min(nb_all, b0 + chunk)
+        nb = b1 - b0
+        sel = slice(b0, b1)
+
+        def sub(ops):
+            for op in ops:
+                yield BatchOp(op.kind, op.targets, op.controls, op.data[sel], op.slot)
+
+        psi = torch.zeros((nb, 2**n), dtype=tdt, device=dev)
+        psi[:, 0] = 1
+        keep = []
+        for kind, payload in blocking.partition(sub(fwd), n, low, singles=False):
+            if kind != "block":
+                raise _Unsupported("single")
+            bits, pops = payload
+            progs, count, _ = blocking.compile_pass_batched(bits, pops, n, cdt, nb)
+            pdev, stride, nbytes = _upload_programs(progs, dev)
+            _lib.check(lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(bits),
+                                                      pdev.data_ptr(), stride, nbytes, count, stream), lib)
+            keep.append(pdev)
+        lam = torch.empty_like(psi)
+        for b in range(nb):
+            _lib.check(lib.qsim_apply_pauli_sum(psi[b].data_ptr(), n, code, len(c), xm.ctypes.data, zm.ctypes.data,
+                                                ny.ctypes.data, c.ctypes.data, lam[b].data_ptr(), stream), lib)
+            energies[b0 + b] = backend.expectation_pauli(psi[b], n, xm, zm, ny, c) + hamiltonian.constant
+        gsub = grads[sel]
+        for kind, payload in blocking.partition(sub(rev), n, low, singles=False):
+            if kind != "block":
+                raise _Unsupported("single")
+            bits, pops = payload
+            progs, count, slots = blocking.compile_pass_batched(bits, pops, n, cdt, nb)
+            pdev, stride, nbytes = _upload_programs(progs, dev)
+            npart = max(1, lib.qsim_adjoint_partials_count(n, nb, len(slots)))
+            part = torch.empty(npart, dtype=torch.float64, device=dev)
+            _lib.check(lib.qsim_adjoint_blocked(psi.data_ptr(), lam.data_ptr(), nb, n, code, _lib.int_array(bits),
+                                                pdev.data_ptr(), stride, nbytes, count, len(slots),
+                                                part.data_ptr(), npart, stream), lib)
+            if slots:
+                pof = torch.tensor(slots, dtype=torch.int32).pin_memory().to(dev, non_blocking=True)
+                _lib.check(lib.qsim_adjoint_reduce(part.data_ptr(), nb, n, len(slots), pof.data_ptr(), nparams,
+                                                   gsub.data_ptr(), stream), lib)
+                keep.append(pof)
+            keep.append(pdev)
+            keep.append(part)
+        for t in keep:
+            t.record_stream(torch.cuda.current_stream(dev))
+    return energies, grads
 
 
 class AdjointExpectation(torch.autograd.Function):

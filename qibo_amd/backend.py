@@ -227,6 +227,30 @@ class GatePlanner:
             return plan_matrix(gate.matrix(self), gate.target_qubits, gate.control_qubits)
         return plan_matrix(gate.matrix(self), gate.qubits, ())
 
+    # -- Pauli strings (host) ---------------------------------------------
+    def pauli_tables(self, terms: Sequence[str], term_qubits, nqubits: int):
+        """X/Z bit masks and Y counts of Pauli strings over amplitude bits."""
+        xm, zm, ny = [], [], []
+        for term, qubits in zip(terms, term_qubits):
+            x = z = y = 0
+            for factor, q in zip(term, qubits):
+                bit = 1 << (nqubits - 1 - int(q))
+                if factor == "X":
+                    x ^= bit
+                elif factor == "Z":
+                    z ^= bit
+                elif factor == "Y":
+                    x ^= bit
+                    z ^= bit
+                    y += 1
+                elif factor != "I":
+                    raise_error(ValueError, f"Unknown Pauli factor {factor}.")
+            xm.append(x)
+            zm.append(z)
+            ny.append(y)
+        return (np.asarray(xm, dtype=np.uint64), np.asarray(zm, dtype=np.uint64),
+                np.asarray(ny, dtype=np.int32))
+
 
 class MI355XBackend(GatePlanner):
     """State-vector simulation backend running on one MI355X (gfx950)."""
@@ -774,29 +798,6 @@ class MI355XBackend(GatePlanner):
         return torch.stack([self.cast(s, dtype=s.dtype) for s in shots])
 
     # -- expectation values ------------------------------------------------------
-    def pauli_tables(self, terms: Sequence[str], term_qubits, nqubits: int):
-        """X/Z bit masks and Y counts of Pauli strings over amplitude bits."""
-        xm, zm, ny = [], [], []
-        for term, qubits in zip(terms, term_qubits):
-            x = z = y = 0
-            for factor, q in zip(term, qubits):
-                bit = 1 << (nqubits - 1 - int(q))
-                if factor == "X":
-                    x ^= bit
-                elif factor == "Z":
-                    z ^= bit
-                elif factor == "Y":
-                    x ^= bit
-                    z ^= bit
-                    y += 1
-                elif factor != "I":
-                    raise_error(ValueError, f"Unknown Pauli factor {factor}.")
-            xm.append(x)
-            zm.append(z)
-            ny.append(y)
-        return (np.asarray(xm, dtype=np.uint64), np.asarray(zm, dtype=np.uint64),
-                np.asarray(ny, dtype=np.int32))
-
     def expectation_pauli(self, state: torch.Tensor, nqubits: int, xm, zm, ny, coeffs) -> torch.Tensor:
         """Device float64 scalar ``sum_t c_t <psi|P_t|psi>`` (no host sync)."""
         nterms = len(coeffs)

@@ -73,7 +73,7 @@ class _Item:
         self.idx = idx
         tb = {nqubits - 1 - q for q in op.targets}
         cb = {nqubits - 1 - q for q in op.controls}
-        self.nd_targets = frozenset() if op.kind == "diag" else frozenset(tb)
+        self.nd_targets = frozenset() if op.kind in ("diag", "innerd") else frozenset(tb)
         self.all_bits = frozenset(tb | cb)
 
 
@@ -82,20 +82,24 @@ def _commute(x: _Item, y: _Item) -> bool:
 
 
 def blockable(op) -> bool:
-    return op.kind in ("diag", "dense", "swap") and len(op.targets) <= 3 and (
-        op.kind != "diag" or len(op.targets) <= 5)
+    if op.kind in ("diag", "innerd"):
+        return len(op.targets) <= 5
+    if op.kind == "inner1":
+        return len(op.targets) == 1
+    return op.kind in ("dense", "swap") and len(op.targets) <= 3
 
 
 WINDOW = 768  # planned gates a pass may look ahead over
 
 
 def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
-              window: int = WINDOW) -> Iterator[Tuple[str, object]]:
+              window: int = WINDOW, singles: bool = True) -> Iterator[Tuple[str, object]]:
     """Split ``ops`` (planned gates in circuit order, any iterable) into passes.
 
     Yields ("single", op) and ("block", (tile_bits, [ops])) items whose
     sequential execution equals the circuit, one at a time (the caller can
-    launch a pass before the next is formed). A pass is chosen among the
+    launch a pass before the next is formed; with ``singles=False`` a
+    one-gate pass is still a block). A pass is chosen among the
     first ``window`` pending gates, so ``ops`` is consumed lazily: gates
     further on are planned while earlier passes run. With ``ordered`` a pass
     ends at the first gate that does not fit (no gate moves ahead).
@@ -138,7 +142,7 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
                 ad |= it.all_bits
         items[:] = deferred
         fill()
-        if len(chosen) == 1:
+        if len(chosen) == 1 and singles:
             yield ("single", chosen[0].op)
             continue
         bits = sorted(tile)
@@ -215,6 +219,51 @@ def _dependencies(items, ordered):
 
 
 _RVALS = np.array([[(r >> j) & 1 for r in range(16)] for j in range(4)], dtype=np.int64)  # [j, r]
+
+
+def _schedule(items, npred, succ, ready, runs_here):
+    """List scheduling of one pass. Yields ("map", m) when the register
+    mapping must change to m and ("op", i) for each item in issue order.
+    ``runs_here(i, m)`` says whether item i runs in mapping m without a change;
+    items that need a register bit (one-qubit dense gates) pick the mapping,
+    looking ahead at how many of them each mapping can run."""
+    cur = 0
+
+    def reg_gate(i):
+        op = items[i].op
+        return (op.kind == "dense" and len(op.targets) == 1) or op.kind == "inner1"
+
+    def reach(mapping):
+        left = list(npred)
+        frontier = [i for i in ready if runs_here(i, mapping)]
+        done = 0
+        while frontier:
+            i = frontier.pop()
+            if reg_gate(i):
+                done += 1
+            for j in succ[i]:
+                left[j] -= 1
+                if left[j] == 0 and runs_here(j, mapping):
+                    frontier.append(j)
+        return done
+
+    while ready:
+        ready.sort()
+        pick = next((i for i in ready if runs_here(i, cur)), None)
+        if pick is None:
+            if any(reg_gate(i) for i in ready):
+                counts = {m: reach(m) for m in (0, 1, 2) if m != cur}
+                cur = max(counts, key=lambda m: (counts[m], -m))
+                yield "map", cur
+                pick = next(i for i in ready if runs_here(i, cur))
+            else:
+                pick = ready[0]
+        ready.remove(pick)
+        yield "op", pick
+        for j in succ[pick]:
+            npred[j] -= 1
+            if npred[j] == 0:
+                ready.append(j)
 
 
 def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
@@ -295,39 +344,14 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         else:
             emit(OP_DIAG, 0, cm, nt, enc, d, rreq)
 
-    def reach(mapping):
-        """Register one-qubit gates that could run in ``mapping`` from the
-        current state before another mapping change is needed (the mapping
-        choice looks ahead instead of counting only the ready gates)."""
-        left = list(npred)
-        frontier = [i for i in ready if runs_here(i, mapping)]
-        done = 0
-        while frontier:
-            i = frontier.pop()
-            if items[i].op.kind != "diag":
-                done += 1
-            for j in succ[i]:
-                left[j] -= 1
-                if left[j] == 0 and runs_here(j, mapping):
-                    frontier.append(j)
-        return done
-
-    while ready:
-        ready.sort()
-        pick = next((i for i in ready if runs_here(i, cur)), None)
-        if pick is None:
-            dense1 = [i for i in ready if items[i].op.kind == "dense" and len(items[i].op.targets) == 1]
-            if dense1:
-                counts = {m: reach(m) for m in (0, 1, 2) if m != cur}
-                best = max(counts, key=lambda m: (counts[m], -m))
-                flush_reg()
-                emit(OP_MAP, best, 0, 0, (), None)
-                switches += 1
-                cur = best
-                pick = next(i for i in ready if runs_here(i, cur))
-            else:
-                pick = ready[0]
-        ready.remove(pick)
+    for event, arg in _schedule(items, npred, succ, ready, runs_here):
+        if event == "map":
+            flush_reg()
+            emit(OP_MAP, arg, 0, 0, (), None)
+            switches += 1
+            cur = arg
+            continue
+        pick = arg
         op = items[pick].op
         if op.kind == "diag":
             emit_diag(op)
@@ -345,12 +369,136 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             k = len(lbits)
             cm = sum(1 << (nqubits - 1 - q) for q in op.controls)
             emit(OP_DENSE_LDS, k, cm, 0, lbits + [0] * (4 - k) + sorted(lbits), mat)
-        for j in succ[pick]:
-            npred[j] -= 1
-            if npred[j] == 0:
-                ready.append(j)
     flush_reg()
     return bytes(buf), count, switches
+
+
+OP_INNER1, OP_INNER_DIAG = 8, 9
+
+
+class BatchOp:
+    """A planned operation for a batch of B parameter sets: the structure
+    (kind, targets, controls) is shared, ``data`` holds one matrix or diagonal
+    per set (leading axis B; a broadcast view when all sets share it).
+    kind: "dense" (one target), "diag", "inner1" / "innerd" (the adjoint
+    sweep's Re <lambda| M |psi> for a 2x2 / diagonal M, accumulated into
+    gradient slot ``slot``)."""
+
+    __slots__ = ("kind", "targets", "controls", "data", "slot")
+
+    def __init__(self, kind, targets, controls, data, slot=-1):
+        self.kind = kind
+        self.targets = tuple(targets)
+        self.controls = tuple(controls)
+        self.data = data
+        self.slot = slot
+
+
+def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
+                         nbatch: int) -> Tuple[np.ndarray, int, list]:
+    """Records of one blocked pass for a batch of parameter sets.
+
+    Returns (programs [B, nbytes] uint8, record count, gradient slot of each
+    INNER record in program order). Every set's program has the same records
+    (same layout and headers), only the entries differ. One record per
+    operation: DENSE1_REAL (when the 2x2 is real for every set), DENSE1,
+    DIAG, LANE_PHASE, INNER1, INNER_DIAG and MAP."""
+    local = {b: i for i, b in enumerate(tile_bits)}
+    items = [_Item(op, nqubits, i) for i, op in enumerate(ops)]
+    npred, succ = _dependencies(items, False)
+    ready = [i for i in range(len(items)) if npred[i] == 0]
+    isz = np.dtype(cdtype).itemsize
+    headers, entries = [], []  # per record: header bytes, [B, nent] entries or None
+    slots = []
+    cur = 0
+
+    def emit(op, arg, cmask, nt, bits, data, rreq=0):
+        nent = 0 if data is None else data.shape[-1]
+        hdr = np.zeros(1, dtype=_HDR)
+        hdr["op"], hdr["arg"], hdr["nt"], hdr["rreq"], hdr["nent"], hdr["cmask"] = op, arg, nt, rreq, nent, cmask
+        b = np.zeros(8, dtype=np.uint8)
+        b[: len(bits)] = bits
+        hdr["bits"] = b
+        headers.append(hdr.tobytes())
+        if data is None:
+            entries.append(None)
+        else:
+            d = np.broadcast_to(np.asarray(data, dtype=cdtype), (nbatch, nent))
+            entries.append(d)
+
+    def reg_bits(mapping):
+        return [tile_bits[lb] for lb in _REG_GROUPS[mapping]]
+
+    def split_cond(op, mapping):
+        rb = reg_bits(mapping)
+        cm, rreq = 0, 0
+        for q in op.controls:
+            b = nqubits - 1 - q
+            if b in rb:
+                rreq |= 1 << rb.index(b)
+            else:
+                cm |= 1 << b
+        return cm, rreq
+
+    def runs_here(i, mapping):
+        op = items[i].op
+        if op.kind in ("diag", "innerd"):
+            return True
+        if op.kind in ("dense", "inner1") and len(op.targets) == 1:
+            return local[nqubits - 1 - op.targets[0]] in _REG_GROUPS[mapping]
+        return False
+
+    def flat(data, nent):
+        return np.asarray(data).reshape(nbatch, nent)
+
+    for event, arg in _schedule(items, npred, succ, ready, runs_here):
+        if event == "map":
+            emit(OP_MAP, arg, 0, 0, (), None)
+            cur = arg
+            continue
+        op = items[arg].op
+        rb = reg_bits(cur)
+        if op.kind in ("diag", "innerd"):
+            tb = [nqubits - 1 - q for q in op.targets]
+            cb = [nqubits - 1 - q for q in op.controls]
+            nt = len(tb)
+            d = flat(op.data, 2**nt)
+            if op.kind == "diag" and all(b not in rb for b in tb + cb):
+                emit(OP_LANE_PHASE, 0, sum(1 << b for b in cb), nt, tb, d)
+                continue
+            cm, rreq = split_cond(op, cur)
+            enc = [(REG_FLAG | rb.index(b)) if b in rb else b for b in tb]
+            if op.kind == "diag":
+                emit(OP_DIAG, 0, cm, nt, enc, d, rreq)
+            else:
+                enc = enc + [0] * (6 - nt) + [op.slot & 255, op.slot >> 8]
+                emit(OP_INNER_DIAG, 0, cm, nt, enc, d, rreq)
+                slots.append(op.slot)
+            continue
+        if op.kind not in ("dense", "inner1") or len(op.targets) != 1:
+            raise ValueError(f"batched passes take one-qubit dense and diagonal operations, not {op.kind}")
+        lb = local[nqubits - 1 - op.targets[0]]
+        j = _REG_GROUPS[cur].index(lb)
+        cm, rreq = split_cond(op, cur)
+        m = flat(op.data, 4)
+        if op.kind == "inner1":
+            emit(OP_INNER1, j, cm, 0, [op.slot & 255, op.slot >> 8], m, rreq)
+            slots.append(op.slot)
+        else:
+            real = not np.any(np.imag(m))
+            emit(OP_DENSE1_REAL if real else OP_DENSE1, j, cm, 0, (), m, rreq)
+    # pack: the same headers for every set, entries per set, records padded to 16
+    cols = []
+    for hdr, ent in zip(headers, entries):
+        cols.append(np.broadcast_to(np.frombuffer(hdr, dtype=np.uint8), (nbatch, HEADER_BYTES)))
+        if ent is not None:
+            raw = np.ascontiguousarray(ent).view(np.uint8).reshape(nbatch, -1)
+            pad = (-raw.shape[1]) % 16
+            cols.append(raw)
+            if pad:
+                cols.append(np.zeros((nbatch, pad), dtype=np.uint8))
+    prog = np.concatenate(cols, axis=1) if cols else np.zeros((nbatch, 0), dtype=np.uint8)
+    return np.ascontiguousarray(prog), len(headers), slots
 
 
 def compile_circuit(ops: Sequence, nqubits: int, cdtype, low_bits: int, ordered: bool = False):

@@ -64,7 +64,9 @@ enum : uint32_t {
   kOpDense1Real = 4,
   kOpRegPhase = 5,
   kOpLanePhase = 6,
-  kOpNeg = 7
+  kOpNeg = 7,
+  kOpInner1 = 8,    // adjoint sweep only: Re <w| M |v> for a 2x2 M on a register bit
+  kOpInnerDiag = 9  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
 };
 
 // Record header (the host packs the same layout).
@@ -85,7 +87,9 @@ static_assert(sizeof(BlkHdr) == 32, "record header is 32 bytes");
 
 struct BlkArgs {
   uint64_t ntiles;
-  uint64_t nbytes;   // program size: records past it are never read
+  uint64_t nbytes;        // program size: records past it are never read
+  uint64_t state_stride;  // amplitudes between the states of a batch (grid z)
+  uint64_t prog_stride;   // bytes between the programs of a batch
   int ninstr;
   uint8_t tbits[kBTileBits];  // global bit of local tile bit i, ascending
 };
@@ -473,6 +477,8 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   __shared__ R lds[kBTile];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
+  psi += blockIdx.z * a.state_stride;  // state and program of this batch entry
+  prog += blockIdx.z * a.prog_stride;
   uint64_t gbit[kBTileBits];
 #pragma clang loop unroll(full)
   for (int i = 0; i < kBTileBits; ++i) gbit[i] = uint64_t(1) << a.tbits[i];
@@ -597,15 +603,263 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
 
 template <typename R>
 static int launch_blocked(void* state, int n, const int* tbits, const void* prog, size_t nbytes, int ninstr,
-                          hipStream_t s) {
+                          int nbatch, size_t prog_stride, hipStream_t s) {
   BlkArgs a;
   std::memset(&a, 0, sizeof(a));
   a.ntiles = uint64_t(1) << (n - kBTileBits);
   a.nbytes = nbytes;
+  a.state_stride = uint64_t(1) << n;
+  a.prog_stride = prog_stride;
   a.ninstr = ninstr;
   for (int i = 0; i < kBTileBits; ++i) a.tbits[i] = uint8_t(tbits[i]);
-  hipLaunchKernelGGL((k_blocked<R>), launch_grid(a.ntiles), dim3(kBT), 0, s, a,
-                     reinterpret_cast<const uint8_t*>(prog), reinterpret_cast<cplx<R>*>(state));
+  dim3 grid = launch_grid(a.ntiles);
+  grid.z = unsigned(nbatch);
+  hipLaunchKernelGGL((k_blocked<R>), grid, dim3(kBT), 0, s, a, reinterpret_cast<const uint8_t*>(prog),
+                     reinterpret_cast<cplx<R>*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint sweep: the same records applied to two states of each batch entry
+// (psi and lambda), plus INNER records that add Re <lambda| M |psi> over the
+// tile into per-wave partial sums.
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+  uint64_t ntiles;
+  uint64_t nbytes;
+  uint64_t state_stride;
+  uint64_t prog_stride;
+  int ninstr;
+  int nslots;  // INNER records per program
+  uint8_t tbits[kBTileBits];
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+template <typename R>
+__device__ __forceinline__ double re_conj_mul(cplx<R> w, double yr, double yi) {
+  return double(w.re) * yr + double(w.im) * yi;
+}
+
+// Re sum over the lane's pairs of conj(w) . (M v) for a 2x2 M on register bit J.
+template <typename R, int J>
+__device__ __forceinline__ double inner1_regs(const cplx<R>* v, const cplx<R>* w, const Hdr& h,
+                                              const cplx<R>* m) {
+  const double m0r = m[0].re, m0i = m[0].im, m1r = m[1].re, m1i = m[1].im;
+  const double m2r = m[2].re, m2i = m[2].im, m3r = m[3].re, m3i = m[3].im;
+  const uint32_t rreq = h.rreq;
+  double acc = 0.0;
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    if ((r >> J) & 1) continue;
+    if (!reg_ok(r, rreq)) continue;
+    const int r1 = r | (1 << J);
+    const double x0r = v[r].re, x0i = v[r].im, x1r = v[r1].re, x1i = v[r1].im;
+    const double y0r = m0r * x0r - m0i * x0i + m1r * x1r - m1i * x1i;
+    const double y0i = m0r * x0i + m0i * x0r + m1r * x1i + m1i * x1r;
+    const double y1r = m2r * x0r - m2i * x0i + m3r * x1r - m3i * x1i;
+    const double y1i = m2r * x0i + m2i * x0r + m3r * x1i + m3i * x1r;
+    acc += re_conj_mul(w[r], y0r, y0i) + re_conj_mul(w[r1], y1r, y1i);
+  }
+  return acc;
+}
+
+template <typename R>
+__device__ __forceinline__ double inner_diag_regs(const cplx<R>* v, const cplx<R>* w, const Hdr& h,
+                                                  const cplx<R>* m, uint64_t lane_idx) {
+  uint32_t rm[4];
+  const uint32_t dl = diag_lane_part(h, lane_idx, rm);
+  const uint32_t rreq = h.rreq;
+  double acc = 0.0;
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    if (!reg_ok(r, rreq)) continue;
+    const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
+                        ((r & 8) ? rm[3] : 0u);
+    const cplx<R> d = m[di];
+    const double yr = double(d.re) * double(v[r].re) - double(d.im) * double(v[r].im);
+    const double yi = double(d.re) * double(v[r].im) + double(d.im) * double(v[r].re);
+    acc += re_conj_mul(w[r], yr, yi);
+  }
+  return acc;
+}
+
+template <typename R>
+__device__ __forceinline__ void flush_lane_phase2(cplx<R>* v, cplx<R>* w, cplx<R>& P, bool& pdirty) {
+  if (!pdirty) return;
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    v[r] = cmul(P, v[r]);
+    w[r] = cmul(P, w[r]);
+  }
+  P.re = R(1);
+  P.im = R(0);
+  pdirty = false;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
+                                                         cplx<R>* __restrict__ psi, cplx<R>* __restrict__ lam,
+                                                         double* __restrict__ part) {
+  __shared__ R lds[kBTile];
+  const uint64_t tile = xcd_block();
+  if (tile >= a.ntiles) return;
+  const uint64_t bz = blockIdx.z;
+  psi += bz * a.state_stride;
+  lam += bz * a.state_stride;
+  prog += bz * a.prog_stride;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t gbit[kBTileBits];
+#pragma clang loop unroll(full)
+  for (int i = 0; i < kBTileBits; ++i) gbit[i] = uint64_t(1) << a.tbits[i];
+  const uint64_t tile_base = insert_zeros(tile, a.tbits, kBTileBits);
+  int map = 0;
+  uint64_t lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
+  cplx<R> v[kBReg], w[kBReg];
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    const uint64_t g = lane_idx | deposit(uint32_t(r) << 8, gbit);
+    v[r] = ld_nt(psi, g);
+    w[r] = ld_nt(lam, g);
+  }
+  cplx<R> P;
+  P.re = R(1);
+  P.im = R(0);
+  bool pdirty = false;
+  const uint64_t nbytes = a.nbytes;
+  uint64_t off = 0;
+  int slot = 0;
+  for (int k = 0; k < a.ninstr && off + sizeof(BlkHdr) <= nbytes; ++k) {
+    const Hdr h = load_hdr(prog + off);
+    const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
+    off += sizeof(BlkHdr) + ((uint64_t(h.nent) * sizeof(cplx<R>) + 15) & ~uint64_t(15));
+    const bool tile_ok = (tile_base & h.tmask) == h.tmask;
+    const bool lane_ok = tile_ok && (lane_idx & h.cmask) == h.cmask;
+    if (h.op == kOpInner1 || h.op == kOpInnerDiag) {
+      // every INNER record writes its partial, zero where the condition fails
+      double acc = 0.0;
+      if (tile_ok) {
+        if (h.op == kOpInnerDiag) {
+          acc = inner_diag_regs(v, w, h, m, lane_idx);
+        } else {
+          switch (h.arg & 3) {
+            case 0: acc = inner1_regs<R, 0>(v, w, h, m); break;
+            case 1: acc = inner1_regs<R, 1>(v, w, h, m); break;
+            case 2: acc = inner1_regs<R, 2>(v, w, h, m); break;
+            default: acc = inner1_regs<R, 3>(v, w, h, m); break;
+          }
+        }
+        // a pending lane factor P (a product of diagonal unitaries, applied
+        // to both states) drops out: conj(P w) M (P v) = |P|^2 conj(w) M v
+      }
+      acc = lane_ok ? acc : 0.0;
+      acc = wave_sum(acc);
+      if (lane == 0 && slot < a.nslots)
+        part[((bz * uint64_t(a.nslots) + uint64_t(slot)) * a.ntiles + tile) * 4 + wave] = acc;
+      ++slot;
+      continue;
+    }
+    if (!tile_ok) continue;
+    switch (h.op) {
+      case kOpDense1Real:
+      case kOpDense1: {
+        const bool real = h.op == kOpDense1Real;
+        switch ((h.arg & 3) | (real ? 4 : 0)) {
+          case 0: dense1_regs<R, 0, false>(v, h, m, lane_ok); dense1_regs<R, 0, false>(w, h, m, lane_ok); break;
+          case 1: dense1_regs<R, 1, false>(v, h, m, lane_ok); dense1_regs<R, 1, false>(w, h, m, lane_ok); break;
+          case 2: dense1_regs<R, 2, false>(v, h, m, lane_ok); dense1_regs<R, 2, false>(w, h, m, lane_ok); break;
+          case 3: dense1_regs<R, 3, false>(v, h, m, lane_ok); dense1_regs<R, 3, false>(w, h, m, lane_ok); break;
+          case 4: dense1_regs<R, 0, true>(v, h, m, lane_ok); dense1_regs<R, 0, true>(w, h, m, lane_ok); break;
+          case 5: dense1_regs<R, 1, true>(v, h, m, lane_ok); dense1_regs<R, 1, true>(w, h, m, lane_ok); break;
+          case 6: dense1_regs<R, 2, true>(v, h, m, lane_ok); dense1_regs<R, 2, true>(w, h, m, lane_ok); break;
+          default: dense1_regs<R, 3, true>(v, h, m, lane_ok); dense1_regs<R, 3, true>(w, h, m, lane_ok); break;
+        }
+        break;
+      }
+      case kOpDiag:
+        diag_regs(v, h, m, lane_idx, lane_ok);
+        diag_regs(w, h, m, lane_idx, lane_ok);
+        break;
+      case kOpLanePhase: {
+        uint32_t rm[4];
+        const uint32_t di = diag_lane_part(h, lane_idx, rm);
+        P = csel(lane_ok, cmul(m[di], P), P);
+        pdirty = true;
+        break;
+      }
+      case kOpMap: {
+        flush_lane_phase2(v, w, P, pdirty);
+        const int nmap = h.arg < 3 ? int(h.arg) : 0;
+        remap(lds, v, map, nmap);
+        remap(lds, w, map, nmap);
+        map = nmap;
+        lane_idx = tile_base | deposit(local_index(map, threadIdx.x, 0), gbit);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  flush_lane_phase2(v, w, P, pdirty);
+  if (map != 0) {
+    remap(lds, v, map, 0);
+    remap(lds, w, map, 0);
+    map = 0;
+    lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
+  }
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    const uint64_t g = lane_idx | deposit(uint32_t(r) << 8, gbit);
+    st_nt(psi, g, v[r]);
+    st_nt(lam, g, w[r]);
+  }
+}
+
+// out[b * nparams + param_of[s]] = 2 * (sum over tiles and waves of part[b][s]),
+// summed in a fixed order (one workgroup per (b, s)).
+__global__ __launch_bounds__(256) void k_adjoint_reduce(const double* __restrict__ part, uint64_t per_slot,
+                                                        int nslots, const int* __restrict__ param_of,
+                                                        int nparams, double* __restrict__ out) {
+  __shared__ double red[256];
+  const uint64_t bs = blockIdx.x;  // b * nslots + s
+  const double* src = part + bs * per_slot;
+  double acc = 0.0;
+  for (uint64_t i = threadIdx.x; i < per_slot; i += 256) acc += src[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t b = bs / uint64_t(nslots);
+    const int s = int(bs % uint64_t(nslots));
+    const int p = param_of[s];
+    if (p >= 0 && p < nparams) out[b * uint64_t(nparams) + uint64_t(p)] = 2.0 * red[0];
+  }
+}
+
+template <typename R>
+static int launch_adjoint(void* psi, void* lam, int n, const int* tbits, const void* prog, size_t nbytes,
+                          int ninstr, int nbatch, size_t prog_stride, double* part, int nslots, hipStream_t s) {
+  AdjArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.ntiles = uint64_t(1) << (n - kBTileBits);
+  a.nbytes = nbytes;
+  a.state_stride = uint64_t(1) << n;
+  a.prog_stride = prog_stride;
+  a.ninstr = ninstr;
+  a.nslots = nslots;
+  for (int i = 0; i < kBTileBits; ++i) a.tbits[i] = uint8_t(tbits[i]);
+  dim3 grid = launch_grid(a.ntiles);
+  grid.z = unsigned(nbatch);
+  hipLaunchKernelGGL((k_blocked_adjoint<R>), grid, dim3(kBT), 0, s, a, reinterpret_cast<const uint8_t*>(prog),
+                     reinterpret_cast<cplx<R>*>(psi), reinterpret_cast<cplx<R>*>(lam), part);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -617,6 +871,79 @@ using namespace qsim;
 extern "C" {
 
 size_t qsim_blocked_header_bytes(void) { return sizeof(BlkHdr); }
+
+static int check_blocked_args(int nqubits, int dtype, const int* tile_bits, const void* program_dev,
+                              size_t program_bytes, int ninstr, const char* what) {
+  QSIM_CHECK_ARG(tile_bits != nullptr, what);
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, what);
+  QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, what);
+  QSIM_CHECK_ARG(ninstr >= 0 && (ninstr == 0 || (program_dev != nullptr && program_bytes >= sizeof(BlkHdr))), what);
+  QSIM_CHECK_ARG((reinterpret_cast<uintptr_t>(program_dev) & 15) == 0, what);
+  for (int i = 0; i < kBTileBits; ++i) {
+    QSIM_CHECK_ARG(tile_bits[i] >= 0 && tile_bits[i] < nqubits, what);
+    QSIM_CHECK_ARG(i == 0 || tile_bits[i] > tile_bits[i - 1], what);
+  }
+  return QSIM_OK;
+}
+
+int qsim_apply_blocked_batched(void* states, int nbatch, int nqubits, int dtype, const int* tile_bits,
+                               const void* programs_dev, size_t program_stride, size_t program_bytes, int ninstr,
+                               void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(states != nullptr && nbatch >= 1 && nbatch <= 65535, "qsim_apply_blocked_batched: bad batch");
+  QSIM_CHECK_ARG(program_stride % 16 == 0 && (nbatch == 1 || program_stride >= program_bytes),
+                 "qsim_apply_blocked_batched: bad program stride");
+  if (check_blocked_args(nqubits, dtype, tile_bits, programs_dev, program_bytes, ninstr,
+                         "qsim_apply_blocked_batched: bad arguments") != QSIM_OK)
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return launch_blocked<double>(states, nqubits, tile_bits, programs_dev, program_bytes, ninstr, nbatch,
+                                  program_stride, s);
+  return launch_blocked<float>(states, nqubits, tile_bits, programs_dev, program_bytes, ninstr, nbatch,
+                               program_stride, s);
+}
+
+size_t qsim_adjoint_partials_count(int nqubits, int nbatch, int nslots) {
+  if (nqubits < kBTileBits || nqubits > QSIM_MAX_QUBITS || nbatch < 1 || nslots < 0) return 0;
+  return (size_t(1) << (nqubits - kBTileBits)) * 4 * size_t(nbatch) * size_t(nslots);
+}
+
+int qsim_adjoint_blocked(void* psi, void* lam, int nbatch, int nqubits, int dtype, const int* tile_bits,
+                         const void* programs_dev, size_t program_stride, size_t program_bytes, int ninstr,
+                         int nslots, double* partials, size_t partials_count, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(psi != nullptr && lam != nullptr && psi != lam, "qsim_adjoint_blocked: bad states");
+  QSIM_CHECK_ARG(nbatch >= 1 && nbatch <= 65535, "qsim_adjoint_blocked: bad batch");
+  QSIM_CHECK_ARG(program_stride % 16 == 0 && (nbatch == 1 || program_stride >= program_bytes),
+                 "qsim_adjoint_blocked: bad program stride");
+  QSIM_CHECK_ARG(nslots >= 0 && (nslots == 0 || partials != nullptr), "qsim_adjoint_blocked: bad slots");
+  QSIM_CHECK_ARG(partials_count >= qsim_adjoint_partials_count(nqubits, nbatch, nslots),
+                 "qsim_adjoint_blocked: partials buffer too small");
+  if (check_blocked_args(nqubits, dtype, tile_bits, programs_dev, program_bytes, ninstr,
+                         "qsim_adjoint_blocked: bad arguments") != QSIM_OK)
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return launch_adjoint<double>(psi, lam, nqubits, tile_bits, programs_dev, program_bytes, ninstr, nbatch,
+                                  program_stride, partials, nslots, s);
+  return launch_adjoint<float>(psi, lam, nqubits, tile_bits, programs_dev, program_bytes, ninstr, nbatch,
+                               program_stride, partials, nslots, s);
+}
+
+int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nslots, const int* param_of_dev,
+                        int nparams, double* grads, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(partials != nullptr && param_of_dev != nullptr && grads != nullptr, "qsim_adjoint_reduce: null");
+  QSIM_CHECK_ARG(nbatch >= 1 && nslots >= 1 && nparams >= 1, "qsim_adjoint_reduce: bad sizes");
+  QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, "qsim_adjoint_reduce: bad nqubits");
+  const uint64_t per_slot = (uint64_t(1) << (nqubits - kBTileBits)) * 4;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_adjoint_reduce, dim3(unsigned(uint64_t(nbatch) * uint64_t(nslots))), dim3(256), 0, s,
+                     partials, per_slot, nslots, param_of_dev, nparams, grads);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
 
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                        size_t program_bytes, int ninstr, void* stream) {
@@ -634,8 +961,8 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128)
-    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, s);
-  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, s);
+    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s);
+  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s);
 }
 
 }  // extern "C"

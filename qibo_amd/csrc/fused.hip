@@ -128,21 +128,31 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
   constexpr int MB = (2 * D) / 32;      // 32-row blocks of the real embedding (1 or 2)
   constexpr int KS = (2 * D) / 2;       // K = 2 steps over the 2D real inner dimension
   constexpr int NCOL = 1 << (kMfmaTileBits - K);
+  constexpr int NCG = NCOL / 32;        // 32-column groups (4 for K = 5, 8 for K = 4)
   __shared__ c64 tile[1 << kMfmaTileBits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t t = xcd_block();
   const uint64_t base = insert_zeros(t, a.tbits, kMfmaTileBits);
-  uint64_t gbit[kMfmaTileBits];
+  __shared__ uint32_t rowoff[D];  // local-index bits of matrix row r
+  if (threadIdx.x < D) rowoff[threadIdx.x] = mfma_local<K>(a, threadIdx.x, 0);
+  // coalesced streaming load: thread u of each 256-slice reads local index
+  // u, whose global offset is the lane part (bits 0-7) | the slice part (8-11)
+  uint64_t glow = 0;
 #pragma clang loop unroll(full)
-  for (int i = 0; i < kMfmaTileBits; ++i) gbit[i] = uint64_t(1) << a.tbits[i];
-  // coalesced streaming load of the tile (local index = linear position)
-  for (int u = threadIdx.x; u < (1 << kMfmaTileBits); u += 256) {
-    uint64_t g = base;
+  for (int i = 0; i < 8; ++i)
+    if ((threadIdx.x >> i) & 1) glow |= uint64_t(1) << a.tbits[i];
+  glow |= base;
+  c64 ld[16];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < kMfmaTileBits; ++i)
-      if ((u >> i) & 1) g |= gbit[i];
-    tile[u] = ld_nt(psi, g);
+  for (int k = 0; k < 16; ++k) {
+    uint64_t g = glow;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i)
+      if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
+    ld[k] = ld_nt(psi, g);
   }
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 16; ++k) tile[k * 256 + threadIdx.x] = ld[k];
   // A fragments of the real embedding: row i = 32 mb + l % 32, col k = 2 ks + l / 32
   float afrag[MB][KS];
 #pragma clang loop unroll(full)
@@ -159,22 +169,24 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
     }
   }
   __syncthreads();
-  for (int cg = wave; cg < NCOL / 32; cg += 4) {
-    const uint32_t col = 32 * cg + (lane & 31);
+  // one 32-column group per wave at a time; the groups are disjoint, so each
+  // wave reads and writes only its own columns of the tile
+  for (int cg = wave; cg < NCG; cg += 4) {
+    const uint32_t colpart = mfma_local<K>(a, 0, 32 * cg + (lane & 31));
     floatx16 acc[MB];
 #pragma clang loop unroll(full)
     for (int mb = 0; mb < MB; ++mb) acc[mb] = floatx16(0.0f);
 #pragma clang loop unroll(full)
     for (int ks = 0; ks < KS; ++ks) {
-      const int k = 2 * ks + (lane >> 5);  // real inner index: < D -> re of row k, else im of row k - D
-      const c64 x = tile[mfma_local<K>(a, uint32_t(k % D), col)];
-      const float b = (k < D) ? x.re : x.im;
+      // real inner index k = 2 ks + lane / 32: k < D -> re of row k, else im of row k - D
+      const c64 x = tile[rowoff[(2 * ks + (lane >> 5)) % D] | colpart];
+      const float b = (2 * ks + (lane >> 5) < D) ? x.re : x.im;
 #pragma clang loop unroll(full)
       for (int mb = 0; mb < MB; ++mb)
         acc[mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(afrag[mb][ks], b, acc[mb], 0, 0, 0);
     }
-    __syncthreads();  // all reads of this column group done before overwriting
-    // D rows: real part rows [0, D), imaginary part rows [D, 2D)
+    // D rows: real part rows [0, D), imaginary part rows [D, 2D); the wave's
+    // reads of these columns are complete (they fed the accumulators above)
 #pragma clang loop unroll(full)
     for (int j = 0; j < 16; ++j) {
       const int row = 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
@@ -182,20 +194,23 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
       for (int mb = 0; mb < MB; ++mb) {
         const int i = 32 * mb + row;
         if (i < D) {
-          tile[mfma_local<K>(a, uint32_t(i), col)].re = acc[mb][j];
+          tile[rowoff[i] | colpart].re = acc[mb][j];
         } else {
-          tile[mfma_local<K>(a, uint32_t(i - D), col)].im = acc[mb][j];
+          tile[rowoff[i - D] | colpart].im = acc[mb][j];
         }
       }
     }
-    __syncthreads();
   }
-  for (int u = threadIdx.x; u < (1 << kMfmaTileBits); u += 256) {
-    uint64_t g = base;
+  __syncthreads();
 #pragma clang loop unroll(full)
-    for (int i = 0; i < kMfmaTileBits; ++i)
-      if ((u >> i) & 1) g |= gbit[i];
-    st_nt(psi, g, tile[u]);
+  for (int k = 0; k < 16; ++k) ld[k] = tile[k * 256 + threadIdx.x];
+#pragma clang loop unroll(full)
+  for (int k = 0; k < 16; ++k) {
+    uint64_t g = glow;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i)
+      if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
+    st_nt(psi, g, ld[k]);
   }
 }
 

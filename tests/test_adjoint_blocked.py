@@ -1,0 +1,154 @@
+"""The batched blocked adjoint sweep, checked on the CPU.
+
+A numpy interpreter of the batched records (forward and reverse passes,
+INNER records accumulating Re <lambda| M |psi>) runs the programs that
+qibo_amd.adjoint compiles and must reproduce the gradients of the oracle's
+parameter-shift rule (the reference's derivative.py restated) and its
+energies. Needs no GPU.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import OracleBackend
+from qibo_amd import Circuit, blocking, gates, hamiltonians, models
+from qibo_amd.adjoint import _batched_ops
+from qibo_amd.backend import GatePlanner
+
+from tests.test_blocking import GROUPS, decode
+
+
+def _apply(states, n, tile_bits, recs, grads_out, slots, b):
+    cur = 0
+    psi, lam = states
+    idx = np.arange(2**n)
+    k_inner = 0
+    for op, arg, cmask, nt, bits, rreq, mat in recs:
+        rb = [tile_bits[lb] for lb in GROUPS[cur]]
+        active = (idx & cmask) == cmask
+        for j in range(4):
+            if (rreq >> j) & 1:
+                active &= ((idx >> rb[j]) & 1) == 1
+        if op == blocking.OP_MAP:
+            cur = arg
+            continue
+        if op in (blocking.OP_DIAG, blocking.OP_LANE_PHASE, blocking.OP_INNER_DIAG):
+            di = np.zeros_like(idx)
+            for tb in bits[:nt]:
+                if tb & blocking.REG_FLAG:
+                    tb = rb[tb & 3]
+                di = (di << 1) | ((idx >> tb) & 1)
+            d = mat[di]
+            if op == blocking.OP_INNER_DIAG:
+                grads_out[b, slots[k_inner]] = 2 * np.sum(np.real(np.conj(lam) * d * psi)[active])
+                k_inner += 1
+            else:
+                psi[:] = np.where(active, psi * d, psi)
+                lam[:] = np.where(active, lam * d, lam)
+            continue
+        bit = rb[arg]
+        m = mat[:4].reshape(2, 2)
+        i0 = idx[((idx >> bit) & 1) == 0]
+        i1 = i0 | (1 << bit)
+        act = active[i0]
+        if op == blocking.OP_INNER1:
+            y0 = m[0, 0] * psi[i0] + m[0, 1] * psi[i1]
+            y1 = m[1, 0] * psi[i0] + m[1, 1] * psi[i1]
+            val = np.real(np.conj(lam[i0]) * y0 + np.conj(lam[i1]) * y1)
+            grads_out[b, slots[k_inner]] = 2 * np.sum(val[act])
+            k_inner += 1
+            continue
+        assert op in (blocking.OP_DENSE1, blocking.OP_DENSE1_REAL)
+        for st in (psi, lam):
+            x0, x1 = st[i0].copy(), st[i1].copy()
+            st[i0] = np.where(act, m[0, 0] * x0 + m[0, 1] * x1, x0)
+            st[i1] = np.where(act, m[1, 0] * x0 + m[1, 1] * x1, x1)
+
+
+def _pauli_apply(ham, psi, n):
+    """sum_t c_t i^ny_t X^x_t Z^z_t psi (the Hamiltonian's Pauli tables)."""
+    xm, zm, ny, coeffs = ham.pauli_tables()
+    idx = np.arange(2**n)
+    out = np.zeros_like(psi)
+    for x, z, y, c in zip(xm, zm, ny, coeffs):
+        zi = idx & int(z)
+        sign = 1 - 2 * (np.array([bin(int(v)).count("1") for v in zi]) & 1)
+        out = out + c * (1j**y) * (sign * psi)[..., idx ^ int(x)]
+    return out
+
+
+def emulate_blocked_adjoint(circuit, ham, params):
+    n = circuit.nqubits
+    nb, npar = params.shape
+    gp = GatePlanner()
+    fwd, rev, _ = _batched_ops(gp, circuit, params)
+    psi = np.zeros((nb, 2**n), dtype=np.complex128)
+    psi[:, 0] = 1
+    grads = np.zeros((nb, npar))
+    for kind, (bits, pops) in blocking.partition(fwd, n, 3, singles=False):
+        progs, count, slots = blocking.compile_pass_batched(bits, pops, n, np.complex128, nb)
+        assert not slots
+        for b in range(nb):
+            recs = decode(progs[b].tobytes(), 0, count, 16)
+            _apply((psi[b], np.zeros_like(psi[b])), n, bits, recs, grads, slots, b)
+    lam = _pauli_apply(ham, psi, n)
+    energies = np.real(np.sum(np.conj(psi) * lam, axis=1)) + ham.constant
+    for kind, (bits, pops) in blocking.partition(rev, n, 3, singles=False):
+        progs, count, slots = blocking.compile_pass_batched(bits, pops, n, np.complex128, nb)
+        for b in range(nb):
+            recs = decode(progs[b].tobytes(), 0, count, 16)
+            _apply((psi[b], lam[b]), n, bits, recs, grads, slots, b)
+    return energies, grads
+
+
+def _ansatz(n, layers):
+    c = Circuit(n)
+    for _ in range(layers):
+        for q in range(n):
+            c.add(gates.RY(q, 0.1))
+            c.add(gates.RZ(q, 0.2))
+        for q in range(0, n - 1):
+            c.add(gates.CNOT(q, q + 1))
+        c.add(gates.CRX(0, n - 1, 0.3))
+        c.add(gates.CU1(n - 2, 1, 0.4))
+        c.add(gates.RZZ(2, 5, 0.5))
+        c.add(gates.RX(3, 0.6).controlled_by(7))
+    return c
+
+
+@pytest.mark.parametrize("n", [12, 13])
+def test_blocked_adjoint_matches_central_differences(n):
+    c = _ansatz(n, 2)
+    ham = hamiltonians.TFIM(n, h=0.7, closed_boundary=False, backend=GatePlanner())
+    npar = len(c.get_parameters())
+    rng = np.random.default_rng(n)
+    params = rng.uniform(0, 2 * np.pi, (2, npar))
+    energies, grads = emulate_blocked_adjoint(c, ham, params)
+    ob = OracleBackend()
+
+    def energy(theta):
+        c.set_parameters(theta)
+        psi = ob.execute(c)
+        return float(np.real(np.vdot(psi, _pauli_apply(ham, psi, n)))) + ham.constant
+
+    eps = 1e-5
+    for b in range(2):
+        assert abs(energies[b] - energy(params[b])) < 1e-10
+        for k in rng.choice(npar, 10, replace=False):
+            tp, tm = params[b].copy(), params[b].copy()
+            tp[k] += eps
+            tm[k] -= eps
+            fd = (energy(tp) - energy(tm)) / (2 * eps)
+            assert abs(grads[b, k] - fd) < 1e-7, (b, k)
+
+
+def test_layered_circuit_compiles_to_few_passes():
+    n = 20
+    c = models.hardware_efficient_ansatz(n, 4)
+    params = np.random.default_rng(0).uniform(0, 6, (3, len(c.get_parameters())))
+    fwd, rev, npar = _batched_ops(GatePlanner(), c, params)
+    assert npar == params.shape[1]
+    assert sum(op.kind.startswith("inner") for op in rev) == npar
+    passes = list(blocking.partition(rev, n, 3, singles=False))
+    assert all(k == "block" for k, _ in passes)
+    assert len(passes) < len(rev) / 10
