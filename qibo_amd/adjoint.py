@@ -256,6 +256,7 @@ def _batched_ops(backend, circuit, params: np.ndarray):
         raise _Unsupported("parameter count")
     cdt = np.complex128 if backend.dtype == "complex128" else np.complex64
     fwd, rev = [], []
+    fixed = {}  # plans of parameter-free gates, by matrix and qubits
     for g in queue:
         if not isinstance(g, gates.Gate) or isinstance(g, gates.SpecialGate):
             raise _Unsupported("special gate")
@@ -263,12 +264,16 @@ def _batched_ops(backend, circuit, params: np.ndarray):
         qubits = tuple(g.qubits) if not g.is_controlled_by else tuple(g.target_qubits)
         name = g.__class__.__name__
         if id(g) in slot_of and g.trainable:
-            mats = _batched_rotation(name, params[:, slot_of[id(g)]]).astype(cdt)
+            op, dag = _rotation_ops(g, name, params[:, slot_of[id(g)]], cdt)
         else:
+            # the same matrix for every set: plan it once, share its data
             m = np.asarray(g.matrix(backend), dtype=cdt)
-            mats = np.broadcast_to(m, (nb,) + m.shape)
-        op = _plan_batch(mats, qubits, controls)
-        dag = _plan_batch(np.conj(np.swapaxes(mats, 1, 2)), qubits, controls)
+            key = (m.tobytes(), qubits, tuple(controls))
+            if key not in fixed:
+                o1 = plan_matrix(m, qubits, controls)
+                o2 = plan_matrix(np.conj(m.T), qubits, controls)
+                fixed[key] = tuple(None if o.kind == "noop" else _shared(o, nb) for o in (o1, o2))
+            op, dag = fixed[key]
         if op is not None:
             fwd.append(op)
         step = []
@@ -282,6 +287,32 @@ def _batched_ops(backend, circuit, params: np.ndarray):
         rev.append(step)
     reverse = [op for step in reversed(rev) for op in step]
     return fwd, reverse, k
+
+
+def _shared(op, nb):
+    if op.kind not in ("dense", "diag") or (op.kind == "dense" and len(op.targets) != 1):
+        raise _Unsupported(op.kind)
+    d = np.asarray(op.data)
+    return BatchOp(op.kind, op.targets, op.controls, np.broadcast_to(d, (nb,) + d.shape))
+
+
+def _rotation_ops(g, name, th, cdt):
+    """Forward and dagger BatchOps of a trainable rotation gate for angles th [B],
+    in the form plan_matrix gives each set's matrix (dense 2x2 for X/Y
+    rotations, the diagonal for Z rotations, U1-type phases as a phase on the
+    subspace where target and controls are 1)."""
+    t = g.target_qubits
+    cq = tuple(g.control_qubits)
+    mats = _batched_rotation(name, th).astype(cdt)
+    if name.startswith("C"):
+        mats = mats[:, 2:, 2:]  # the controlled block
+    if name in ("RX", "RY", "CRX", "CRY"):
+        return (BatchOp("dense", t, cq, mats), BatchOp("dense", t, cq, np.conj(np.swapaxes(mats, 1, 2))))
+    d = np.diagonal(mats, axis1=1, axis2=2)
+    if name in ("U1", "CU1"):  # diag(1, e^{i theta}): the target joins the controls
+        d = d[:, 1:]
+        return (BatchOp("diag", (), cq + t, d), BatchOp("diag", (), cq + t, np.conj(d)))
+    return (BatchOp("diag", t, cq, d), BatchOp("diag", t, cq, np.conj(d)))
 
 
 def _upload_programs(progs: np.ndarray, device):

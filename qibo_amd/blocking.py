@@ -47,6 +47,7 @@ lane, ``rreq`` the register bits (as a 4-bit mask over register positions);
 DENSE_LDS keeps all condition bits in ``cmask``.
 """
 
+import struct
 from typing import Iterable, Iterator, Sequence, Tuple
 
 import numpy as np
@@ -159,6 +160,8 @@ def record_bytes(nent: int, itemsize: int) -> int:
 _HDR = np.dtype([("op", "u1"), ("arg", "u1"), ("nt", "u1"), ("rreq", "u1"), ("nent", "<u4"),
                  ("cmask", "<u8"), ("bits", "u1", (8,)), ("reserved", "<u8")])
 assert _HDR.itemsize == HEADER_BYTES
+_HDR_STRUCT = struct.Struct("<BBBBIQ8BQ")  # the same layout
+assert _HDR_STRUCT.size == HEADER_BYTES
 
 
 def _pack(buf: bytearray, op: int, arg: int, cmask: int, nt: int, bits: Sequence[int],
@@ -414,12 +417,8 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
 
     def emit(op, arg, cmask, nt, bits, data, rreq=0):
         nent = 0 if data is None else data.shape[-1]
-        hdr = np.zeros(1, dtype=_HDR)
-        hdr["op"], hdr["arg"], hdr["nt"], hdr["rreq"], hdr["nent"], hdr["cmask"] = op, arg, nt, rreq, nent, cmask
-        b = np.zeros(8, dtype=np.uint8)
-        b[: len(bits)] = bits
-        hdr["bits"] = b
-        headers.append(hdr.tobytes())
+        b = list(bits) + [0] * (8 - len(bits))
+        headers.append(_HDR_STRUCT.pack(op, arg, nt, rreq, nent, cmask, *b, 0))
         if data is None:
             entries.append(None)
         else:
