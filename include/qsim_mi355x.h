@@ -118,6 +118,13 @@ size_t qsim_blocked_header_bytes(void);
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                        size_t program_bytes, int ninstr, void* stream);
 
+/* qsim_apply_blocked with the input taken to be |0...0>: the buffer's old
+ * contents are never read, only the result is written. The first pass of
+ * Backend._execute_circuit from zero_state (abstract.py:2243-2275 then
+ * 3321-3322) without writing the zero state first. */
+int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
+                                 size_t program_bytes, int ninstr, void* stream);
+
 /* The blocked pass for a batch of `nbatch` states stored one after the other
  * (2^nqubits amplitudes apart), state b running the program at
  * programs_dev + b * program_stride (all programs program_bytes long, same

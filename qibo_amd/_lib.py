@@ -49,6 +49,7 @@ SIGNATURES = {
     "qsim_inner_matrix": (_i, [_vp, _vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp, _vp, _sz, _vp]),
     "qsim_blocked_header_bytes": (_sz, []),
     "qsim_apply_blocked": (_i, [_vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
+    "qsim_apply_blocked_from_zero": (_i, [_vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
     "qsim_apply_blocked_batched": (_i, [_vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
     "qsim_adjoint_partials_count": (_sz, [_i, _i, _i]),
     "qsim_adjoint_blocked": (_i, [_vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _i, _vp, _sz, _vp]),

@@ -531,15 +531,21 @@ class MI355XBackend(GatePlanner):
         from .result import CircuitResult, QuantumState
 
         nqubits = circuit.nqubits
-        state = self.zero_state(nqubits) if initial_state is None else initial_state
-        state = self.run_queue(circuit.queue, state, nqubits)
+        if initial_state is None and self.blocking and nqubits >= blocking.TILE_BITS:
+            # |0...0> is produced by the first blocked pass itself (no zero fill)
+            self._validate_nqubits(nqubits)
+            state = self._alloc_state(nqubits)  # budget check and MemoryError as zero_state
+            state = self.run_queue(circuit.queue, state, nqubits, zero_pending=True)
+        else:
+            state = self.zero_state(nqubits) if initial_state is None else initial_state
+            state = self.run_queue(circuit.queue, state, nqubits)
         if circuit.measurements:
             circuit._final_state = CircuitResult(state, circuit.measurements, backend=self, nshots=nshots)
         else:
             circuit._final_state = QuantumState(state, backend=self)
         return circuit._final_state
 
-    def run_queue(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+    def run_queue(self, queue, state: torch.Tensor, nqubits: int, zero_pending: bool = False) -> torch.Tensor:
         """Apply the gates of ``queue`` in order (the loop of abstract.py:3321-3322).
 
         With ``self.blocking`` (the default) consecutive gates are grouped into
@@ -550,10 +556,17 @@ class MI355XBackend(GatePlanner):
         that of applying the gates one by one.
         Measurement gates only register the backend (non-collapsing ``M`` is a
         no-op on the state); other special gates run in order between passes.
+        With ``zero_pending`` the contents of ``state`` are not yet written and
+        stand for |0...0>: a first blocked pass produces it itself, anything
+        else fills it first.
         """
         from .measurements_gate import M
         from .special import FusedGate
 
+        zero = [zero_pending]  # still to be materialised
+        if zero_pending and (not self.blocking or nqubits < blocking.TILE_BITS):
+            self._fill_zero(state, nqubits)
+            zero[0] = False
         if not self.blocking:
             for gate in queue:
                 state = gate.apply(self, state, nqubits)
@@ -576,7 +589,10 @@ class MI355XBackend(GatePlanner):
                 yield self.plan_gate(gate)
 
         while True:
-            state = self._run_blocked(segment(), state, nqubits)
+            state = self._run_blocked(segment(), state, nqubits, zero)
+            if zero[0]:  # no blocked pass produced it
+                self._fill_zero(state, nqubits)
+                zero[0] = False
             if not stop:
                 return state
             state = stop.pop().apply(self, state, nqubits)
@@ -612,10 +628,16 @@ class MI355XBackend(GatePlanner):
         state.copy_(big[: state.numel()])
         return state
 
-    def _run_blocked(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+    def _fill_zero(self, state: torch.Tensor, nqubits: int):
+        _lib.check(self.lib.qsim_zero_state(state.data_ptr(), nqubits, self._dtype_code(state),
+                                            self._stream(state)), self.lib)
+
+    def _run_blocked(self, ops, state: torch.Tensor, nqubits: int, zero=None) -> torch.Tensor:
         """Partition ``ops`` (an iterable of planned gates) into passes and
         launch each pass as soon as it is compiled, so the host plans and
-        compiles pass k+1 while the GPU runs pass k."""
+        compiles pass k+1 while the GPU runs pass k. ``zero`` = [True] means the
+        state still stands for |0...0>: the first pass starts from it (or the
+        state is filled before a one-gate launch); it is cleared once done."""
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         low = 3 if code == _lib.C128 else 4
@@ -624,6 +646,9 @@ class MI355XBackend(GatePlanner):
         cur = torch.cuda.current_stream(state.device)
         for kind, payload in blocking.partition(ops, nqubits, low, ordered):
             if kind == "single":
+                if zero is not None and zero[0]:
+                    self._fill_zero(state, nqubits)
+                    zero[0] = False
                 self.apply_op(payload, state, nqubits)
                 continue
             tile_bits, pops = payload
@@ -635,8 +660,12 @@ class MI355XBackend(GatePlanner):
             if timer is not None and timer[0](_BLOCK_OP):
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record(cur)
-            rc = self.lib.qsim_apply_blocked(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits),
-                                             prog.data_ptr(), len(program), count, stream)
+            launch = self.lib.qsim_apply_blocked
+            if zero is not None and zero[0]:
+                launch = self.lib.qsim_apply_blocked_from_zero
+                zero[0] = False
+            rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits), prog.data_ptr(), len(program),
+                        count, stream)
             _lib.check(rc, self.lib)
             if timer is not None and timer[0](_BLOCK_OP):
                 ev[1].record(cur)

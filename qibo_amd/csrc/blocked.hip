@@ -470,7 +470,8 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #define QSIM_BLOCKED_WAVES 4
 #endif
 
-template <typename R>
+// FROM_ZERO: the input is |0...0>, generated in registers (the state is only written).
+template <typename R, bool FROM_ZERO>
 __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_BLOCKED_WAVES)))
 void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
                                                  cplx<R>* __restrict__ psi) {
@@ -489,8 +490,16 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   uint64_t lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
 
   cplx<R> v[kBReg];
+  if constexpr (FROM_ZERO) {
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
+    for (int r = 0; r < kBReg; ++r) {
+      v[r].re = (lane_idx | deposit(uint32_t(r) << 8, gbit)) == 0 ? R(1) : R(0);
+      v[r].im = R(0);
+    }
+  } else {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
+  }
 
   cplx<R> P;  // pending per-lane factor of LANE_PHASE records
   P.re = R(1);
@@ -603,7 +612,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
 
 template <typename R>
 static int launch_blocked(void* state, int n, const int* tbits, const void* prog, size_t nbytes, int ninstr,
-                          int nbatch, size_t prog_stride, hipStream_t s) {
+                          int nbatch, size_t prog_stride, hipStream_t s, bool from_zero = false) {
   BlkArgs a;
   std::memset(&a, 0, sizeof(a));
   a.ntiles = uint64_t(1) << (n - kBTileBits);
@@ -614,8 +623,12 @@ static int launch_blocked(void* state, int n, const int* tbits, const void* prog
   for (int i = 0; i < kBTileBits; ++i) a.tbits[i] = uint8_t(tbits[i]);
   dim3 grid = launch_grid(a.ntiles);
   grid.z = unsigned(nbatch);
-  hipLaunchKernelGGL((k_blocked<R>), grid, dim3(kBT), 0, s, a, reinterpret_cast<const uint8_t*>(prog),
-                     reinterpret_cast<cplx<R>*>(state));
+  if (from_zero)
+    hipLaunchKernelGGL((k_blocked<R, true>), grid, dim3(kBT), 0, s, a, reinterpret_cast<const uint8_t*>(prog),
+                       reinterpret_cast<cplx<R>*>(state));
+  else
+    hipLaunchKernelGGL((k_blocked<R, false>), grid, dim3(kBT), 0, s, a, reinterpret_cast<const uint8_t*>(prog),
+                       reinterpret_cast<cplx<R>*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -943,6 +956,19 @@ int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nsl
                      partials, per_slot, nslots, param_of_dev, nparams, grads);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
+}
+
+int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
+                                 size_t program_bytes, int ninstr, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_apply_blocked_from_zero: null state");
+  if (check_blocked_args(nqubits, dtype, tile_bits, program_dev, program_bytes, ninstr,
+                         "qsim_apply_blocked_from_zero: bad arguments") != QSIM_OK)
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true);
+  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true);
 }
 
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,

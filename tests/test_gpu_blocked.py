@@ -107,3 +107,22 @@ def test_blocked_pass_count_and_norm(backend):
     assert abs(float(p.sum()) - 1.0) < 1e-12
     del state, p
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("first", ["blocked", "single", "empty"])
+def test_zero_state_produced_by_first_pass(backend, first):
+    """From |0...0> the first blocked pass writes the state without reading it;
+    a one-gate first launch, or no gate at all, fills the zero state first."""
+    n = 14
+    c = Circuit(n)
+    if first == "single":
+        u = np.linalg.qr(np.arange(256).reshape(16, 16) * 1j + np.eye(16))[0]
+        c.add(gates.Unitary(u, 0, 5, 9, 13))
+    if first != "empty":
+        for q in range(n):
+            c.add(gates.RY(q, 0.1 * (q + 1)))
+        c.add(gates.CZ(0, 1))
+    c.add(gates.M(*range(n)))
+    got = backend.execute_circuit(c, nshots=10).state().detach().cpu().numpy()
+    want = OracleBackend().execute(c)
+    assert np.max(np.abs(got - want)) <= 1e-12
