@@ -339,6 +339,13 @@ def main():
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
     # this same workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scales read off a
     # plain copy of one state buffer); only quoted for the workload it measured
+    # FP64 work of the timed passes (the same compile the backend performs)
+    from qibo_amd import blocking
+
+    ops = [backend.plan_gate(g) for g in circuit.queue if g.__class__.__name__ != "M"]
+    plan, program = blocking.compile_circuit(ops, n, np.complex128, 3)
+    pass_flops = [blocking.program_flops(program[e[2]:], e[3], 16) for e in plan if e[0] == "block"]
+    flops_per_launch = float(np.mean(pass_flops)) * 2**n if pass_flops else 0.0
     traffic, traffic_src = None, None
     prof = os.path.join(ROOT, "profiles", "k_blocked_layered30_c128.json")
     if n == 30 and args.nlayers == 20 and os.path.exists(prof):
@@ -385,6 +392,14 @@ def main():
             "avg_launch_ms": avg_dense_ms,
             "gates_per_launch": (sum(passes) / len(passes)) if passes else None,
             "passes_per_step": len(passes) / args.steps,
+            # the pass is bound by FP64 VALU work, not HBM: its FP64 rate against
+            # the MI355X FP64 peak (78.6 TFLOP/s counts every operation as an FMA)
+            "compute": {
+                "flops_per_launch": flops_per_launch,
+                "achieved_TFLOPs": flops_per_launch / (avg_dense_ms * 1e-3) / 1e12,
+                "peak_TFLOPs_fp64": 78.6,
+                "frac": flops_per_launch / (avg_dense_ms * 1e-3) / 78.6e12,
+            },
         },
         "copy_calibration": {
             "bytes": copy_bytes,

@@ -500,6 +500,30 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
     return np.ascontiguousarray(prog), len(headers), slots
 
 
+# FP64/FP32 floating-point operations per amplitude of the tile that each
+# record form performs (multiply = 1, fused multiply-add = 2), as the kernel
+# executes them: real 2x2: 2 mul + 2 fma per component; complex 2x2: a complex
+# multiply (6) and a complex multiply-add (8); diagonal / table: a complex
+# multiply; a per-lane factor: one complex multiply per 16 amplitudes;
+# dense 2^k block: 6 + 8 (2^k - 1).
+_FLOPS = {OP_DENSE1_REAL: 6.0, OP_DENSE1: 14.0, OP_DIAG: 6.0, OP_REG_PHASE: 6.0, OP_LANE_PHASE: 6.0 / 16,
+          OP_NEG: 0.0, OP_MAP: 0.0}
+
+
+def program_flops(program: bytes, count: int, itemsize: int) -> float:
+    """Floating-point operations per amplitude of one blocked pass's records."""
+    off, total = 0, 0.0
+    for _ in range(count):
+        op, arg = program[off], program[off + 1]
+        nent = int.from_bytes(program[off + 4: off + 8], "little")
+        if op == OP_DENSE_LDS:
+            total += 6.0 + 8.0 * (2**arg - 1)
+        else:
+            total += _FLOPS.get(op, 0.0)
+        off += record_bytes(nent, itemsize)
+    return total
+
+
 def compile_circuit(ops: Sequence, nqubits: int, cdtype, low_bits: int, ordered: bool = False):
     """Partition and compile: a list of ("single", op) / ("block", tile_bits, byte_offset,
     count, ngates) plus the concatenated records of all blocked passes."""
