@@ -67,7 +67,7 @@ _SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype
 class _Item:
     """A planned op with its qubit sets as amplitude-index bit positions."""
 
-    __slots__ = ("op", "nd_targets", "all_bits", "idx")
+    __slots__ = ("op", "nd_targets", "all_bits", "idx", "nd_mask", "all_mask", "ok")
 
     def __init__(self, op, nqubits, idx):
         self.op = op
@@ -76,6 +76,9 @@ class _Item:
         cb = {nqubits - 1 - q for q in op.controls}
         self.nd_targets = frozenset() if op.kind in ("diag", "innerd") else frozenset(tb)
         self.all_bits = frozenset(tb | cb)
+        self.nd_mask = sum(1 << b for b in self.nd_targets)  # the same sets as bit masks
+        self.all_mask = sum(1 << b for b in self.all_bits)
+        self.ok = blockable(op)
 
 
 def _commute(x: _Item, y: _Item) -> bool:
@@ -91,6 +94,28 @@ def blockable(op) -> bool:
 
 
 WINDOW = 768  # planned gates a pass may look ahead over
+
+
+def _scan(items, tile, ordered):
+    """Greedy pass over ``items`` starting from the tile bits ``tile``: take
+    every gate that commutes with all gates deferred before it and whose
+    non-diagonal targets fit in the 12 tile bits. Returns (chosen, deferred, tile)."""
+    tmask = sum(1 << b for b in tile)
+    chosen, deferred = [], []
+    td = ad = 0  # non-diagonal targets / all bits of the deferred items (masks)
+    for it in items:
+        nd = it.nd_mask
+        ok = it.ok and not (nd & ad) and not (td & it.all_mask)
+        if ok and nd:
+            ok = (tmask | nd).bit_count() <= TILE_BITS
+        if ok and not (ordered and deferred):
+            chosen.append(it)
+            tmask |= nd
+        else:
+            deferred.append(it)
+            td |= nd
+            ad |= it.all_mask
+    return chosen, deferred, {b for b in range(tmask.bit_length()) if (tmask >> b) & 1}
 
 
 def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
@@ -127,20 +152,18 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
             del items[0]
             fill()
             continue
-        tile = set(range(low_bits))
-        chosen, deferred = [], []
-        td, ad = set(), set()  # non-diagonal targets / all bits of deferred items
-        for it in items:
-            ok = blockable(it.op) and not (it.nd_targets & ad) and not (td & it.all_bits)
-            if ok and it.nd_targets:
-                ok = len(tile | it.nd_targets) <= TILE_BITS
-            if ok and not (ordered and deferred):
-                chosen.append(it)
-                tile |= it.nd_targets
-            else:
-                deferred.append(it)
-                td |= it.nd_targets
-                ad |= it.all_bits
+        chosen, deferred, tile = _scan(items, set(range(low_bits)), ordered)
+        if not ordered:
+            # also try tiles of low bits + a window of consecutive bits, and
+            # keep the tile that takes the most gates: on circuits of
+            # neighbouring gates a contiguous block of qubits can advance
+            # several layers before its boundary stops it
+            width = TILE_BITS - low_bits
+            for start in range(low_bits, nqubits - width + 1):
+                seed = set(range(low_bits)) | set(range(start, start + width))
+                c2, d2, t2 = _scan(items, seed, False)
+                if len(c2) > len(chosen):
+                    chosen, deferred, tile = c2, d2, t2
         items[:] = deferred
         fill()
         if len(chosen) == 1 and singles:
