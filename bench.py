@@ -18,6 +18,7 @@ sample of the same circuit's gates on this host.
 
 Beside the headline line the same JSON object carries the other parts of
 the metric, each on its own bounded workload:
+* "qft": qibo's QFT circuit at 30 qubits in complex128 and complex64;
 * "fused_c64": the layered circuit at 32 qubits in complex64 and complex128
   (64 GiB state) through blocked passes; 24 dense 5-qubit blocks at 32 qubits
   (complex64 on MFMA, complex128 in FP64) with ms per block, HBM GB/s and
@@ -190,6 +191,32 @@ def bench_fused(seed, n_time=32, n_fid=30, nlayers=20, nblocks=24):
     res["max_abs_err"] = float(torch.max(torch.abs(a - s128)).item())
     del s64, s128, a
     torch.cuda.empty_cache()
+    return res
+
+
+def bench_qft(seed, n=30):
+    """qibo's QFT (models.QFT) at 30 qubits: H, controlled phases and the final
+    SWAPs, complex128 and complex64 (the circuit of qibo's own benchmarks)."""
+    import torch
+
+    from qibo_amd import models
+    from qibo_amd.backend import MI355XBackend
+
+    res = {"nqubits": n}
+    c = models.QFT(n)
+    res["gates"] = c.ngates
+    for dtype in ("complex128", "complex64"):
+        backend = MI355XBackend(dtype=dtype)
+        c._final_state = None
+        backend.execute_circuit(c)
+        c._final_state = None
+        torch.cuda.empty_cache()
+        r, dt = _timed(lambda: backend.execute_circuit(c))
+        del r
+        c._final_state = None
+        torch.cuda.empty_cache()
+        res[f"{dtype}_s"] = dt
+        res[f"{dtype}_gate_apps_per_s"] = c.ngates / dt
     return res
 
 
@@ -408,7 +435,8 @@ def main():
         },
     }
     if not args.no_extras:
-        for key, fn in (("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint)):
+        for key, fn in (("qft", bench_qft), ("fused_c64", bench_fused), ("serving", bench_serving),
+                        ("adjoint", bench_adjoint)):
             try:
                 out[key] = fn(args.seed)
             except Exception as exc:  # noqa: BLE001 - an extra must not hide the headline line

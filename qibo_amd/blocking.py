@@ -245,6 +245,7 @@ def _dependencies(items, ordered):
 
 
 _RVALS = np.array([[(r >> j) & 1 for r in range(16)] for j in range(4)], dtype=np.int64)  # [j, r]
+_RFLIP = [np.arange(16) ^ (1 << j) for j in range(4)]  # register r -> r with bit j flipped
 
 
 def _schedule(items, npred, succ, ready, runs_here):
@@ -313,11 +314,16 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         _pack(buf, op, arg, cmask, nt, bits, data, cdtype, rreq)
         count += 1
 
-    def flush_reg():
+    def flush_reg(j=None):
+        """Emit the folded table; before a register gate on register bit j
+        only when the table depends on that bit (otherwise the two commute)."""
         nonlocal pending
-        if pending is not None:
-            emit(OP_REG_PHASE, 0, 0, 0, (), pending)
-            pending = None
+        if pending is None:
+            return
+        if j is not None and np.array_equal(pending, pending[_RFLIP[j]]):
+            return
+        emit(OP_REG_PHASE, 0, 0, 0, (), pending)
+        pending = None
 
     def reg_bits(mapping):
         return [tile_bits[lb] for lb in _REG_GROUPS[mapping]]  # global bit of register bit j
@@ -382,8 +388,8 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         if op.kind == "diag":
             emit_diag(op)
         elif op.kind == "dense" and len(op.targets) == 1 and runs_here(pick, cur):
-            flush_reg()
             lb = local[nqubits - 1 - op.targets[0]]
+            flush_reg(_REG_GROUPS[cur].index(lb))
             cm, rreq = split_cond(op, cur)
             m = np.asarray(op.data)
             real = (not ordered) and not np.any(np.imag(m))

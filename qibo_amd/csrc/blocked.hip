@@ -344,14 +344,22 @@ __device__ __forceinline__ void diag_regs(cplx<R>* v, const Hdr& h, const cplx<R
   uint32_t rm[4];
   const uint32_t dl = diag_lane_part(h, lane_idx, rm);
   const uint32_t rreq = h.rreq;
-  if (h.nt <= 1) {  // entries are uniform: scalar loads and a per-register select
+  if (h.nt <= 1) {
+    // Uniform entries. Two per-lane factors, chosen once: f0 for amplitudes
+    // whose diagonal index is the lane part alone, f1 for those a register
+    // bit sets; a lane outside the condition multiplies by exactly 1 (the
+    // per-gate arithmetic, cmul(d, v), for the others).
     const cplx<R> d0 = m[0], d1 = m[h.nt];
+    cplx<R> one;
+    one.re = R(1);
+    one.im = R(0);
+    const cplx<R> f0 = csel(lane_ok, dl ? d1 : d0, one);
+    const cplx<R> f1 = csel(lane_ok, d1, one);
 #pragma clang loop unroll(full)
     for (int r = 0; r < kBReg; ++r) {
       if (!reg_ok(r, rreq)) continue;
-      const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
-                          ((r & 8) ? rm[3] : 0u);
-      v[r] = csel(lane_ok, cmul(di ? d1 : d0, v[r]), v[r]);
+      const bool rbit = ((r & 1) && rm[0]) || ((r & 2) && rm[1]) || ((r & 4) && rm[2]) || ((r & 8) && rm[3]);
+      v[r] = cmul(rbit ? f1 : f0, v[r]);
     }
     return;
   }
