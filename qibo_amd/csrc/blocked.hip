@@ -167,7 +167,13 @@ __device__ __forceinline__ void lds_put(R* lds, const R* x, int map) {
   const uint32_t s = swz(lane_local(map, threadIdx.x));
   const int r0 = reg_base(map);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) lds[s ^ swz(uint32_t(r) << r0)] = x[r];
+  for (int r = 0; r < kBReg; ++r) {
+    // one XOR per access with a uniform constant: the 16 slots are not kept
+    // live across the transpose (they would cost 16 registers each way)
+    uint32_t sr = s;
+    asm volatile("" : "+v"(sr));
+    lds[sr ^ swz(uint32_t(r) << r0)] = x[r];
+  }
 }
 
 template <typename R>
@@ -175,7 +181,11 @@ __device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
   const uint32_t s = swz(lane_local(map, threadIdx.x));
   const int r0 = reg_base(map);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) x[r] = lds[s ^ swz(uint32_t(r) << r0)];
+  for (int r = 0; r < kBReg; ++r) {
+    uint32_t sr = s;
+    asm volatile("" : "+v"(sr));
+    x[r] = lds[sr ^ swz(uint32_t(r) << r0)];
+  }
 }
 
 // Change of register mapping: amplitude (lane, r) of mapping `from` moves to
@@ -614,6 +624,9 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
     map = 0;
     lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
   }
+  // recompute the store addresses here instead of keeping the load addresses
+  // live through the instruction loop (which spilled them to scratch)
+  asm volatile("" : "+v"(lane_idx));
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) st_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit), v[r]);
 }
