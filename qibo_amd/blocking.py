@@ -41,6 +41,13 @@ lane):
   mapping change or LDS gate (it commutes with every register-local
   operation in between).
 * NEG: the -1 phase of CZ-like gates under register and lane conditions.
+* ROT: an unconditional one-qubit rotation [[c, -s], [s, c]] (RY), reflection
+  [[c, s], [s, -c]] (H) or [[c, -is], [-is, c]] (RX) on a register bit as
+  three shears x0 -= t x1; x1 += s x0; x0 -= t x1 with t = s / (1 + c)
+  (Paeth): three multiply-adds per real pair instead of four operations. The
+  shears need c >= 0; a gate with c < 0 is applied as -(its negation), and
+  the pass's net sign is folded into the data of one unconditional linear
+  record of the pass (or, if it has none, a final sign flip).
 
 Controls are split: ``cmask`` holds the condition bits that are fixed per
 lane, ``rreq`` the register bits (as a 4-bit mask over register positions);
@@ -59,6 +66,8 @@ PAD_BYTES = 1056  # readable bytes required past a record stream (QSIM_BLOCKED_P
 
 OP_DIAG, OP_DENSE1, OP_MAP, OP_DENSE_LDS = 0, 1, 2, 3
 OP_DENSE1_REAL, OP_REG_PHASE, OP_LANE_PHASE, OP_NEG = 4, 5, 6, 7
+OP_ROT = 10
+ROT_RY, ROT_H, ROT_RX = 0, 1, 2  # ROT modes (arg = register bit | mode << 2)
 REG_FLAG = 0x80  # Diag target byte: register bit j of the current mapping is REG_FLAG | j
 _REG_GROUPS = {0: (8, 9, 10, 11), 1: (4, 5, 6, 7), 2: (0, 1, 2, 3)}
 _SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=complex)
@@ -187,6 +196,28 @@ _HDR_STRUCT = struct.Struct("<BBBBIQ8BQ")  # the same layout
 assert _HDR_STRUCT.size == HEADER_BYTES
 
 
+def rotation_form(m: np.ndarray):
+    """(mode, t, s, negated) when the 2x2 ``m`` is exactly one of the ROT
+    forms, else None. negated: m = -(the form with c >= 0 that is encoded)."""
+    m = np.asarray(m, dtype=np.complex128).reshape(2, 2)
+    re, im = m.real, m.imag
+    if not np.any(im) and re[0, 0] == re[1, 1] and re[0, 1] == -re[1, 0]:
+        mode, c, s = ROT_RY, re[0, 0], re[1, 0]
+    elif not np.any(im) and re[0, 0] == -re[1, 1] and re[0, 1] == re[1, 0]:
+        mode, c, s = ROT_H, re[0, 0], re[1, 0]
+    elif (re[0, 0] == re[1, 1] and im[0, 0] == 0 and im[1, 1] == 0 and re[0, 1] == 0
+          and re[1, 0] == 0 and im[0, 1] == im[1, 0]):
+        mode, c, s = ROT_RX, re[0, 0], -im[0, 1]
+    else:
+        return None
+    if abs(c * c + s * s - 1.0) > 1e-14:  # the shears reproduce only an exact rotation
+        return None
+    neg = bool(c < 0)
+    if neg:
+        c, s = -c, -s
+    return mode, s / (1.0 + c), s, neg
+
+
 def _pack(buf: bytearray, op: int, arg: int, cmask: int, nt: int, bits: Sequence[int],
           data, cdtype, rreq: int = 0):
     """Append one variable-length record: header (u8 op, u8 arg, u8 nt, u8 rreq,
@@ -308,9 +339,14 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
     switches = 0
     cur = 0
     pending = None  # REG_PHASE table being folded (complex128, 16 entries)
+    sign = False  # net -1 left by ROT records of gates with c < 0
+    absorb = None  # (byte offset, entries) of the first unconditional linear record
 
     def emit(op, arg, cmask, nt, bits, data, rreq=0):
-        nonlocal count
+        nonlocal count, absorb
+        if absorb is None and cmask == 0 and rreq == 0 and data is not None and op in (
+                OP_DENSE1, OP_DENSE1_REAL, OP_REG_PHASE, OP_LANE_PHASE, OP_DIAG):
+            absorb = (len(buf) + HEADER_BYTES, np.asarray(data).size)
         _pack(buf, op, arg, cmask, nt, bits, data, cdtype, rreq)
         count += 1
 
@@ -392,8 +428,15 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             flush_reg(_REG_GROUPS[cur].index(lb))
             cm, rreq = split_cond(op, cur)
             m = np.asarray(op.data)
+            j = _REG_GROUPS[cur].index(lb)
+            rot = None if (ordered or cm or rreq) else rotation_form(m)
+            if rot is not None:
+                mode, t, sn, neg = rot
+                emit(OP_ROT, j | (mode << 2), 0, 0, (), np.array([complex(t, sn)]))
+                sign ^= neg
+                continue
             real = (not ordered) and not np.any(np.imag(m))
-            emit(OP_DENSE1_REAL if real else OP_DENSE1, _REG_GROUPS[cur].index(lb), cm, 0, (), m.ravel()[:4], rreq)
+            emit(OP_DENSE1_REAL if real else OP_DENSE1, j, cm, 0, (), m.ravel()[:4], rreq)
         else:
             flush_reg()
             mat = _SWAP if op.kind == "swap" else op.data
@@ -402,6 +445,16 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             cm = sum(1 << (nqubits - 1 - q) for q in op.controls)
             emit(OP_DENSE_LDS, k, cm, 0, lbits + [0] * (4 - k) + sorted(lbits), mat)
     flush_reg()
+    if sign:
+        # a global -1 commutes with every record: scale one unconditional
+        # linear record's entries (all of them), else flip every sign at the end
+        if absorb is not None:
+            off, nent = absorb
+            view = np.frombuffer(buf, dtype=cdtype, count=nent, offset=off)
+            view *= -1
+            del view
+        else:
+            emit(OP_NEG, 0, 0, 0, (), None)
     return bytes(buf), count, switches
 
 
@@ -531,11 +584,12 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
 
 # FP64/FP32 floating-point operations per amplitude of the tile that each
 # record form performs (multiply = 1, fused multiply-add = 2), as the kernel
-# executes them: real 2x2: 2 mul + 2 fma per component; complex 2x2: a complex
+# executes them: real 2x2: 2 mul + 2 fma per component; rotation: 3 fma per
+# real pair; complex 2x2: a complex
 # multiply (6) and a complex multiply-add (8); diagonal / table: a complex
 # multiply; a per-lane factor: one complex multiply per 16 amplitudes;
 # dense 2^k block: 6 + 8 (2^k - 1).
-_FLOPS = {OP_DENSE1_REAL: 6.0, OP_DENSE1: 14.0, OP_DIAG: 6.0, OP_REG_PHASE: 6.0, OP_LANE_PHASE: 6.0 / 16,
+_FLOPS = {OP_DENSE1_REAL: 6.0, OP_ROT: 6.0, OP_DENSE1: 14.0, OP_DIAG: 6.0, OP_REG_PHASE: 6.0, OP_LANE_PHASE: 6.0 / 16,
           OP_NEG: 0.0, OP_MAP: 0.0}
 
 

@@ -65,14 +65,15 @@ enum : uint32_t {
   kOpRegPhase = 5,
   kOpLanePhase = 6,
   kOpNeg = 7,
-  kOpInner1 = 8,    // adjoint sweep only: Re <w| M |v> for a 2x2 M on a register bit
-  kOpInnerDiag = 9  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
+  kOpInner1 = 8,     // adjoint sweep only: Re <w| M |v> for a 2x2 M on a register bit
+  kOpInnerDiag = 9,  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
+  kOpRot = 10        // unconditional 2x2 rotation / reflection on a register bit, as three shears
 };
 
 // Record header (the host packs the same layout).
 struct BlkHdr {
   uint8_t op;
-  uint8_t arg;       // DENSE1*: register bit 0..3; MAP: mapping 0..2; DENSE_LDS: number of targets 1..3
+  uint8_t arg;       // DENSE1*: register bit 0..3; ROT: bit | mode << 2; MAP: mapping 0..2; DENSE_LDS: number of targets 1..3
   uint8_t nt;        // DIAG / LANE_PHASE: number of target bits (0..5)
   uint8_t rreq;      // register-bit conditions (bit j: register bit j must be 1)
   uint32_t nent;     // complex entries following the header
@@ -242,6 +243,87 @@ __device__ __forceinline__ void real2_inplace(float& x0, float& x1, float a, flo
       "v_fma_f32 %[x1], %[d], %[x1], %[t1]"
       : [x0] "+v"(x0), [x1] "+v"(x1), [t0] "=&v"(t0), [t1] "=&v"(t1)
       : [a] "s"(a), [b] "s"(b), [c] "s"(c), [d] "s"(d));
+}
+
+// One shear step in place, d <- F(d, a*x) with a uniform:
+// F = 0: d - a*x, 1: d + a*x, 2: a*x - d (source negation is free in VOP3).
+template <int F>
+__device__ __forceinline__ void shear_inplace(double& d, double a, double x) {
+  if constexpr (F == 0)
+    asm("v_fma_f64 %[d], -%[a], %[x], %[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+  else if constexpr (F == 1)
+    asm("v_fma_f64 %[d], %[a], %[x], %[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+  else
+    asm("v_fma_f64 %[d], %[a], %[x], -%[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+}
+
+template <int F>
+__device__ __forceinline__ void shear_inplace(float& d, float a, float x) {
+  if constexpr (F == 0)
+    asm("v_fma_f32 %[d], -%[a], %[x], %[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+  else if constexpr (F == 1)
+    asm("v_fma_f32 %[d], %[a], %[x], %[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+  else
+    asm("v_fma_f32 %[d], %[a], %[x], -%[d]" : [d] "+v"(d) : [a] "s"(a), [x] "v"(x));
+}
+
+// The rotation [[c, -s], [s, c]] of the real pair (x0, x1) as three shears
+// with t = s / (1 + c), c >= 0 (Paeth's decomposition):
+//   x0 -= t x1;  x1 += s x0;  x0 -= t x1
+// -- three multiply-adds instead of two multiplies and two multiply-adds.
+// In exact arithmetic 1 - t s = c and t (2 - t s) = s, so the product is the
+// rotation; each step rounds once. NEG=true rotates by the opposite angle.
+template <bool NEG, typename R>
+__device__ __forceinline__ void rot_pair(R& x0, R& x1, R t, R s) {
+  if constexpr (!NEG) {
+    shear_inplace<0>(x0, t, x1);
+    shear_inplace<1>(x1, s, x0);
+    shear_inplace<0>(x0, t, x1);
+  } else {
+    shear_inplace<1>(x0, t, x1);
+    shear_inplace<0>(x1, s, x0);
+    shear_inplace<1>(x0, t, x1);
+  }
+}
+
+// The reflection [[c, s], [s, -c]] = rotation . diag(1, -1), same shears:
+//   x0 += t x1;  x1 = s x0 - x1;  x0 -= t x1
+template <typename R>
+__device__ __forceinline__ void refl_pair(R& x0, R& x1, R t, R s) {
+  shear_inplace<1>(x0, t, x1);
+  shear_inplace<2>(x1, s, x0);
+  shear_inplace<0>(x0, t, x1);
+}
+
+// ROT record on register bit J, mode = arg >> 2, entry 0 = (t, s):
+//   0: [[c, -s], [s, c]]   (RY)           -- each component rotated
+//   1: [[c, s], [s, -c]]   (H)            -- each component reflected
+//   2: [[c, -is], [-is, c]] (RX)          -- (x0.im, x1.re) rotated by the
+//      angle, (x0.re, x1.im) by its opposite
+template <typename R, int J>
+__device__ __forceinline__ void rot_regs(cplx<R>* v, uint32_t mode, R t, R s) {
+  if (mode == 0) {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      rot_pair<false>(v[r].re, v[r | (1 << J)].re, t, s);
+      rot_pair<false>(v[r].im, v[r | (1 << J)].im, t, s);
+    }
+  } else if (mode == 1) {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      refl_pair(v[r].re, v[r | (1 << J)].re, t, s);
+      refl_pair(v[r].im, v[r | (1 << J)].im, t, s);
+    }
+  } else {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      rot_pair<false>(v[r].im, v[r | (1 << J)].re, t, s);
+      rot_pair<true>(v[r].re, v[r | (1 << J)].im, t, s);
+    }
+  }
 }
 
 // x *= d with d uniform: exactly cmul(d, x) (re = fma(d.re, x.re, -(d.im x.im)),
@@ -562,6 +644,17 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
           default: dense1_regs<R, 3, false>(v, h, m, lane_ok); break;
         }
         break;
+      case kOpRot: {
+        const R t = m[0].re, sn = m[0].im;
+        const uint32_t mode = h.arg >> 2;
+        switch (h.arg & 3) {
+          case 0: rot_regs<R, 0>(v, mode, t, sn); break;
+          case 1: rot_regs<R, 1>(v, mode, t, sn); break;
+          case 2: rot_regs<R, 2>(v, mode, t, sn); break;
+          default: rot_regs<R, 3>(v, mode, t, sn); break;
+        }
+        break;
+      }
       case kOpRegPhase:
 #pragma clang loop unroll(full)
         for (int r = 0; r < kBReg; ++r) cmul_inplace(v[r], m[r]);

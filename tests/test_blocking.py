@@ -83,6 +83,36 @@ def run_pass(state, n, tile_bits, instrs):
             tile = tile * mat[r]
         elif op == blocking.OP_MAP:
             cur = arg
+        elif op == blocking.OP_ROT:
+            # three shears per real pair (the kernel's order of steps), c >= 0
+            assert cmask == 0 and rreq == 0
+            lb = GROUPS[cur][arg & 3]
+            mode = arg >> 2
+            t, s = mat[0].real, mat[0].imag
+            i0 = u[(u >> lb) & 1 == 0]
+            i1 = i0 | (1 << lb)
+            x0r, x0i = tile[:, i0].real.copy(), tile[:, i0].imag.copy()
+            x1r, x1i = tile[:, i1].real.copy(), tile[:, i1].imag.copy()
+
+            def shears(a, b, t, s):
+                a = a - t * b
+                b = b + s * a
+                return a - t * b, b
+
+            if mode == blocking.ROT_RY:
+                x0r, x1r = shears(x0r, x1r, t, s)
+                x0i, x1i = shears(x0i, x1i, t, s)
+            elif mode == blocking.ROT_H:
+                for a, b in ((x0r, x1r), (x0i, x1i)):
+                    a += t * b
+                    b[:] = s * a - b
+                    a -= t * b
+            else:
+                assert mode == blocking.ROT_RX
+                x0i, x1r = shears(x0i, x1r, t, s)
+                x0r, x1i = shears(x0r, x1i, -t, -s)
+            tile[:, i0] = x0r + 1j * x0i
+            tile[:, i1] = x1r + 1j * x1i
         elif op in (blocking.OP_DENSE1, blocking.OP_DENSE1_REAL):
             lb = GROUPS[cur][arg]
             m = mat[:4].reshape(2, 2)
@@ -250,13 +280,14 @@ def test_fast_forms_used_on_layered_circuit():
             continue
         for rec in decode(program, entry[2], entry[3], 16):
             kinds[rec[0]] = kinds.get(rec[0], 0) + 1
-    # RY is real: every register one-qubit gate uses the real form
-    assert kinds.get(blocking.OP_DENSE1, 0) == 0 and kinds[blocking.OP_DENSE1_REAL] > 0
+    # RY is a rotation: every register one-qubit gate uses the shear form
+    assert kinds.get(blocking.OP_DENSE1, 0) == 0 and kinds.get(blocking.OP_DENSE1_REAL, 0) == 0
+    assert kinds[blocking.OP_ROT] > 0
     # RZ / CZ fold into register tables (several gates per record), sign flips
     # or per-lane factors; no general per-amplitude diagonal is left
     assert kinds[blocking.OP_REG_PHASE] > 0 and kinds.get(blocking.OP_DIAG, 0) == 0
     ngates = sum(e[4] for e in plan if e[0] == "block")
-    folded = ngates - kinds[blocking.OP_DENSE1_REAL] - kinds.get(blocking.OP_NEG, 0) \
+    folded = ngates - kinds[blocking.OP_ROT] - kinds.get(blocking.OP_NEG, 0) \
         - kinds.get(blocking.OP_LANE_PHASE, 0)
     assert kinds[blocking.OP_REG_PHASE] < folded / 2
 
@@ -282,3 +313,62 @@ def test_padding_contract_matches_header():
     assert pad == blocking.PAD_BYTES
     # the largest record's entries, read from a record that starts inside the stream
     assert pad >= blocking.record_bytes(blocking.NMAT, 16)
+
+
+@pytest.mark.parametrize("theta", [0.3, 2.0, 3.1, 4.0, 6.2, -1.7, np.pi])
+def test_rotation_forms(theta):
+    from qibo_amd import matrices
+
+    for gate, mode in ((gates.RY(0, theta), blocking.ROT_RY), (gates.RX(0, theta), blocking.ROT_RX)):
+        m = GatePlanner().matrix(gate)
+        mode_got, t, s, neg = blocking.rotation_form(m)
+        assert mode_got == mode
+        c, sn = np.cos(theta / 2), np.sin(theta / 2)
+        assert neg == (c < 0) and abs(t) <= 1 + 1e-15
+        assert abs(s - (-sn if neg else sn)) < 1e-16
+        # the shear parameters reproduce the (sign-adjusted) matrix: 1 - t s = |c|, t (2 - t s) = s
+        assert abs((1 - t * s) - abs(c)) < 1e-15 and abs(t * (2 - t * s) - s) < 1e-15
+    assert blocking.rotation_form(GatePlanner().matrix(gates.H(0)))[0] == blocking.ROT_H
+    for g in (gates.RZ(0, theta), gates.U3(0, theta, 0.2, 0.3), gates.X(0) if theta > 10 else gates.S(0)):
+        assert blocking.rotation_form(GatePlanner().matrix(g)) is None
+
+
+def _rotation_only_circuit(n, angles):
+    c = Circuit(n)
+    for k, a in enumerate(angles):
+        c.add(gates.RY(k % 3, a))
+    return c
+
+
+@pytest.mark.parametrize("angles,expect_neg", [([4.0, 0.5, 1.0], True), ([4.0, 4.5, 1.0], False)])
+def test_rotation_sign_fold(angles, expect_neg, rng):
+    """A pass with only rotations and an odd number of c < 0 ends with a sign flip;
+    with an even number no record is added. Either way the state is exact."""
+    n = 13
+    c = _rotation_only_circuit(n, angles)
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi /= np.linalg.norm(psi)
+    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    got, plan = run_plan(ops, n, psi)
+    want = OracleBackend().execute(c, psi)
+    assert np.max(np.abs(got - want)) < 1e-14
+    _, program = blocking.compile_circuit(ops, n, np.complex128, 3)
+    recs = decode(program, 0, plan[0][3], 16)
+    assert [r[0] for r in recs].count(blocking.OP_NEG) == (1 if expect_neg else 0)
+
+
+def test_rotation_sign_absorbed(rng):
+    """With an unconditional linear record in the pass the sign goes into its entries."""
+    n = 13
+    c = _rotation_only_circuit(n, [4.0, 0.5])
+    c.add(gates.RZ(1, 0.7))
+    c.add(gates.RY(1, 0.2))
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi /= np.linalg.norm(psi)
+    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    got, plan = run_plan(ops, n, psi)
+    want = OracleBackend().execute(c, psi)
+    assert np.max(np.abs(got - want)) < 1e-14
+    _, program = blocking.compile_circuit(ops, n, np.complex128, 3)
+    kinds = [r[0] for r in decode(program, 0, plan[0][3], 16)]
+    assert blocking.OP_NEG not in kinds and blocking.OP_REG_PHASE in kinds
