@@ -163,30 +163,34 @@ __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
 // values: 32 KB in complex128, 16 KB in complex64), so a transpose moves the
 // real parts, then the imaginary parts. Half the LDS of a whole complex tile
 // lets twice as many workgroups share a CU.
+// Byte offset of register r's slot: the lane's part XOR a uniform constant,
+// formed by one VALU instruction right before each access (volatile, so the
+// 16 offsets are not kept live across the transpose: they would cost 16
+// registers each way).
+__device__ __forceinline__ uint32_t slot_bytes(uint32_t lane_bytes, uint32_t reg_bytes) {
+  uint32_t a;
+  asm volatile("v_xor_b32 %0, %1, %2" : "=v"(a) : "s"(reg_bytes), "v"(lane_bytes));
+  return a;
+}
+
 template <typename R>
 __device__ __forceinline__ void lds_put(R* lds, const R* x, int map) {
-  const uint32_t s = swz(lane_local(map, threadIdx.x));
+  const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
   const int r0 = reg_base(map);
+  char* base = reinterpret_cast<char*>(lds);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) {
-    // one XOR per access with a uniform constant: the 16 slots are not kept
-    // live across the transpose (they would cost 16 registers each way)
-    uint32_t sr = s;
-    asm volatile("" : "+v"(sr));
-    lds[sr ^ swz(uint32_t(r) << r0)] = x[r];
-  }
+  for (int r = 0; r < kBReg; ++r)
+    *reinterpret_cast<R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R)))) = x[r];
 }
 
 template <typename R>
 __device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
-  const uint32_t s = swz(lane_local(map, threadIdx.x));
+  const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
   const int r0 = reg_base(map);
+  const char* base = reinterpret_cast<const char*>(lds);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) {
-    uint32_t sr = s;
-    asm volatile("" : "+v"(sr));
-    x[r] = lds[sr ^ swz(uint32_t(r) << r0)];
-  }
+  for (int r = 0; r < kBReg; ++r)
+    x[r] = *reinterpret_cast<const R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R))));
 }
 
 // Change of register mapping: amplitude (lane, r) of mapping `from` moves to
