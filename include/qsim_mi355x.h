@@ -105,14 +105,17 @@ int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, in
  * entries, padded to 16 bytes. Ops: 0 diagonal on up to 5 bits, 1 complex 2x2
  * on a register bit of the current mapping, 2 change of register mapping,
  * 3 dense 2^k block on k <= 3 local tile bits, 4 real 2x2 on a register bit,
- * 5 16-entry register phase table, 6 per-lane diagonal factor, 7 sign flip.
+ * 5 16-entry register phase table, 6 per-lane diagonal factor, 7 sign flip,
+ * 10 unconditional rotation / reflection on a register bit as three shears
+ * (arg = bit | mode << 2, one entry (t, s); mode 0 [[c,-s],[s,c]],
+ * 1 [[c,s],[s,-c]], 2 [[c,-is],[-is,c]], with t = s / (1 + c), c >= 0).
  * The buffer must stay readable for program_bytes + QSIM_BLOCKED_PAD bytes
- * (the kernel reads one header ahead and trusts each record's entry count,
- * so no read leaves that range); records are read while they start inside
- * program_bytes. A program of ops 0-3 only gives
- * bitwise the state of applying the same gates one by one with
- * qsim_apply_dense / qsim_apply_diag / qsim_apply_swap; ops 4-7 reorder
- * roundings. Requires nqubits >= 12. */
+ * (the kernel reads one header ahead and reads at most 64 entries of a
+ * record, so no read leaves that range); records are read while they start
+ * inside program_bytes, which must be below 2^31. A program of ops 0-3 only
+ * gives bitwise the state of applying the same gates one by one with
+ * qsim_apply_dense / qsim_apply_diag / qsim_apply_swap; the other ops
+ * reorder roundings. Requires nqubits >= 12. */
 #define QSIM_BLOCKED_PAD 1056
 size_t qsim_blocked_header_bytes(void);
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,

@@ -129,6 +129,8 @@ __device__ __forceinline__ uint32_t hbit(const Hdr& h, uint32_t t) {
 // read-ahead of one header and the entry reads of a record that starts inside
 // the stream never leave the allocation.
 constexpr uint64_t kBlockedPad = 1056;
+constexpr uint32_t kMaxEntries = 64;  // largest record: an 8x8 block
+constexpr size_t kMaxProgramBytes = size_t(1) << 31;  // record offsets are 32-bit
 
 __device__ __forceinline__ int reg_base(int map) { return map == 0 ? 8 : (map == 1 ? 4 : 0); }
 
@@ -386,7 +388,7 @@ __device__ __forceinline__ cplx<R> csel(bool c, cplx<R> a, cplx<R> b) {
 // Lane conditions are applied with selects, not branches: a divergent branch
 // around register updates makes the compiler copy the whole register array.
 template <typename R, int J, bool REAL>
-__device__ __forceinline__ void dense1_regs(cplx<R>* v, const Hdr& h, const cplx<R>* m, bool lane_ok) {
+__device__ __forceinline__ void dense1_regs(cplx<R>* v, const Hdr& h, const cplx<R>* m, uint64_t lane_idx) {
   const cplx<R> m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
   const uint32_t rreq = h.rreq;
   if (rreq == 0 && h.cmask == 0) {  // unconditional: straight-line code
@@ -402,6 +404,7 @@ __device__ __forceinline__ void dense1_regs(cplx<R>* v, const Hdr& h, const cplx
     }
     return;
   }
+  const bool lane_ok = (lane_idx & h.cmask) == h.cmask;
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) {
     if ((r >> J) & 1) continue;
@@ -613,39 +616,42 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   // The record stream is read ahead by one header. The buffer is readable for
   // nbytes + kBlockedPad bytes, so a header or entry read that starts inside
   // nbytes stays inside the allocation whatever the record claims.
-  const uint64_t nbytes = __builtin_amdgcn_readfirstlane(uint32_t(a.nbytes)) |
-                          (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(a.nbytes >> 32))) << 32);
+  // 32-bit offsets (the host keeps programs below 2^31 bytes): the bound
+  // checks then stay on the scalar unit (it has no 64-bit unsigned compare).
+  const uint32_t nbytes = __builtin_amdgcn_readfirstlane(uint32_t(a.nbytes));
   const uint64_t tb_lo = __builtin_amdgcn_readfirstlane(uint32_t(tile_base));
   const uint64_t tb_hi = __builtin_amdgcn_readfirstlane(uint32_t(tile_base >> 32));
   const uint64_t tile_base_s = tb_lo | (tb_hi << 32);  // uniform copy (scalar registers)
-  uint64_t off = 0;
+  uint32_t off = 0;
   Hdr h = {};
   if (a.ninstr > 0) h = load_hdr(prog);
-  for (int k = 0; k < a.ninstr && off + sizeof(BlkHdr) <= nbytes; ++k) {
+  for (int k = 0; k < a.ninstr && off + uint32_t(sizeof(BlkHdr)) <= nbytes; ++k) {
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
-    off += sizeof(BlkHdr) + ((uint64_t(h.nent) * sizeof(cplx<R>) + 15) & ~uint64_t(15));
-    const uint64_t next = off < nbytes ? off : nbytes;  // stays inside the padded buffer
+    const uint32_t nent = h.nent < kMaxEntries ? h.nent : kMaxEntries;  // no record is larger
+    off += uint32_t(sizeof(BlkHdr)) + ((nent * uint32_t(sizeof(cplx<R>)) + 15u) & ~15u);
+    const uint32_t next = off < nbytes ? off : nbytes;  // stays inside the padded buffer
     const Hdr hn = load_hdr(prog + next);  // next header, in flight while this record runs
     if ((tile_base_s & h.tmask) != h.tmask) {  // condition fails on the whole tile
       h = hn;
       continue;
     }
-    const bool lane_ok = (lane_idx & h.cmask) == h.cmask;
+    // the lane condition is formed only by the forms that use it
+    auto lane_ok = [&]() { return (lane_idx & h.cmask) == h.cmask; };
     switch (h.op) {
       case kOpDense1Real:
         switch (h.arg & 3) {
-          case 0: dense1_regs<R, 0, true>(v, h, m, lane_ok); break;
-          case 1: dense1_regs<R, 1, true>(v, h, m, lane_ok); break;
-          case 2: dense1_regs<R, 2, true>(v, h, m, lane_ok); break;
-          default: dense1_regs<R, 3, true>(v, h, m, lane_ok); break;
+          case 0: dense1_regs<R, 0, true>(v, h, m, lane_idx); break;
+          case 1: dense1_regs<R, 1, true>(v, h, m, lane_idx); break;
+          case 2: dense1_regs<R, 2, true>(v, h, m, lane_idx); break;
+          default: dense1_regs<R, 3, true>(v, h, m, lane_idx); break;
         }
         break;
       case kOpDense1:
         switch (h.arg & 3) {
-          case 0: dense1_regs<R, 0, false>(v, h, m, lane_ok); break;
-          case 1: dense1_regs<R, 1, false>(v, h, m, lane_ok); break;
-          case 2: dense1_regs<R, 2, false>(v, h, m, lane_ok); break;
-          default: dense1_regs<R, 3, false>(v, h, m, lane_ok); break;
+          case 0: dense1_regs<R, 0, false>(v, h, m, lane_idx); break;
+          case 1: dense1_regs<R, 1, false>(v, h, m, lane_idx); break;
+          case 2: dense1_regs<R, 2, false>(v, h, m, lane_idx); break;
+          default: dense1_regs<R, 3, false>(v, h, m, lane_idx); break;
         }
         break;
       case kOpRot: {
@@ -668,9 +674,9 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         const uint32_t di = diag_lane_part(h, lane_idx, rm);  // no register bits: rm stays 0
         if (h.nt <= 1) {  // uniform entries: scalar loads and a per-lane select
           const cplx<R> d0 = m[0], d1 = m[h.nt];
-          P = csel(lane_ok, cmul(di ? d1 : d0, P), P);
+          P = csel(lane_ok(), cmul(di ? d1 : d0, P), P);
         } else {
-          P = csel(lane_ok, cmul(m[di], P), P);
+          P = csel(lane_ok(), cmul(m[di], P), P);
         }
         pdirty = true;
         break;
@@ -678,7 +684,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
       case kOpNeg: {
         // flip the sign bits where the lane condition holds: one XOR per component
         const uint32_t rreq = h.rreq;
-        const uint32_t sign = lane_ok ? 0x80000000u : 0u;
+        const uint32_t sign = lane_ok() ? 0x80000000u : 0u;
 #pragma clang loop unroll(full)
         for (int r = 0; r < kBReg; ++r) {
           if (!reg_ok(r, rreq)) continue;
@@ -688,7 +694,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
       }
       case kOpDiag:
-        diag_regs(v, h, m, lane_idx, lane_ok);
+        diag_regs(v, h, m, lane_idx, lane_ok());
         break;
       case kOpMap: {
         flush_lane_phase(v, P, pdirty);
@@ -901,14 +907,14 @@ __global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const 
       case kOpDense1: {
         const bool real = h.op == kOpDense1Real;
         switch ((h.arg & 3) | (real ? 4 : 0)) {
-          case 0: dense1_regs<R, 0, false>(v, h, m, lane_ok); dense1_regs<R, 0, false>(w, h, m, lane_ok); break;
-          case 1: dense1_regs<R, 1, false>(v, h, m, lane_ok); dense1_regs<R, 1, false>(w, h, m, lane_ok); break;
-          case 2: dense1_regs<R, 2, false>(v, h, m, lane_ok); dense1_regs<R, 2, false>(w, h, m, lane_ok); break;
-          case 3: dense1_regs<R, 3, false>(v, h, m, lane_ok); dense1_regs<R, 3, false>(w, h, m, lane_ok); break;
-          case 4: dense1_regs<R, 0, true>(v, h, m, lane_ok); dense1_regs<R, 0, true>(w, h, m, lane_ok); break;
-          case 5: dense1_regs<R, 1, true>(v, h, m, lane_ok); dense1_regs<R, 1, true>(w, h, m, lane_ok); break;
-          case 6: dense1_regs<R, 2, true>(v, h, m, lane_ok); dense1_regs<R, 2, true>(w, h, m, lane_ok); break;
-          default: dense1_regs<R, 3, true>(v, h, m, lane_ok); dense1_regs<R, 3, true>(w, h, m, lane_ok); break;
+          case 0: dense1_regs<R, 0, false>(v, h, m, lane_idx); dense1_regs<R, 0, false>(w, h, m, lane_idx); break;
+          case 1: dense1_regs<R, 1, false>(v, h, m, lane_idx); dense1_regs<R, 1, false>(w, h, m, lane_idx); break;
+          case 2: dense1_regs<R, 2, false>(v, h, m, lane_idx); dense1_regs<R, 2, false>(w, h, m, lane_idx); break;
+          case 3: dense1_regs<R, 3, false>(v, h, m, lane_idx); dense1_regs<R, 3, false>(w, h, m, lane_idx); break;
+          case 4: dense1_regs<R, 0, true>(v, h, m, lane_idx); dense1_regs<R, 0, true>(w, h, m, lane_idx); break;
+          case 5: dense1_regs<R, 1, true>(v, h, m, lane_idx); dense1_regs<R, 1, true>(w, h, m, lane_idx); break;
+          case 6: dense1_regs<R, 2, true>(v, h, m, lane_idx); dense1_regs<R, 2, true>(w, h, m, lane_idx); break;
+          default: dense1_regs<R, 3, true>(v, h, m, lane_idx); dense1_regs<R, 3, true>(w, h, m, lane_idx); break;
         }
         break;
       }
@@ -1009,6 +1015,7 @@ static int check_blocked_args(int nqubits, int dtype, const int* tile_bits, cons
   QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, what);
   QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, what);
   QSIM_CHECK_ARG(ninstr >= 0 && (ninstr == 0 || (program_dev != nullptr && program_bytes >= sizeof(BlkHdr))), what);
+  QSIM_CHECK_ARG(program_bytes < kMaxProgramBytes, what);
   QSIM_CHECK_ARG((reinterpret_cast<uintptr_t>(program_dev) & 15) == 0, what);
   for (int i = 0; i < kBTileBits; ++i) {
     QSIM_CHECK_ARG(tile_bits[i] >= 0 && tile_bits[i] < nqubits, what);
@@ -1097,6 +1104,7 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
   QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, "qsim_apply_blocked: nqubits out of range");
   QSIM_CHECK_ARG(ninstr >= 0 && (ninstr == 0 || (program_dev != nullptr && program_bytes >= sizeof(BlkHdr))),
                  "qsim_apply_blocked: bad program");
+  QSIM_CHECK_ARG(program_bytes < kMaxProgramBytes, "qsim_apply_blocked: program of 2^31 bytes or more");
   static_assert(kBlockedPad == QSIM_BLOCKED_PAD, "padding contract of the C ABI");
   QSIM_CHECK_ARG((reinterpret_cast<uintptr_t>(program_dev) & 15) == 0, "qsim_apply_blocked: program not 16-byte aligned");
   for (int i = 0; i < kBTileBits; ++i) {
