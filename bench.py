@@ -351,6 +351,7 @@ def main():
 
     # events around every blocked-pass launch on the stream it runs on
     backend.block_stats.clear()
+    hits0 = backend.program_cache_hits
     timer = backend.enable_kernel_timer("block")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -361,6 +362,20 @@ def main():
     elapsed = time.perf_counter() - t0
     kernel_ms = backend.disable_kernel_timer()
     passes = list(backend.block_stats)
+    cache_hits = backend.program_cache_hits - hits0
+
+    # the same steps with the compiled-program cache off: every step plans and
+    # compiles its passes on the host again (overlapped with the GPU after the
+    # first pass)
+    backend.program_cache = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, s = step()
+        del r, s
+    torch.cuda.synchronize()
+    elapsed_nocache = time.perf_counter() - t0
+    backend.program_cache = True
 
     ms_per_step = 1000.0 * elapsed / args.steps
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
@@ -406,6 +421,14 @@ def main():
             "parallelism": "single GPU",
         },
         "hbm_gbs_step": step_bytes / (elapsed / args.steps) / 1e9,
+        # the backend keeps the compiled passes of a repeated identical gate
+        # queue (same gates, qubits and parameter values); every timed step
+        # still runs all passes, probabilities and sampling on the GPU
+        "host_program_cache": {
+            "hits_in_timed_steps": cache_hits,
+            "ms_per_step_compiling_every_step": 1000.0 * elapsed_nocache / args.steps,
+            "value_compiling_every_step": ngates * args.steps / elapsed_nocache,
+        },
         "roofline": {
             "bound": "hbm",
             "kernel": "k_blocked<double> (blocked pass: state read + written once)",

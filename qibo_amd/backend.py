@@ -283,6 +283,12 @@ class MI355XBackend(GatePlanner):
         # path); the default lets commuting gates move ahead, giving fewer passes
         self.blocking_ordered = False
         self.block_stats = collections.deque(maxlen=1 << 16)  # gates per blocked pass (diagnostics)
+        # compiled blocked passes of recently executed gate queues, keyed by the
+        # gates' kinds, qubits and exact parameter values (see run_queue)
+        self.program_cache = True
+        self.program_cache_size = 16
+        self._programs = collections.OrderedDict()
+        self.program_cache_hits = 0
         rec = self.lib.qsim_blocked_header_bytes()
         if rec != blocking.HEADER_BYTES:
             raise ImportError(f"qibo_amd: blocked-pass record header of {rec} bytes does not match the host layout")
@@ -573,6 +579,16 @@ class MI355XBackend(GatePlanner):
             return state
         if nqubits < blocking.TILE_BITS:
             return self._run_small(queue, state, nqubits)
+        key = self._queue_key(queue, nqubits, state, zero_pending) if self.program_cache else None
+        if key is not None:
+            actions = self._programs.get(key)
+            if actions is not None:
+                self._programs.move_to_end(key)
+                self.program_cache_hits += 1
+                return self._replay(actions, queue, state, nqubits, zero)
+            record = []
+        else:
+            record = None
         gates_left = iter(queue)
         stop = []  # the special gate that ended the current segment
 
@@ -589,13 +605,90 @@ class MI355XBackend(GatePlanner):
                 yield self.plan_gate(gate)
 
         while True:
-            state = self._run_blocked(segment(), state, nqubits, zero)
+            state = self._run_blocked(segment(), state, nqubits, zero, record)
             if zero[0]:  # no blocked pass produced it
                 self._fill_zero(state, nqubits)
                 zero[0] = False
             if not stop:
+                if record is not None:
+                    self._programs[key] = record
+                    while len(self._programs) > self.program_cache_size:
+                        self._programs.popitem(last=False)
                 return state
             state = stop.pop().apply(self, state, nqubits)
+
+    # -- compiled-program cache ---------------------------------------------
+    @staticmethod
+    def _param_key(p):
+        if isinstance(p, (float, np.floating)):
+            return float(p).hex()  # exact value, sign of zero included
+        if isinstance(p, (int, np.integer, bool)):
+            return int(p)
+        if isinstance(p, (complex, np.complexfloating)):
+            return (float(p.real).hex(), float(p.imag).hex())
+        if isinstance(p, np.ndarray):
+            return (p.dtype.str, p.shape, p.tobytes())
+        if isinstance(p, torch.Tensor):
+            a = p.detach().cpu().numpy()
+            return (a.dtype.str, a.shape, a.tobytes())
+        return None  # unknown parameter type: not cacheable
+
+    def _gate_key(self, gate):
+        from .measurements_gate import M
+        from .special import FusedGate
+
+        if isinstance(gate, M):
+            return None if gate.collapse else ("M",)
+        if isinstance(gate, FusedGate):
+            parts = tuple(self._gate_key(g) for g in gate.gates)
+            return None if any(k is None for k in parts) else ("Fused", gate.target_qubits, parts)
+        if isinstance(gate, gates_mod.SpecialGate):
+            return None
+        params = gate.parameters if isinstance(gate, gates_mod.ParametrizedGate) else ()
+        pk = tuple(self._param_key(x) for x in params)
+        if any(k is None for k in pk):
+            return None
+        return (type(gate).__name__, gate.target_qubits, gate.control_qubits, gate.is_controlled_by, pk)
+
+    def _queue_key(self, queue, nqubits, state, zero_pending):
+        """Exact description of what run_queue would compile for ``queue``:
+        every gate's class, qubits and parameter values, the register size,
+        dtype, device, the pass options and whether the first pass starts from
+        |0...0>. None when some gate (a callback, a collapsing measurement, an
+        unknown parameter type) cannot be described."""
+        parts = []
+        for gate in queue:
+            k = self._gate_key(gate)
+            if k is None:
+                return None
+            parts.append(k)
+        return (tuple(parts), nqubits, str(state.dtype), str(state.device), self.dtype,
+                bool(zero_pending), bool(self.blocking_ordered))
+
+    def _replay(self, actions, queue, state, nqubits, zero):
+        """Run the recorded launches of an identical queue: the same blocked
+        programs (still on the device) and one-gate kernels, in the same order."""
+        from .measurements_gate import M
+
+        for gate in queue:
+            if isinstance(gate, M):
+                gate.result.backend = self
+        code = self._dtype_code(state)
+        stream = self._stream(state)
+        cur = torch.cuda.current_stream(state.device)
+        for act in actions:
+            if act[0] == "single":
+                if zero[0]:
+                    self._fill_zero(state, nqubits)
+                    zero[0] = False
+                self.apply_op(act[1], state, nqubits)
+                continue
+            _, tile_bits, prog, nbytes, count, ngates = act
+            self._launch_blocked(state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero)
+        if zero[0]:
+            self._fill_zero(state, nqubits)
+            zero[0] = False
+        return state
 
     def _run_small(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
         """Circuits on fewer than 12 qubits: run as one blocked pass on a
@@ -632,12 +725,13 @@ class MI355XBackend(GatePlanner):
         _lib.check(self.lib.qsim_zero_state(state.data_ptr(), nqubits, self._dtype_code(state),
                                             self._stream(state)), self.lib)
 
-    def _run_blocked(self, ops, state: torch.Tensor, nqubits: int, zero=None) -> torch.Tensor:
+    def _run_blocked(self, ops, state: torch.Tensor, nqubits: int, zero=None, record=None) -> torch.Tensor:
         """Partition ``ops`` (an iterable of planned gates) into passes and
         launch each pass as soon as it is compiled, so the host plans and
         compiles pass k+1 while the GPU runs pass k. ``zero`` = [True] means the
         state still stands for |0...0>: the first pass starts from it (or the
-        state is filled before a one-gate launch); it is cleared once done."""
+        state is filled before a one-gate launch); it is cleared once done.
+        ``record`` (a list) receives the launches for the program cache."""
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         low = 3 if code == _lib.C128 else 4
@@ -650,29 +744,37 @@ class MI355XBackend(GatePlanner):
                     self._fill_zero(state, nqubits)
                     zero[0] = False
                 self.apply_op(payload, state, nqubits)
+                if record is not None:
+                    record.append(("single", payload))
                 continue
             tile_bits, pops = payload
             program, count, _ = blocking.compile_pass(tile_bits, pops, nqubits, cdtype, ordered)
             # zero padding: the kernel may read QSIM_BLOCKED_PAD bytes past the stream
             host = torch.frombuffer(bytearray(program) + bytes(blocking.PAD_BYTES), dtype=torch.uint8).pin_memory()
             prog = host.to(state.device, non_blocking=True)
-            timer = self._timer
-            if timer is not None and timer[0](_BLOCK_OP):
-                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-                ev[0].record(cur)
-            launch = self.lib.qsim_apply_blocked
-            if zero is not None and zero[0]:
-                launch = self.lib.qsim_apply_blocked_from_zero
-                zero[0] = False
-            rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(tile_bits), prog.data_ptr(), len(program),
-                        count, stream)
-            _lib.check(rc, self.lib)
-            if timer is not None and timer[0](_BLOCK_OP):
-                ev[1].record(cur)
-                timer[1].append(ev)
-            _keep_alive(prog, state)
-            self.block_stats.append(len(pops))
+            self._launch_blocked(state, nqubits, code, tuple(tile_bits), prog, len(program), count, len(pops),
+                                 stream, cur, zero)
+            if record is not None:
+                record.append(("block", tuple(tile_bits), prog, len(program), count, len(pops)))
         return state
+
+    def _launch_blocked(self, state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero):
+        timer = self._timer
+        if timer is not None and timer[0](_BLOCK_OP):
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record(cur)
+        launch = self.lib.qsim_apply_blocked
+        if zero is not None and zero[0]:
+            launch = self.lib.qsim_apply_blocked_from_zero
+            zero[0] = False
+        rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
+                    count, stream)
+        _lib.check(rc, self.lib)
+        if timer is not None and timer[0](_BLOCK_OP):
+            ev[1].record(cur)
+            timer[1].append(ev)
+        _keep_alive(prog, state)
+        self.block_stats.append(ngates)
 
     def execute_circuits(self, circuits, initial_states=None, nshots=1000, processes=None):
         if initial_states is None:

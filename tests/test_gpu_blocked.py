@@ -126,3 +126,39 @@ def test_zero_state_produced_by_first_pass(backend, first):
     got = backend.execute_circuit(c, nshots=10).state().detach().cpu().numpy()
     want = OracleBackend().execute(c)
     assert np.max(np.abs(got - want)) <= 1e-12
+
+
+@pytest.mark.parametrize("first", ["blocked", "single"])
+def test_program_cache_replay(backend, first, rng):
+    """A second execution of an identical gate queue replays the compiled
+    passes (bitwise the same state); a changed parameter compiles anew."""
+    n = 16
+    c = models.random_layered_circuit(n, 6, seed=5, measure=False)
+    if first == "single":
+        u = np.linalg.qr(rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32)))[0]
+        c0 = Circuit(n)
+        c0.add(gates.Unitary(u, 0, 3, 6, 9, 12))
+        c = c0 + c
+    backend._programs.clear()
+    hits = backend.program_cache_hits
+    a = backend.execute_circuit(c).state().detach().cpu().numpy()
+    b = backend.execute_circuit(c).state().detach().cpu().numpy()
+    assert backend.program_cache_hits == hits + 1
+    assert np.array_equal(a, b)
+    want = OracleBackend().execute(c)
+    assert np.max(np.abs(a - want)) <= 1e-12
+    # with an initial state the key differs (no from-zero first pass)
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi /= np.linalg.norm(psi)
+    s1 = backend.execute_circuit(c, initial_state=psi).state().detach().cpu().numpy()
+    s2 = backend.execute_circuit(c, initial_state=psi).state().detach().cpu().numpy()
+    assert backend.program_cache_hits == hits + 2
+    assert np.array_equal(s1, s2)
+    assert np.max(np.abs(s1 - OracleBackend().execute(c, psi))) <= 1e-12
+    # a changed parameter is a different program
+    params = [g for g in c.queue if isinstance(g, gates.RY)]
+    params[3].parameters = params[3].parameters[0] + 0.25
+    d = backend.execute_circuit(c).state().detach().cpu().numpy()
+    assert backend.program_cache_hits == hits + 2
+    assert np.max(np.abs(d - OracleBackend().execute(c))) <= 1e-12
+    assert np.max(np.abs(d - a)) > 1e-3
