@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 3
+#define QSIM_ABI_VERSION 4
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -69,6 +69,14 @@ int qsim_zero_state(void* state, int nqubits, int dtype, void* stream);
  * (abstract.py:3176-3197). `matrix` is a host pointer. */
 int qsim_apply_dense(void* state, int nqubits, int dtype, const int* targets, int ntargets,
                      const int* controls, int ncontrols, const void* matrix, void* stream);
+
+/* Collapse onto the outcome `outcome` of measuring `qubits` (qibo order: bit
+ * j of the outcome, most significant first, is qubits[j]): amplitudes whose
+ * measured bits equal the outcome are divided by `norm`, all others set to 0.
+ * Replaces Backend._collapse_statevector (abstract.py:3279-3304); the caller
+ * passes norm = sqrt(probability of the outcome), or 1 without normalising. */
+int qsim_collapse(void* state, int nqubits, int dtype, const int* qubits, int nmeasured, int64_t outcome,
+                  double norm, void* stream);
 
 /* Diagonal matrix (2^k complex entries, k = ntargets, 0..5) on `targets`,
  * restricted to amplitudes whose `controls` bits are all 1. k = 0 multiplies

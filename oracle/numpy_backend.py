@@ -361,6 +361,64 @@ class OracleBackend:
             state = self.apply_gate(gate, state, nqubits)
         return state
 
+    # -- collapse and repeated execution (state vectors) -----------------------
+    def collapse_state(self, state, qubits, shot, nqubits, normalize=True):
+        """abstract.py:3279-3304 (_collapse_statevector) with _append_zeros
+        (abstract.py:3236-3247)."""
+        shape = state.shape
+        binshot = list(self.samples_to_binary(np.asarray(shot), len(qubits))[0])
+        state = np.reshape(state, nqubits * (2,))
+        order = list(qubits) + [q for q in range(nqubits) if q not in qubits]
+        state = np.transpose(state, order)
+        subshape = (2 ** len(qubits),) + (nqubits - len(qubits)) * (2,)
+        state = np.reshape(state, subshape)[int(shot[0])]
+        if normalize:
+            norm = np.sqrt(np.sum(np.abs(state) ** 2))
+            state = state / norm
+        for q, r in zip(qubits, binshot):
+            state = np.expand_dims(state, q)
+            if r == 1:
+                state = np.concatenate([np.zeros_like(state), state], axis=q)
+            else:
+                state = np.concatenate([state, np.zeros_like(state)], axis=q)
+        return np.reshape(state, shape)
+
+    def execute_repeated(self, circuit, nshots, initial_state=None):
+        """execute_circuit_repeated (abstract.py:2532-2636), state-vector branch:
+        per shot, a copy of the initial state runs the queue; a collapsing M
+        samples one outcome of its sorted qubits (MeasurementResult.add_shot,
+        measurements.py:114-123, one draw from np.random) and collapses
+        (gates/measurements.py:189-204); the final measurements are sampled
+        once (CircuitResult(..., nshots=1).samples()). Returns the (nshots, m)
+        final samples and, per collapsing M (in queue order), its (nshots, k)
+        outcomes."""
+        nqubits = circuit.nqubits
+        state_copy = self.zero_state(nqubits) if initial_state is None else np.array(initial_state, copy=True)
+        final_qubits = [q for g in circuit.measurements for q in g.target_qubits]
+        collapsing = [g for g in circuit.queue if g.__class__.__name__ == "M" and g.collapse]
+        records = [[] for _ in collapsing]
+        results = []
+        for _ in range(nshots):
+            state = state_copy.copy()
+            k = 0
+            for gate in circuit.queue:
+                if gate.__class__.__name__ == "M":
+                    if not gate.collapse:
+                        continue
+                    qubits = sorted(gate.target_qubits)
+                    probs = self.calculate_probabilities(state, qubits, nqubits)
+                    shot = self.sample_shots(probs, 1)
+                    records[k].append(self.samples_to_binary(shot, len(qubits))[0])
+                    k += 1
+                    state = self.collapse_state(state, qubits, shot, nqubits)
+                    continue
+                state = self.apply_gate(gate, state, nqubits)
+            if final_qubits:
+                probs = self.calculate_probabilities(state, final_qubits, nqubits)
+                sample = self.sample_shots(probs, 1)
+                results.append(self.samples_to_binary(sample, len(final_qubits))[0])
+        return np.asarray(results), [np.asarray(r) for r in records]
+
     # -- probabilities (abstract.py:2734-2758, 3371-3381) ---------------------
     def calculate_probabilities(self, state, qubits, nqubits):
         rtype = np.real(state).dtype

@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
@@ -29,6 +29,7 @@ SIGNATURES = {
     "qsim_version": (_i, []),
     "qsim_last_error": (ctypes.c_char_p, []),
     "qsim_zero_state": (_i, [_vp, _i, _i, _vp]),
+    "qsim_collapse": (_i, [_vp, _i, _i, _c_int_p, _i, _i64, ctypes.c_double, _vp]),
     "qsim_apply_dense": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
     "qsim_apply_diag": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
     "qsim_apply_swap": (_i, [_vp, _i, _i, _i, _i, _c_int_p, _i, _vp]),

@@ -519,11 +519,15 @@ class MI355XBackend(GatePlanner):
                     f"expected {(2**nqubits,)}.",
                 )
         if circuit.has_collapse or circuit.repeated_execution:
-            raise_error(
-                NotImplementedError,
-                "Collapsing measurements and noise channels need repeated execution, which "
-                "qibo_amd does not implement.",
-            )
+            if not circuit.measurements and not circuit.has_collapse:
+                raise_error(
+                    RuntimeError,
+                    "Attempting to perform noisy simulation with `density_matrix=False` "
+                    "and no Measurement gate in the Circuit. If you wish to retrieve the "
+                    "statistics of the outcomes please include measurements in the circuit, "
+                    "otherwise set `density_matrix=True` to recover the final state.",
+                )
+            return self.execute_circuit_repeated(circuit, nshots, initial_state)
         try:
             return self._execute_circuit(circuit, initial_state=initial_state, nshots=nshots)
         except self.oom_error:
@@ -550,6 +554,64 @@ class MI355XBackend(GatePlanner):
         else:
             circuit._final_state = QuantumState(state, backend=self)
         return circuit._final_state
+
+    def execute_circuit_repeated(self, circuit, nshots: int, initial_state=None):
+        """Run the circuit once per shot (abstract.py:2532-2636, state vectors).
+
+        Each run starts from a copy of the initial state; a collapsing ``M``
+        draws its outcome from the current marginal probabilities (one uniform
+        from numpy's global generator, as ``MeasurementResult.add_shot``) and
+        collapses the state; the final measurements are sampled once per run.
+        The uniforms are drawn in the reference's order, so for the same seed
+        the samples are the reference's."""
+        from .result import MeasurementOutcomes
+
+        if circuit.has_collapse and not circuit.measurements:
+            raise_error(
+                RuntimeError,
+                "The circuit contains only collapsing measurements (`collapse=True`) but "
+                "`density_matrix=False`. Please set `density_matrix=True` to retrieve "
+                "the final state after execution.",
+            )
+        from .result import CircuitResult
+
+        nqubits = circuit.nqubits
+        state_copy = self.zero_state(nqubits) if initial_state is None else self.cast(initial_state, copy=True)
+        results, samples = [], []
+        for _ in range(nshots):
+            state = state_copy.clone()
+            state = self.run_queue(circuit.queue, state, nqubits)
+            if circuit.measurements:
+                result = CircuitResult(state, circuit.measurements, backend=self, nshots=1)
+                sample = result.samples()[0]
+                results.append(sample)
+                samples.append("".join(str(int(x)) for x in sample.tolist()))
+                for gate in circuit.measurements:
+                    gate.result.reset()
+        final_result = MeasurementOutcomes(
+            circuit.measurements, backend=self, samples=self.aggregate_shots(results), nshots=nshots
+        )
+        final_result._repeated_execution_frequencies = self.calculate_frequencies(samples)
+        circuit._final_state = final_result
+        return final_result
+
+    def collapse_state(self, state, qubits, shot, nqubits, normalize: bool = True, density_matrix=False):
+        """Project ``state`` onto the outcome ``shot`` of measuring ``qubits``
+        and renormalise it (abstract.py:2424-2440, 3279-3304): the other
+        amplitudes become zero, the kept ones are divided by the square root of
+        the outcome's probability."""
+        if density_matrix:
+            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+        qubits = [int(q) for q in qubits]
+        outcome = int(shot[0]) if hasattr(shot, "__len__") else int(shot)
+        if normalize:
+            p = self.calculate_probabilities(state, qubits, nqubits)
+            norm = math.sqrt(float(p[outcome]))
+        else:
+            norm = 1.0
+        _lib.check(self.lib.qsim_collapse(state.data_ptr(), nqubits, self._dtype_code(state), _lib.int_array(qubits),
+                                          len(qubits), outcome, norm, self._stream(state)), self.lib)
+        return state
 
     def run_queue(self, queue, state: torch.Tensor, nqubits: int, zero_pending: bool = False) -> torch.Tensor:
         """Apply the gates of ``queue`` in order (the loop of abstract.py:3321-3322).
@@ -593,12 +655,16 @@ class MI355XBackend(GatePlanner):
         stop = []  # the special gate that ended the current segment
 
         def segment():
-            # planned gates up to the next special gate, produced lazily so
-            # planning overlaps the passes already launched
+            # planned gates up to the next special gate or collapsing
+            # measurement, produced lazily so planning overlaps the passes
+            # already launched
             for gate in gates_left:
-                if isinstance(gate, M) and not gate.collapse:
-                    gate.result.backend = self
-                    continue
+                if isinstance(gate, M):
+                    if not gate.collapse:
+                        gate.result.backend = self
+                        continue
+                    stop.append(gate)
+                    return
                 if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
                     stop.append(gate)
                     return
@@ -706,7 +772,7 @@ class MI355XBackend(GatePlanner):
             if isinstance(gate, M) and not gate.collapse:
                 gate.result.backend = self
                 continue
-            if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
+            if isinstance(gate, M) or (isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate)):
                 for g in queue:  # callbacks need the real state between gates
                     state = g.apply(self, state, nqubits)
                 return state
@@ -921,6 +987,10 @@ class MI355XBackend(GatePlanner):
         return (s * w).sum(dim=1).to(torch.int32)
 
     def calculate_frequencies(self, samples):
+        if isinstance(samples, (list, tuple)) and samples and isinstance(samples[0], str):
+            # bitstrings of repeated execution (abstract.py:2727-2732 on a list of str)
+            res, counts = np.unique(samples, return_counts=True)
+            return collections.Counter(dict(zip(res, counts)))
         s = samples if isinstance(samples, torch.Tensor) else torch.as_tensor(np.asarray(samples))
         res, counts = torch.unique(s, return_counts=True)
         return collections.Counter(dict(zip(res.cpu().tolist(), counts.cpu().tolist())))

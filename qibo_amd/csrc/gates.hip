@@ -88,6 +88,25 @@ __global__ __launch_bounds__(256) void k_zero_state(cplx<R>* __restrict__ psi, u
   }
 }
 
+// Collapse onto a measurement outcome: amplitudes whose measured bits equal
+// `value` are divided by `norm`, the others set to zero (one pass).
+template <typename R>
+__global__ __launch_bounds__(256) void k_collapse(cplx<R>* __restrict__ psi, uint64_t n, uint64_t mask,
+                                                  uint64_t value, R norm) {
+  const uint64_t i = xcd_block() * 256 + threadIdx.x;
+  if (i < n) {
+    cplx<R> v = ld_nt(psi, i);
+    if ((i & mask) == value) {
+      v.re = v.re / norm;
+      v.im = v.im / norm;
+    } else {
+      v.re = R(0);
+      v.im = R(0);
+    }
+    st_nt(psi, i, v);
+  }
+}
+
 constexpr int kThreads = 256;
 
 // Amplitudes per 128-byte line: a target bit below log2 of this pairs
@@ -325,6 +344,33 @@ int qsim_zero_state(void* state, int nqubits, int dtype, void* stream) {
     hipLaunchKernelGGL(k_zero_state<double>, blocks, dim3(256), 0, s, (c128*)state, n);
   else
     hipLaunchKernelGGL(k_zero_state<float>, blocks, dim3(256), 0, s, (c64*)state, n);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_collapse(void* state, int nqubits, int dtype, const int* qubits, int nmeasured, int64_t outcome,
+                  double norm, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_collapse: null state");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_collapse: bad dtype");
+  QSIM_CHECK_ARG(nmeasured >= 1 && nmeasured <= nqubits && nmeasured <= 62, "qsim_collapse: bad qubit count");
+  if (!validate_qubits(nqubits, qubits, nmeasured, nullptr, 0, "qsim_collapse")) return QSIM_EINVAL;
+  QSIM_CHECK_ARG(outcome >= 0 && outcome < (int64_t(1) << nmeasured), "qsim_collapse: outcome out of range");
+  QSIM_CHECK_ARG(norm > 0.0 && norm < 1e300, "qsim_collapse: norm must be positive and finite");
+  // outcome bit j (most significant first) belongs to qubits[j], amplitude bit n-1-q
+  uint64_t mask = 0, value = 0;
+  for (int j = 0; j < nmeasured; ++j) {
+    const uint64_t b = uint64_t(1) << (nqubits - 1 - qubits[j]);
+    mask |= b;
+    if ((uint64_t(outcome) >> (nmeasured - 1 - j)) & 1) value |= b;
+  }
+  const uint64_t n = uint64_t(1) << nqubits;
+  const dim3 blocks = launch_grid(ceil_div_u(n, 256));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    hipLaunchKernelGGL(k_collapse<double>, blocks, dim3(256), 0, s, (c128*)state, n, mask, value, norm);
+  else
+    hipLaunchKernelGGL(k_collapse<float>, blocks, dim3(256), 0, s, (c64*)state, n, mask, value, float(norm));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }

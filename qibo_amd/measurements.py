@@ -30,6 +30,21 @@ class MeasurementResult:
     def register_samples(self, samples):
         self._samples = samples
 
+    def add_shot(self, probs, backend=None):
+        """Draw one outcome of the sorted target qubits and append its bits
+        (measurements.py:114-123)."""
+        from .backends import _check_backend
+
+        backend = _check_backend(backend)
+        qubits = sorted(self.target_qubits)
+        shot = backend.sample_shots(probs, 1)
+        bshot = backend.samples_to_binary(shot, len(qubits))
+        if self._samples is not None and len(self._samples):
+            self._samples = backend.concatenate([self._samples, bshot[:1]], axis=0)
+        else:
+            self._samples = bshot[:1]
+        return shot
+
     def register_frequencies(self, frequencies):
         self._frequencies = frequencies
 

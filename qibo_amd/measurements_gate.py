@@ -3,9 +3,9 @@
 Follows ``src/qibo/gates/measurements.py:11-204``: ``M(*q)`` records which
 qubits are measured (``target_qubits``), owns a per-register
 ``MeasurementResult``, adds basis-rotation gates for non-Z bases, and on a
-state vector is a no-op unless ``collapse=True`` (measurements.py:189-197).
-Collapsing measurements need repeated (per-shot) execution and are not part
-of this backend's path: they raise ``NotImplementedError`` when executed.
+state vector is a no-op unless ``collapse=True`` (measurements.py:189-204):
+then it draws an outcome and collapses the state (circuits with collapsing
+measurements run once per shot, ``execute_circuit_repeated``).
 """
 
 from typing import Optional
@@ -84,13 +84,16 @@ class M(_g.Gate):
         raise_error(NotImplementedError, "Measurement gates do not have a matrix representation.")
 
     def apply(self, backend, state, nqubits):
+        """Non-collapsing: no-op. Collapsing (measurements.py:189-204): draw one
+        outcome from the marginal probabilities of the sorted target qubits,
+        record it, and collapse the state onto it."""
         self.result.backend = backend
         if not self.collapse:
             return state
-        raise_error(
-            NotImplementedError,
-            "Collapsing measurements (mid-circuit collapse) are not supported by this backend.",
-        )
+        qubits = sorted(self.target_qubits)
+        probs = backend.calculate_probabilities(state, qubits, nqubits)
+        shot = self.result.add_shot(probs, backend=backend)
+        return backend.collapse_state(state, qubits, shot, nqubits)
 
 
 def _basis_rotation(cls, qubit):

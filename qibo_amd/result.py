@@ -68,6 +68,7 @@ class MeasurementOutcomes:
         self._probs = probabilities
         self._samples = samples
         self._frequencies = None
+        self._repeated_execution_frequencies = None  # set by execute_circuit_repeated
         if samples is not None:
             for m in measurements:
                 indices = [self.measurement_gate.qubits.index(q) for q in m.qubits]
@@ -94,6 +95,10 @@ class MeasurementOutcomes:
 
     def frequencies(self, binary: bool = True, registers: bool = False):
         qubits = self.measurement_gate.qubits
+        if self._repeated_execution_frequencies is not None:  # result.py:235-241
+            if binary:
+                return self._repeated_execution_frequencies
+            return collections.Counter({int(k, 2): v for k, v in self._repeated_execution_frequencies.items()})
         if self._frequencies is None:
             if not self.has_samples():
                 self._frequencies = self.backend.sample_frequencies(self._probs, self.nshots)
