@@ -184,6 +184,15 @@ int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qub
  * Probabilities must be finite and >= 0.
  * `workspace` must hold qsim_cumsum_workspace(n) bytes. */
 size_t qsim_cumsum_workspace(int64_t n);
+/* Checks np.random.choice makes on p before drawing (mtrand choice: finite,
+ * non-negative, sums to 1 within sqrt(eps)), in one pass: out[0] = sum of p
+ * (FP64, fixed order), out[1] = number of negative entries, out[2] = number
+ * of NaNs (device pointer, 3 doubles). Used by sample_shots /
+ * sample_frequencies (abstract.py:2760-2781) before their draws.
+ * `workspace` must hold qsim_prob_stats_workspace(n) bytes. */
+size_t qsim_prob_stats_workspace(int64_t n);
+int qsim_prob_stats(const void* probs, int prob_dtype, int64_t n, double* out, void* workspace, size_t workspace_bytes,
+                    void* stream);
 int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
                       void* workspace, size_t workspace_bytes, void* stream);
 

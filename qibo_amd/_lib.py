@@ -30,6 +30,8 @@ SIGNATURES = {
     "qsim_last_error": (ctypes.c_char_p, []),
     "qsim_zero_state": (_i, [_vp, _i, _i, _vp]),
     "qsim_collapse": (_i, [_vp, _i, _i, _c_int_p, _i, _i64, ctypes.c_double, _vp]),
+    "qsim_prob_stats_workspace": (_sz, [_i64]),
+    "qsim_prob_stats": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "qsim_apply_dense": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
     "qsim_apply_diag": (_i, [_vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp]),
     "qsim_apply_swap": (_i, [_vp, _i, _i, _i, _i, _c_int_p, _i, _vp]),

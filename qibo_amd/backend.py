@@ -878,15 +878,23 @@ class MI355XBackend(GatePlanner):
         return torch.as_tensor(np.ascontiguousarray(arr), device=self.torch_device)
 
     def _validate_probabilities(self, p: torch.Tensor):
-        """The checks np.random.choice applies to ``p`` before drawing."""
+        """The checks np.random.choice applies to ``p`` before drawing (one
+        pass over ``p``: sum, negative entries, NaNs)."""
         atol = math.sqrt(np.finfo(np.float64).eps)
         if p.dtype == torch.float32:
             atol = max(atol, math.sqrt(np.finfo(np.float32).eps))
-        pd = p.to(torch.float64)
-        total = float(pd.sum())
-        if math.isnan(total):
+        p = p.contiguous()
+        n = p.numel()
+        out = torch.empty(3, dtype=torch.float64, device=p.device)
+        wsb = self.lib.qsim_prob_stats_workspace(n)
+        ws = self._workspace(wsb, "pstats", p.device)
+        pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
+        _lib.check(self.lib.qsim_prob_stats(p.data_ptr(), pcode, n, out.data_ptr(), ws.data_ptr(), wsb,
+                                            self._stream(p)), self.lib)
+        total, nneg, nnan = out.tolist()
+        if nnan or math.isnan(total):
             raise ValueError("probabilities contain NaN")
-        if bool((pd < 0).any()):
+        if nneg:
             raise ValueError("probabilities are not non-negative")
         if abs(total - 1.0) > atol:
             raise ValueError("probabilities do not sum to 1")

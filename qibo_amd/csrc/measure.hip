@@ -403,7 +403,8 @@ __device__ __forceinline__ Map block_scan_excl(Map v, Map* wave_tot) {
 // (C) per-tile decision and map
 template <typename P>
 __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_t n, const double* __restrict__ astart,
-                                                 const double* __restrict__ tsum, TileInfo* __restrict__ info) {
+                                                 const double* __restrict__ tsum, TileInfo* __restrict__ info,
+                                                 Map* __restrict__ xmap) {
   __shared__ double tile[kTile];
   __shared__ Map wave_tot[kT / 64];
   const int64_t t = blockIdx.x;
@@ -422,6 +423,7 @@ __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_
 #pragma unroll
   for (int k = 0; k < kPer; ++k) f = compose(f, elem_map(tile[threadIdx.x * kPer + k], Eb));
   const Map excl = block_scan_excl(f, wave_tot);
+  xmap[t * kT + threadIdx.x] = excl;  // reused by k_tile_apply (same tile, same grid Eb)
   if (threadIdx.x == kT - 1) {
     const Map tot = compose(excl, f);
     const bool ok = tot.c0 < kTwo53 && tot.c1 < kTwo53;
@@ -643,20 +645,19 @@ __global__ __launch_bounds__(64) void k_tile_starts(TileInfo* __restrict__ info,
 }
 
 // (E) finish every tile that the chain resolved by maps
+// A tile with flag 0 was resolved by its map, so k_tile_map computed its
+// threads' exclusive maps on the grid ti.Eb: they are read back, not redone.
 template <typename P>
 __global__ __launch_bounds__(kT) void k_tile_apply(const P* __restrict__ p, int64_t n, const TileInfo* __restrict__ info,
-                                                   const double* __restrict__ tstart, double* __restrict__ cdf) {
+                                                   const double* __restrict__ tstart, const Map* __restrict__ xmap,
+                                                   double* __restrict__ cdf) {
   __shared__ double tile[kTile];
-  __shared__ Map wave_tot[kT / 64];
   const int64_t t = blockIdx.x;
   const TileInfo ti = info[t];
   if (ti.flag != 0) return;
   const int64_t base = t * kTile;
+  const Map excl = xmap[t * kT + threadIdx.x];
   stage_tile(p, n, base, tile);
-  Map f = {0, 0};
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) f = compose(f, elem_map(tile[threadIdx.x * kPer + k], ti.Eb));
-  const Map excl = block_scan_excl(f, wave_tot);
   const double s0 = tstart[t];
   const int64_t m0 = m_of(s0, ti.Eb);
   double s = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), ti.Eb);
@@ -684,6 +685,7 @@ struct CumsumWs {
   TileInfo* info;
   TileInfo* ginfo;
   double* gstart;
+  Map* xmap;  // per-thread exclusive maps of every tile (16 bytes per 16 elements)
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -692,7 +694,7 @@ static size_t cumsum_ws_bytes(int64_t n) {
   const size_t T = size_t((n + kTile - 1) / kTile);
   const size_t G = (T + 63) / 64;
   return 3 * align256(T * sizeof(double)) + align256(T * sizeof(TileInfo)) + align256(G * sizeof(TileInfo)) +
-         align256(G * sizeof(double));
+         align256(G * sizeof(double)) + align256(T * kT * sizeof(Map));
 }
 
 template <typename P>
@@ -713,11 +715,13 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
   c.ginfo = reinterpret_cast<TileInfo*>(w);
   w += align256(G * sizeof(TileInfo));
   c.gstart = reinterpret_cast<double*>(w);
+  w += align256(G * sizeof(double));
+  c.xmap = reinterpret_cast<Map*>(w);
   hipLaunchKernelGGL(k_tile_sum<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.tsum);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_approx_prefix, dim3(1), dim3(1024), 0, s, c.tsum, T, c.astart);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info);
+  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info, c.xmap);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo);
   QSIM_CHECK_LAUNCH();
@@ -725,7 +729,7 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_tile_starts, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo, c.gstart, c.tstart);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_tile_apply<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.info, c.tstart, cdf);
+  hipLaunchKernelGGL(k_tile_apply<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.info, c.tstart, c.xmap, cdf);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
@@ -842,6 +846,44 @@ __global__ __launch_bounds__(kT) void k_div_by(const F* __restrict__ p, int64_t 
   if (i < n) out[i] = p[i] / d[0];
 }
 
+// ---------------------------------------------------------------------------
+// checks of a probability vector before drawing: sum, negatives, NaNs
+// ---------------------------------------------------------------------------
+constexpr int kStatBlocks = 2048;
+
+template <typename P>
+__global__ __launch_bounds__(kT) void k_prob_stats(const P* __restrict__ p, int64_t n, double* __restrict__ part) {
+  __shared__ double lds[kT];
+  double sum = 0.0, neg = 0.0, nan = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * kT;
+  for (int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x; i < n; i += stride) {
+    const double x = double(p[i]);
+    sum += x;
+    neg += x < 0.0 ? 1.0 : 0.0;
+    nan += x != x ? 1.0 : 0.0;
+  }
+  sum = block_sum_fixed(sum, lds);
+  neg = block_sum_fixed(neg, lds);
+  nan = block_sum_fixed(nan, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = sum;
+    part[3 * blockIdx.x + 1] = neg;
+    part[3 * blockIdx.x + 2] = nan;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_prob_stats_final(const double* __restrict__ part, int nblocks,
+                                                         double* __restrict__ out) {
+  __shared__ double lds[kT];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += kT)
+    for (int k = 0; k < 3; ++k) v[k] += part[3 * b + k];
+  for (int k = 0; k < 3; ++k) {
+    const double r = block_sum_fixed(v[k], lds);
+    if (threadIdx.x == 0) out[k] = r;
+  }
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -869,6 +911,30 @@ int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qub
   if (dtype == QSIM_C128)
     return probabilities_impl<double>(state, nqubits, qubits, nmeasured, probs, workspace, workspace_bytes, s);
   return probabilities_impl<float>(state, nqubits, qubits, nmeasured, probs, workspace, workspace_bytes, s);
+}
+
+size_t qsim_prob_stats_workspace(int64_t n) { return n < 1 ? 0 : size_t(3 * kStatBlocks) * sizeof(double); }
+
+int qsim_prob_stats(const void* probs, int prob_dtype, int64_t n, double* out, void* workspace, size_t workspace_bytes,
+                    void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(probs != nullptr && out != nullptr, "qsim_prob_stats: null pointer");
+  QSIM_CHECK_ARG(n >= 1 && n <= (int64_t(1) << 40), "qsim_prob_stats: bad length");
+  QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_prob_stats: bad dtype");
+  QSIM_CHECK_ARG(workspace != nullptr && workspace_bytes >= qsim_prob_stats_workspace(n),
+                 "qsim_prob_stats: workspace too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  double* part = reinterpret_cast<double*>(workspace);
+  const int64_t want = (n + kT - 1) / kT;
+  const int nb = int(want < kStatBlocks ? want : kStatBlocks);
+  if (prob_dtype == QSIM_F64)
+    hipLaunchKernelGGL(k_prob_stats<double>, dim3(nb), dim3(kT), 0, s, reinterpret_cast<const double*>(probs), n, part);
+  else
+    hipLaunchKernelGGL(k_prob_stats<float>, dim3(nb), dim3(kT), 0, s, reinterpret_cast<const float*>(probs), n, part);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_prob_stats_final, dim3(1), dim3(kT), 0, s, part, nb, out);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
 }
 
 size_t qsim_cumsum_workspace(int64_t n) { return n < 1 ? 0 : cumsum_ws_bytes(n); }
