@@ -41,6 +41,12 @@ lane):
   mapping change or LDS gate (it commutes with every register-local
   operation in between).
 * NEG: the -1 phase of CZ-like gates under register and lane conditions.
+* LANE_REGPHASE: a controlled phase (one entry) whose condition holds one
+  register bit and otherwise bits fixed per lane (CU1-like phases with one
+  qubit in the registers): multiplied into a per-lane pending factor of the
+  registers with that bit set; QFLUSH applies it, placed by the compiler
+  before the next non-diagonal record on that bit, a LANE_REGPHASE on
+  another bit, a mapping change, a tile gather and the end of the pass.
 * ROT: an unconditional one-qubit rotation [[c, -s], [s, c]] (RY), reflection
   [[c, s], [s, -c]] (H) or [[c, -is], [-is, c]] (RX) on a register bit as
   three shears x0 -= t x1; x1 += s x0; x0 -= t x1 with t = s / (1 + c)
@@ -68,6 +74,7 @@ OP_DIAG, OP_DENSE1, OP_MAP, OP_DENSE_LDS = 0, 1, 2, 3
 OP_DENSE1_REAL, OP_REG_PHASE, OP_LANE_PHASE, OP_NEG = 4, 5, 6, 7
 OP_ROT = 10
 ROT_RY, ROT_H, ROT_RX = 0, 1, 2  # ROT modes (arg = register bit | mode << 2)
+OP_LANE_REGPHASE, OP_QFLUSH = 11, 12
 REG_FLAG = 0x80  # Diag target byte: register bit j of the current mapping is REG_FLAG | j
 _REG_GROUPS = {0: (8, 9, 10, 11), 1: (4, 5, 6, 7), 2: (0, 1, 2, 3)}
 _SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=complex)
@@ -341,6 +348,7 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
     pending = None  # REG_PHASE table being folded (complex128, 16 entries)
     sign = False  # net -1 left by ROT records of gates with c < 0
     absorb = None  # (byte offset, entries) of the first unconditional linear record
+    qpend = None  # register bit with a pending LANE_REGPHASE factor
 
     def emit(op, arg, cmask, nt, bits, data, rreq=0):
         nonlocal count, absorb
@@ -349,6 +357,15 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             absorb = (len(buf) + HEADER_BYTES, np.asarray(data).size)
         _pack(buf, op, arg, cmask, nt, bits, data, cdtype, rreq)
         count += 1
+
+    def qflush(j=None):
+        """Apply the pending per-lane factor (before anything that does not
+        commute with it; with j only before a register gate on its bit)."""
+        nonlocal qpend
+        if qpend is None or (j is not None and j != qpend):
+            return
+        emit(OP_QFLUSH, qpend, 0, 0, (), None)
+        qpend = None
 
     def flush_reg(j=None):
         """Emit the folded table; before a register gate on register bit j
@@ -384,7 +401,7 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         return False
 
     def emit_diag(op):
-        nonlocal pending
+        nonlocal pending, qpend
         rb = reg_bits(cur)
         tb = [nqubits - 1 - q for q in op.targets]
         cb = [nqubits - 1 - q for q in op.controls]
@@ -409,12 +426,19 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         enc = [(REG_FLAG | rb.index(b)) if b in rb else b for b in tb]
         if not ordered and nt == 0 and d[0] == -1:
             emit(OP_NEG, 0, cm, 0, (), None, rreq)
+        elif not ordered and nt == 0 and rreq in (1, 2, 4, 8):
+            j = rreq.bit_length() - 1
+            if qpend != j:
+                qflush()
+                qpend = j
+            emit(OP_LANE_REGPHASE, j, cm, 0, (), d[:1])
         else:
             emit(OP_DIAG, 0, cm, nt, enc, d, rreq)
 
     for event, arg in _schedule(items, npred, succ, ready, runs_here):
         if event == "map":
             flush_reg()
+            qflush()
             emit(OP_MAP, arg, 0, 0, (), None)
             switches += 1
             cur = arg
@@ -426,6 +450,7 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
         elif op.kind == "dense" and len(op.targets) == 1 and runs_here(pick, cur):
             lb = local[nqubits - 1 - op.targets[0]]
             flush_reg(_REG_GROUPS[cur].index(lb))
+            qflush(_REG_GROUPS[cur].index(lb))
             cm, rreq = split_cond(op, cur)
             m = np.asarray(op.data)
             j = _REG_GROUPS[cur].index(lb)
@@ -439,12 +464,14 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             emit(OP_DENSE1_REAL if real else OP_DENSE1, j, cm, 0, (), m.ravel()[:4], rreq)
         else:
             flush_reg()
+            qflush()
             mat = _SWAP if op.kind == "swap" else op.data
             lbits = [local[nqubits - 1 - q] for q in op.targets]
             k = len(lbits)
             cm = sum(1 << (nqubits - 1 - q) for q in op.controls)
             emit(OP_DENSE_LDS, k, cm, 0, lbits + [0] * (4 - k) + sorted(lbits), mat)
     flush_reg()
+    qflush()
     if sign:
         # a global -1 commutes with every record: scale one unconditional
         # linear record's entries (all of them), else flip every sign at the end
@@ -589,7 +616,7 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
 # multiply (6) and a complex multiply-add (8); diagonal / table: a complex
 # multiply; a per-lane factor: one complex multiply per 16 amplitudes;
 # dense 2^k block: 6 + 8 (2^k - 1).
-_FLOPS = {OP_DENSE1_REAL: 6.0, OP_ROT: 6.0, OP_DENSE1: 14.0, OP_DIAG: 6.0, OP_REG_PHASE: 6.0, OP_LANE_PHASE: 6.0 / 16,
+_FLOPS = {OP_DENSE1_REAL: 6.0, OP_ROT: 6.0, OP_LANE_REGPHASE: 6.0 / 16, OP_QFLUSH: 3.0, OP_DENSE1: 14.0, OP_DIAG: 6.0, OP_REG_PHASE: 6.0, OP_LANE_PHASE: 6.0 / 16,
           OP_NEG: 0.0, OP_MAP: 0.0}
 
 

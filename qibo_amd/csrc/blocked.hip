@@ -67,7 +67,9 @@ enum : uint32_t {
   kOpNeg = 7,
   kOpInner1 = 8,     // adjoint sweep only: Re <w| M |v> for a 2x2 M on a register bit
   kOpInnerDiag = 9,  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
-  kOpRot = 10        // unconditional 2x2 rotation / reflection on a register bit, as three shears
+  kOpRot = 10,       // unconditional 2x2 rotation / reflection on a register bit, as three shears
+  kOpLaneRegPhase = 11,  // per-lane factor Q of the registers with one bit set (kept in LDS)
+  kOpQFlush = 12         // apply Q to the registers with bit arg set, reset it to 1
 };
 
 // Record header (the host packs the same layout).
@@ -481,6 +483,14 @@ __device__ __forceinline__ void flush_lane_phase(cplx<R>* v, cplx<R>& P, bool& p
   pdirty = false;
 }
 
+// Multiply the registers whose bit J is set by the per-lane factor q.
+template <typename R, int J>
+__device__ __forceinline__ void flush_q_bit(cplx<R>* v, cplx<R> q) {
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r)
+    if ((r >> J) & 1) v[r] = cmul(q, v[r]);
+}
+
 // Dense 2^k block (k = 1..3) on local bits. Each lane owns 16 / 2^k groups of
 // 2^k amplitudes (group g = lane + 256 i); the groups are gathered from the
 // tile through LDS one component at a time, multiplied in registers, and
@@ -612,6 +622,12 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   P.re = R(1);
   P.im = R(0);
   bool pdirty = false;
+  // Per-lane factor of LANE_REGPHASE records (controlled phases between one
+  // register bit and lane bits), applied to the registers with that bit set
+  // by a QFLUSH record; the host places the flushes. It lives in LDS so that
+  // no register is held for it across the record loop.
+  __shared__ cplx<R> q_lds[kBT];
+  q_lds[threadIdx.x] = cplx<R>{R(1), R(0)};
 
   // The record stream is read ahead by one header. The buffer is readable for
   // nbytes + kBlockedPad bytes, so a header or entry read that starts inside
@@ -638,6 +654,22 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
     // the lane condition is formed only by the forms that use it
     auto lane_ok = [&]() { return (lane_idx & h.cmask) == h.cmask; };
     switch (h.op) {
+      case kOpLaneRegPhase: {
+        const cplx<R> q = q_lds[threadIdx.x];
+        q_lds[threadIdx.x] = csel(lane_ok(), cmul(m[0], q), q);
+        break;
+      }
+      case kOpQFlush: {
+        const cplx<R> q = q_lds[threadIdx.x];
+        q_lds[threadIdx.x] = cplx<R>{R(1), R(0)};
+        switch (h.arg & 3) {
+          case 0: flush_q_bit<R, 0>(v, q); break;
+          case 1: flush_q_bit<R, 1>(v, q); break;
+          case 2: flush_q_bit<R, 2>(v, q); break;
+          default: flush_q_bit<R, 3>(v, q); break;
+        }
+        break;
+      }
       case kOpDense1Real:
         switch (h.arg & 3) {
           case 0: dense1_regs<R, 0, true>(v, h, m, lane_idx); break;

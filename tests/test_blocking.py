@@ -50,7 +50,13 @@ def run_pass(state, n, tile_bits, instrs):
     gidx = base[:, None] | loc[None, :]          # (ntiles, 4096) global indices
     tile = state[gidx].copy()
     cur = 0
+    pend, qbit = np.ones_like(tile), None  # pending LANE_REGPHASE factor and its register bit
     for op, arg, cmask, nt, bits, rreq, mat in instrs:
+        # nothing that fails to commute with a pending factor may run
+        if qbit is not None:
+            assert op not in (blocking.OP_MAP, blocking.OP_DENSE_LDS)
+            if op in (blocking.OP_DENSE1, blocking.OP_DENSE1_REAL, blocking.OP_ROT):
+                assert (arg & 3) != qbit
         rb = [tile_bits[lb] for lb in GROUPS[cur]]     # global bit of register bit j
         regmask = sum(1 << b for b in rb)
         if op != blocking.OP_DENSE_LDS:
@@ -75,6 +81,19 @@ def run_pass(state, n, tile_bits, instrs):
             tile = np.where(active, tile * mat[di], tile)
         elif op == blocking.OP_NEG:
             tile = np.where(active, -tile, tile)
+        elif op == blocking.OP_LANE_REGPHASE:
+            # phase on register bit `arg` under the lane-fixed condition bits,
+            # pending until a QFLUSH of that bit
+            assert rreq == 0 and nt == 0 and cmask & regmask == 0
+            assert qbit in (None, arg)
+            qbit = arg
+            on = active & (((gidx >> rb[arg]) & 1) == 1)
+            pend = np.where(on, pend * mat[0], pend)
+        elif op == blocking.OP_QFLUSH:
+            assert qbit == arg
+            tile = tile * pend
+            pend = np.ones_like(tile)
+            qbit = None
         elif op == blocking.OP_REG_PHASE:
             assert cmask == 0 and rreq == 0
             r = np.zeros_like(gidx)
@@ -144,6 +163,7 @@ def run_pass(state, n, tile_bits, instrs):
             act = active[:, bases]
             for r, o in enumerate(offs):
                 tile[:, bases | o] = np.where(act, ys[..., r], xs[..., r])
+    assert qbit is None  # flushed before the end of the pass
     out = state.copy()
     out[gidx] = tile
     return out
