@@ -116,7 +116,11 @@ int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, in
  * 5 16-entry register phase table, 6 per-lane diagonal factor, 7 sign flip,
  * 10 unconditional rotation / reflection on a register bit as three shears
  * (arg = bit | mode << 2, one entry (t, s); mode 0 [[c,-s],[s,c]],
- * 1 [[c,s],[s,-c]], 2 [[c,-is],[-is,c]], with t = s / (1 + c), c >= 0).
+ * 1 [[c,s],[s,-c]], 2 [[c,-is],[-is,c]], with t = s / (1 + c), c >= 0),
+ * 11 controlled phase on register bit arg under lane condition bits, held in
+ * a per-lane factor, 12 application of that factor to the registers with bit
+ * arg set (a program must place 12 before any non-diagonal record on that
+ * bit, an op-11 record on another bit, ops 2 and 3, and its end).
  * The buffer must stay readable for program_bytes + QSIM_BLOCKED_PAD bytes
  * (the kernel reads one header ahead and reads at most 64 entries of a
  * record, so no read leaves that range); records are read while they start
