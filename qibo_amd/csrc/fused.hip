@@ -121,6 +121,24 @@ __device__ __forceinline__ uint32_t mfma_local(const MfmaArgs& a, uint32_t r, ui
   return u;
 }
 
+// LDS slot of local index u: bits 0-4 ^= bits 5-9. Column groups whose 32
+// columns are 32 apart (target bits at the bottom of the tile) would
+// otherwise put all 32 lanes on one bank; the slices of the streaming phases
+// stay conflict-free (a permutation inside each 32-slot row).
+__device__ __forceinline__ uint32_t mfma_slot(uint32_t u) { return u ^ ((u >> 5) & 31u); }
+
+// Global address of element k * 256 + thread of tile t (coalesced slices).
+__device__ __forceinline__ uint64_t mfma_addr(const MfmaArgs& a, uint64_t glow, int k) {
+  uint64_t g = glow;
+#pragma clang loop unroll(full)
+  for (int i = 0; i < 4; ++i)
+    if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
+  return g;
+}
+
+// Persistent workgroups, software-pipelined: while the MFMA product of tile t
+// runs, the loads of the workgroup's next tile are in flight, so memory
+// latency is hidden inside each workgroup (tiles t = blockIdx.x + i gridDim.x).
 template <int K>
 __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const c64* __restrict__ mat,
                                                        c64* __restrict__ psi) {
@@ -130,87 +148,83 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
   constexpr int NCOL = 1 << (kMfmaTileBits - K);
   constexpr int NCG = NCOL / 32;        // 32-column groups (4 for K = 5, 8 for K = 4)
   __shared__ c64 tile[1 << kMfmaTileBits];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t t = xcd_block();
-  const uint64_t base = insert_zeros(t, a.tbits, kMfmaTileBits);
   __shared__ uint32_t rowoff[D];  // local-index bits of matrix row r
+  // the real embedding, column-major (element (i, k) at k * 2D + i): the A
+  // operand of step (mb, ks) is two consecutive 32-float runs
+  __shared__ float amat[4 * D * D];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (threadIdx.x < D) rowoff[threadIdx.x] = mfma_local<K>(a, threadIdx.x, 0);
-  // coalesced streaming load: thread u of each 256-slice reads local index
-  // u, whose global offset is the lane part (bits 0-7) | the slice part (8-11)
-  uint64_t glow = 0;
+  for (int e = threadIdx.x; e < 4 * D * D; e += 256) {
+    const int k = e / (2 * D), i = e % (2 * D);
+    const c64 m = mat[(i % D) * D + (k % D)];
+    amat[e] = (i < D) ? ((k < D) ? m.re : -m.im) : ((k < D) ? m.im : m.re);
+  }
+  // lane part (local bits 0-7) of this thread's slice element, as global bits
+  uint64_t lanebits = 0;
 #pragma clang loop unroll(full)
   for (int i = 0; i < 8; ++i)
-    if ((threadIdx.x >> i) & 1) glow |= uint64_t(1) << a.tbits[i];
-  glow |= base;
-  c64 ld[16];
+    if ((threadIdx.x >> i) & 1) lanebits |= uint64_t(1) << a.tbits[i];
+  uint64_t t = blockIdx.x;
+  c64 nxt[16];
+  if (t < a.ntiles) {
+    const uint64_t glow = insert_zeros(t, a.tbits, kMfmaTileBits) | lanebits;
 #pragma clang loop unroll(full)
-  for (int k = 0; k < 16; ++k) {
-    uint64_t g = glow;
-#pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i)
-      if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
-    ld[k] = ld_nt(psi, g);
+    for (int k = 0; k < 16; ++k) nxt[k] = ld_nt(psi, mfma_addr(a, glow, k));
   }
+  while (t < a.ntiles) {
+    const uint64_t glow = insert_zeros(t, a.tbits, kMfmaTileBits) | lanebits;
+    // each thread writes (and later reads back) only its own slots here
 #pragma clang loop unroll(full)
-  for (int k = 0; k < 16; ++k) tile[k * 256 + threadIdx.x] = ld[k];
-  // A fragments of the real embedding: row i = 32 mb + l % 32, col k = 2 ks + l / 32
-  float afrag[MB][KS];
+    for (int k = 0; k < 16; ++k) tile[mfma_slot(k * 256 + threadIdx.x)] = nxt[k];
+    __syncthreads();
+    const uint64_t tn = t + gridDim.x;
+    if (tn < a.ntiles) {  // next tile's loads overlap this tile's product
+      const uint64_t gnext = insert_zeros(tn, a.tbits, kMfmaTileBits) | lanebits;
 #pragma clang loop unroll(full)
-  for (int mb = 0; mb < MB; ++mb) {
-#pragma clang loop unroll(full)
-    for (int ks = 0; ks < KS; ++ks) {
-      const int i = 32 * mb + (lane & 31), k = 2 * ks + (lane >> 5);
-      const int ri = i % D, ci = k % D;
-      const c64 m = mat[ri * D + ci];
-      float v;
-      if (i < D) v = (k < D) ? m.re : -m.im;
-      else v = (k < D) ? m.im : m.re;
-      afrag[mb][ks] = v;
+      for (int k = 0; k < 16; ++k) nxt[k] = ld_nt(psi, mfma_addr(a, gnext, k));
     }
-  }
-  __syncthreads();
-  // one 32-column group per wave at a time; the groups are disjoint, so each
-  // wave reads and writes only its own columns of the tile
-  for (int cg = wave; cg < NCG; cg += 4) {
-    const uint32_t colpart = mfma_local<K>(a, 0, 32 * cg + (lane & 31));
-    floatx16 acc[MB];
+    // one 32-column group per wave at a time; the groups are disjoint, so each
+    // wave reads and writes only its own columns of the tile
+    for (int cg = wave; cg < NCG; cg += 4) {
+      const uint32_t colpart = mfma_local<K>(a, 0, 32 * cg + (lane & 31));
+      // the A operands are read from LDS at each use (an opaque lane index
+      // keeps the compiler from hoisting all 64 of them into registers)
+      int alane = lane;
+      asm volatile("" : "+v"(alane));
+      floatx16 acc[MB];
 #pragma clang loop unroll(full)
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = floatx16(0.0f);
+      for (int mb = 0; mb < MB; ++mb) acc[mb] = floatx16(0.0f);
 #pragma clang loop unroll(full)
-    for (int ks = 0; ks < KS; ++ks) {
-      // real inner index k = 2 ks + lane / 32: k < D -> re of row k, else im of row k - D
-      const c64 x = tile[rowoff[(2 * ks + (lane >> 5)) % D] | colpart];
-      const float b = (2 * ks + (lane >> 5) < D) ? x.re : x.im;
+      for (int ks = 0; ks < KS; ++ks) {
+        // real inner index k = 2 ks + lane / 32: k < D -> re of row k, else im of row k - D
+        const c64 x = tile[mfma_slot(rowoff[(2 * ks + (lane >> 5)) % D] | colpart)];
+        const float b = (2 * ks + (lane >> 5) < D) ? x.re : x.im;
 #pragma clang loop unroll(full)
-      for (int mb = 0; mb < MB; ++mb)
-        acc[mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(afrag[mb][ks], b, acc[mb], 0, 0, 0);
-    }
-    // D rows: real part rows [0, D), imaginary part rows [D, 2D); the wave's
-    // reads of these columns are complete (they fed the accumulators above)
+        for (int mb = 0; mb < MB; ++mb) {
+          const float av = amat[(2 * ks + (alane >> 5)) * (2 * D) + 32 * mb + (alane & 31)];
+          acc[mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, b, acc[mb], 0, 0, 0);
+        }
+      }
+      // D rows: real part rows [0, D), imaginary part rows [D, 2D); the wave's
+      // reads of these columns are complete (they fed the accumulators above)
 #pragma clang loop unroll(full)
-    for (int j = 0; j < 16; ++j) {
-      const int row = 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
+      for (int j = 0; j < 16; ++j) {
+        const int row = 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
 #pragma clang loop unroll(full)
-      for (int mb = 0; mb < MB; ++mb) {
-        const int i = 32 * mb + row;
-        if (i < D) {
-          tile[rowoff[i] | colpart].re = acc[mb][j];
-        } else {
-          tile[rowoff[i - D] | colpart].im = acc[mb][j];
+        for (int mb = 0; mb < MB; ++mb) {
+          const int i = 32 * mb + row;
+          if (i < D) {
+            tile[mfma_slot(rowoff[i] | colpart)].re = acc[mb][j];
+          } else {
+            tile[mfma_slot(rowoff[i - D] | colpart)].im = acc[mb][j];
+          }
         }
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 #pragma clang loop unroll(full)
-  for (int k = 0; k < 16; ++k) ld[k] = tile[k * 256 + threadIdx.x];
-#pragma clang loop unroll(full)
-  for (int k = 0; k < 16; ++k) {
-    uint64_t g = glow;
-#pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i)
-      if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
-    st_nt(psi, g, ld[k]);
+    for (int k = 0; k < 16; ++k) st_nt(psi, mfma_addr(a, glow, k), tile[mfma_slot(k * 256 + threadIdx.x)]);
+    t = tn;
   }
 }
 
@@ -240,8 +254,22 @@ static int launch_fused_mfma(void* state, int n, const int* targets, const void*
     for (int i = 0; i < kMfmaTileBits; ++i)
       if (bits[i] == gb) a.rowbit[t] = uint8_t(i);
   }
-  hipLaunchKernelGGL((k_fused_mfma_c64<K>), launch_grid(a.ntiles), dim3(256), 0, s, a,
-                     reinterpret_cast<const c64*>(mat), reinterpret_cast<c64*>(state));
+  // persistent grid: as many workgroups as the device keeps resident
+  static int per_cu[2] = {0, 0};  // per K = 4, 5 (the answer is fixed for the kernel)
+  static int cus = 0;
+  if (per_cu[K - 4] == 0) {
+    int nb = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_mfma_c64<K>, 256, 0) != hipSuccess || nb < 1)
+      nb = 2;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    per_cu[K - 4] = nb;
+  }
+  const uint64_t resident = uint64_t(cus) * uint64_t(per_cu[K - 4]);
+  const unsigned grid = unsigned(a.ntiles < resident ? a.ntiles : resident);
+  hipLaunchKernelGGL((k_fused_mfma_c64<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c64*>(mat),
+                     reinterpret_cast<c64*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }
