@@ -18,7 +18,9 @@ sample of the same circuit's gates on this host.
 
 Beside the headline line the same JSON object carries the other parts of
 the metric, each on its own bounded workload:
+* "layered28": the same step at 28 qubits (20 layers, 100 000 shots);
 * "qft": qibo's QFT circuit at 30 qubits in complex128 and complex64;
+* "qft15": qibo's QFT at 15 qubits, complex128, microseconds per circuit;
 * "fused_c64": the layered circuit at 32 qubits in complex64 and complex128
   (64 GiB state) through blocked passes; 24 dense 5-qubit blocks at 32 qubits
   (complex64 on MFMA, complex128 in FP64) with ms per block, HBM GB/s and
@@ -215,6 +217,65 @@ def bench_qft(seed, n=30):
         res[f"{dtype}_s"] = dt
         res[f"{dtype}_gate_apps_per_s"] = c.ngates / dt
     return res
+
+
+def bench_qft15(seed, reps=50):
+    """qibo's QFT at 15 qubits, complex128, from |0...0> (the reference's
+    CPU-runnable benchmark circuit): launch-bound, reported per circuit."""
+    import torch
+
+    from qibo_amd import models
+    from qibo_amd.backend import MI355XBackend
+
+    backend = MI355XBackend(dtype="complex128")
+    c = models.QFT(15)
+    c._final_state = None
+    backend.execute_circuit(c)
+    c._final_state = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = backend.execute_circuit(c)
+        c._final_state = None
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del r
+    return {"nqubits": 15, "gates": c.ngates, "us_per_circuit": dt * 1e6, "gate_apps_per_s": c.ngates / dt,
+            "circuits_timed": reps}
+
+
+def bench_layered28(seed, nshots=100000, steps=3):
+    """The layered circuit at 28 qubits, complex128, 20 layers, M(all) with
+    100 000 shots: the same step as the headline at the register size of the
+    reference's layered-circuit configuration."""
+    import torch
+
+    from qibo_amd import models
+    from qibo_amd.backend import MI355XBackend
+
+    backend = MI355XBackend(dtype="complex128")
+    c = models.random_layered_circuit(28, 20, seed=seed)
+
+    def step():
+        c._final_state = None
+        for gate in c.measurements:
+            gate.result.reset()
+        res = backend.execute_circuit(c, nshots=nshots)
+        return res.samples()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    del s
+    c._final_state = None
+    torch.cuda.empty_cache()
+    ngates = sum(1 for g in c.queue if g.__class__.__name__ != "M")
+    return {"nqubits": 28, "gates": ngates, "nshots": nshots, "ms_per_step": dt * 1e3,
+            "gate_apps_per_s": ngates / dt}
 
 
 def bench_serving(seed, nrequests=48, qps=16.0):
@@ -455,8 +516,8 @@ def main():
         },
     }
     if not args.no_extras:
-        for key, fn in (("qft", bench_qft), ("fused_c64", bench_fused), ("serving", bench_serving),
-                        ("adjoint", bench_adjoint)):
+        for key, fn in (("layered28", bench_layered28), ("qft", bench_qft), ("qft15", bench_qft15),
+                        ("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint)):
             try:
                 out[key] = fn(args.seed)
             except Exception as exc:  # noqa: BLE001 - an extra must not hide the headline line
