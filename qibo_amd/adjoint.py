@@ -203,47 +203,12 @@ _INNER = {
 }
 
 
-def _plan_batch(mats: np.ndarray, qubits, controls):
-    """The planned operation shared by every set's matrix in ``mats`` [B, D, D],
-    with its per-set data; raises _Unsupported when the sets would plan
-    differently or the operation is not a one-qubit dense or diagonal one."""
-    op0 = plan_matrix(mats[0], qubits, controls)
-    nb = mats.shape[0]
-    if op0.kind == "noop":
-        if np.array_equal(mats, np.broadcast_to(mats[0], mats.shape)):
-            return None
-        raise _Unsupported("parameter-dependent identity")
-    if op0.kind not in ("dense", "diag") or (op0.kind == "dense" and len(op0.targets) != 1):
-        raise _Unsupported(op0.kind)
-    q = list(qubits)
-    L = len(q)
-    peeled = [c for c in op0.controls if c not in tuple(controls)]
-    k = len(op0.targets)
-    idx = []
-    for r in range(2**k):
-        i = 0
-        for j, t in enumerate(op0.targets):
-            if (r >> (k - 1 - j)) & 1:
-                i |= 1 << (L - 1 - q.index(t))
-        for c in peeled:
-            i |= 1 << (L - 1 - q.index(c))
-        idx.append(i)
-    idx = np.array(idx)
-    full = np.broadcast_to(np.eye(2**L, dtype=mats.dtype), mats.shape).copy()
-    if op0.kind == "dense":
-        data = mats[:, idx][:, :, idx]
-        full[:, idx[:, None], idx[None, :]] = data
-    else:
-        data = np.diagonal(mats, axis1=1, axis2=2)[:, idx]
-        full[:, idx, idx] = data
-    if not np.array_equal(full, mats):
-        raise _Unsupported("structure differs across parameter sets")
-    assert data.shape[0] == nb
-    return BatchOp(op0.kind, op0.targets, op0.controls, data)
+def _batched_ops(backend, circuit, params: np.ndarray, sources=None, groups=None):
+    """Forward and reverse BatchOp lists and the trainable-parameter count.
 
-
-def _batched_ops(backend, circuit, params: np.ndarray):
-    """Forward and reverse BatchOp lists and the trainable-parameter count."""
+    With ``sources`` and ``groups`` (dicts) it also records, for the ops of
+    trainable gates, id(op) -> (gate name, dagger, index among that name's
+    gates) and, per gate name, the parameter slots in circuit order."""
     queue = [g for g in circuit.queue if not isinstance(g, gates.M)]
     nb = params.shape[0]
     slot_of, k = {}, 0
@@ -265,6 +230,11 @@ def _batched_ops(backend, circuit, params: np.ndarray):
         name = g.__class__.__name__
         if id(g) in slot_of and g.trainable:
             op, dag = _rotation_ops(g, name, params[:, slot_of[id(g)]], cdt)
+            if sources is not None:
+                members = groups.setdefault(name, [])
+                sources[id(op)] = (name, False, len(members))
+                sources[id(dag)] = (name, True, len(members))
+                members.append(slot_of[id(g)])
         else:
             # the same matrix for every set: plan it once, share its data
             m = np.asarray(g.matrix(backend), dtype=cdt)
@@ -296,6 +266,21 @@ def _shared(op, nb):
     return BatchOp(op.kind, op.targets, op.controls, np.broadcast_to(d, (nb,) + d.shape))
 
 
+def _rotation_data(name, th, cdt):
+    """Data rows [N, nent] of a trainable rotation's forward operation and of
+    its dagger for angles th [N]: the 2x2 row-major for X/Y rotations, the
+    diagonal for Z rotations, e^{i theta} alone for U1-type phases."""
+    mats = _batched_rotation(name, th).astype(cdt)
+    if name.startswith("C"):
+        mats = mats[:, 2:, 2:]  # the controlled block
+    if name in ("RX", "RY", "CRX", "CRY"):
+        return mats.reshape(-1, 4), np.conj(np.swapaxes(mats, 1, 2)).reshape(-1, 4)
+    d = np.diagonal(mats, axis1=1, axis2=2)
+    if name in ("U1", "CU1"):
+        d = d[:, 1:]
+    return d, np.conj(d)
+
+
 def _rotation_ops(g, name, th, cdt):
     """Forward and dagger BatchOps of a trainable rotation gate for angles th [B],
     in the form plan_matrix gives each set's matrix (dense 2x2 for X/Y
@@ -303,25 +288,176 @@ def _rotation_ops(g, name, th, cdt):
     subspace where target and controls are 1)."""
     t = g.target_qubits
     cq = tuple(g.control_qubits)
-    mats = _batched_rotation(name, th).astype(cdt)
-    if name.startswith("C"):
-        mats = mats[:, 2:, 2:]  # the controlled block
+    d, dd = _rotation_data(name, th, cdt)
     if name in ("RX", "RY", "CRX", "CRY"):
-        return (BatchOp("dense", t, cq, mats), BatchOp("dense", t, cq, np.conj(np.swapaxes(mats, 1, 2))))
-    d = np.diagonal(mats, axis1=1, axis2=2)
+        return BatchOp("dense", t, cq, d.reshape(-1, 2, 2)), BatchOp("dense", t, cq, dd.reshape(-1, 2, 2))
     if name in ("U1", "CU1"):  # diag(1, e^{i theta}): the target joins the controls
-        d = d[:, 1:]
-        return (BatchOp("diag", (), cq + t, d), BatchOp("diag", (), cq + t, np.conj(d)))
-    return (BatchOp("diag", t, cq, d), BatchOp("diag", t, cq, np.conj(d)))
+        return BatchOp("diag", (), cq + t, d), BatchOp("diag", (), cq + t, dd)
+    return BatchOp("diag", t, cq, d), BatchOp("diag", t, cq, dd)
 
 
-def _upload_programs(progs: np.ndarray, device):
-    nb, nbytes = progs.shape
-    stride = (nbytes + 15) // 16 * 16
-    buf = np.zeros(nb * stride + blocking.PAD_BYTES, dtype=np.uint8)
-    buf[: nb * stride].reshape(nb, stride)[:, :nbytes] = progs
-    dev = torch.from_numpy(buf).pin_memory().to(device, non_blocking=True)
-    return dev, stride, nbytes
+# A generic angle: the record form chosen for a trainable gate at it (complex
+# or real 2x2) is valid at every angle (RY-type matrices are real for all
+# angles, RX-type ones are complex at this one).
+_PROBE_ANGLE = 1.0
+
+
+class _SweepPass:
+    """One batched pass of a sweep: tile bits, record count, gradient slots,
+    the program row with every parameter-free entry in place, and the
+    positions (in entries of the complex dtype) that take columns ``cols`` of
+    the per-set data table."""
+
+    __slots__ = ("bits", "count", "slots", "row", "pos", "cols", "dev_slots")
+
+    def __init__(self, bits, count, slots, row, pos, cols):
+        self.bits, self.count, self.slots = list(bits), count, list(slots)
+        self.row, self.pos, self.cols = row, pos, cols
+        self.dev_slots = {}  # device -> int32 tensor of ``slots``
+
+
+class SweepTemplate:
+    """The forward and reverse passes of the batched adjoint sweep of one
+    circuit structure, compiled once.
+
+    The records, their order and every parameter-free entry do not depend on
+    the trainable angles, so a new batch of parameter sets only computes the
+    per-set data table (``data``: one column block per trainable gate and
+    direction, all gates of a kind at once) and scatters it into copies of
+    the program rows (``programs``). Raises _Unsupported for circuits the
+    batched sweep does not cover."""
+
+    def __init__(self, backend, circuit, cdt):
+        self.cdt = np.dtype(cdt)
+        n = circuit.nqubits
+        slot_of = {}
+        for k, g in enumerate(circuit.trainable_gates):
+            if g.nparams != 1:
+                raise _Unsupported("multi-parameter trainable gate")
+            slot_of[id(g)] = k
+        self.nparams = len(slot_of)
+        probe = np.full((1, self.nparams), _PROBE_ANGLE)
+        groups = {}  # gate name -> parameter slots, in circuit order
+        src = {}  # id(op) -> (gate name, dagger, index in the group)
+        fwd, rev, _ = _batched_ops(backend, circuit, probe, src, groups)
+        # columns of the data table: per gate name, the forward block then the dagger block
+        self.groups, col = [], 0
+        for name, slots in groups.items():
+            nent = _rotation_data(name, probe[:, 0], self.cdt)[0].shape[1]
+            g = len(slots)
+            self.groups.append((name, np.array(slots, dtype=np.int64), nent, col, col + g * nent))
+            col += 2 * g * nent
+        self.ncols = col
+        base = {name: (nent, c0, c1) for name, _, nent, c0, c1 in self.groups}
+
+        def first_col(op):
+            name, dagger, i = src[id(op)]
+            nent, c0, c1 = base[name]
+            return (c1 if dagger else c0) + i * nent
+
+        low = 3 if self.cdt == np.complex128 else 4
+        isz = self.cdt.itemsize
+        self.fwd, self.rev = [], []
+        for ops, out in ((fwd, self.fwd), (rev, self.rev)):
+            for kind, payload in blocking.partition(ops, n, low, singles=False):
+                if kind != "block":
+                    raise _Unsupported("single")
+                bits, pops = payload
+                emap = []
+                progs, count, slots = blocking.compile_pass_batched(bits, pops, n, self.cdt, 1, emap)
+                pos, cols = [], []
+                for off, i, nent in emap:
+                    if id(pops[i]) in src:
+                        pos.append(off // isz + np.arange(nent))
+                        cols.append(first_col(pops[i]) + np.arange(nent))
+                cat = (lambda a: np.concatenate(a) if a else np.zeros(0, dtype=np.int64))
+                out.append(_SweepPass(bits, count, slots, progs[0].copy(), cat(pos), cat(cols)))
+
+    def data(self, params: np.ndarray) -> np.ndarray:
+        """Per-set data table [B, ncols] for parameter sets [B, P]."""
+        nb = params.shape[0]
+        table = np.empty((nb, self.ncols), dtype=self.cdt)
+        for name, slots, nent, c0, c1 in self.groups:
+            g = slots.size
+            d, dd = _rotation_data(name, params[:, slots].reshape(-1), self.cdt)
+            table[:, c0:c1] = d.reshape(nb, g * nent)
+            table[:, c1:c1 + g * nent] = dd.reshape(nb, g * nent)
+        return table
+
+    def programs(self, p: _SweepPass, table: np.ndarray) -> np.ndarray:
+        """Programs [B, nbytes] uint8 of pass ``p`` for the sets of ``table``."""
+        out = np.empty((table.shape[0], p.row.size), dtype=np.uint8)
+        out[:] = p.row
+        if p.pos.size:
+            out.view(self.cdt)[:, p.pos] = table[:, p.cols]
+        return out
+
+
+def _upload_programs(passes, template: SweepTemplate, table: np.ndarray, device):
+    """One host-to-device copy of the programs of ``passes`` for the sets of
+    ``table``. Returns the device buffer and, per pass, (byte offset of its
+    first set's program, stride between sets, program bytes); every program
+    starts on 256 bytes and the buffer ends with the padding the kernels may
+    read past a record stream."""
+    nb = table.shape[0]
+    layout, total = [], 0
+    for p in passes:
+        nbytes = p.row.size
+        stride = (nbytes + 255) // 256 * 256
+        layout.append((total, stride, nbytes))
+        total += nb * stride
+    host = torch.empty(total + blocking.PAD_BYTES, dtype=torch.uint8).pin_memory()
+    buf = host.numpy()
+    buf[total:] = 0
+    for p, (off, stride, nbytes) in zip(passes, layout):
+        buf[off: off + nb * stride].reshape(nb, stride)[:, :nbytes] = template.programs(p, table)
+    return host.to(device, non_blocking=True), layout
+
+
+def _structure_key(backend, circuit):
+    """Everything the sweep's records depend on: per gate its class and qubits,
+    and the parameter slot of a trainable gate or the parameter values of a
+    fixed one. None when some gate cannot be described."""
+    slot_of = {id(g): k for k, g in enumerate(circuit.trainable_gates)}
+    parts = []
+    for g in circuit.queue:
+        if isinstance(g, gates.M):
+            continue
+        if id(g) in slot_of and g.trainable:
+            parts.append((type(g).__name__, g.target_qubits, g.control_qubits, g.is_controlled_by, slot_of[id(g)]))
+            continue
+        k = backend._gate_key(g)
+        if k is None:
+            return None
+        parts.append(k)
+    return circuit.nqubits, backend.dtype, tuple(parts)
+
+
+_TEMPLATE_CACHE_SIZE = 8
+
+
+def sweep_template(backend, circuit) -> SweepTemplate:
+    """The circuit's SweepTemplate, from the backend's cache when a circuit of
+    the same structure was seen (circuits that are not described exactly are
+    compiled every time). Raises _Unsupported."""
+    cdt = np.complex128 if backend.dtype == "complex128" else np.complex64
+    key = _structure_key(backend, circuit)
+    cache = backend.__dict__.setdefault("_sweep_templates", {})
+    if key is not None and key in cache:
+        tmpl = cache.pop(key)
+        cache[key] = tmpl  # most recent last
+    else:
+        try:
+            tmpl = SweepTemplate(backend, circuit, cdt)
+        except _Unsupported as e:
+            tmpl = e
+        if key is not None:
+            cache[key] = tmpl
+            while len(cache) > _TEMPLATE_CACHE_SIZE:
+                cache.pop(next(iter(cache)))
+    if isinstance(tmpl, _Unsupported):
+        raise tmpl
+    return tmpl
 
 
 def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.ndarray):
@@ -330,14 +466,15 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
     n = circuit.nqubits
     if circuit.density_matrix or n < blocking.TILE_BITS:
         raise _Unsupported("size")
+    tmpl = sweep_template(backend, circuit)
+    if params.shape[1] != tmpl.nparams:
+        raise _Unsupported("parameter count")
     nb_all = params.shape[0]
-    fwd, rev, nparams = _batched_ops(backend, circuit, params)
+    nparams = tmpl.nparams
     xm, zm, ny, coeffs = hamiltonian.pauli_tables()
     dev = backend.torch_device
     tdt = torch.complex128 if backend.dtype == "complex128" else torch.complex64
-    cdt = np.complex128 if backend.dtype == "complex128" else np.complex64
     code = _lib.C128 if backend.dtype == "complex128" else _lib.C64
-    low = 3 if code == _lib.C128 else 4
     lib = backend.lib
     state_bytes = (2**n) * (16 if code == _lib.C128 else 8)
     free = torch.cuda.mem_get_info(dev)[0]
@@ -352,48 +489,36 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
     for b0 in range(0, nb_all, chunk):
         b1 = min(nb_all, b0 + chunk)
         nb = b1 - b0
-        sel = slice(b0, b1)
-
-        def sub(ops):
-            for op in ops:
-                yield BatchOp(op.kind, op.targets, op.controls, op.data[sel], op.slot)
-
+        table = tmpl.data(params[b0:b1])
+        keep = []
         psi = torch.zeros((nb, 2**n), dtype=tdt, device=dev)
         psi[:, 0] = 1
-        keep = []
-        for kind, payload in blocking.partition(sub(fwd), n, low, singles=False):
-            if kind != "block":
-                raise _Unsupported("single")
-            bits, pops = payload
-            progs, count, _ = blocking.compile_pass_batched(bits, pops, n, cdt, nb)
-            pdev, stride, nbytes = _upload_programs(progs, dev)
-            _lib.check(lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(bits),
-                                                      pdev.data_ptr(), stride, nbytes, count, stream), lib)
-            keep.append(pdev)
+        pdev, layout = _upload_programs(tmpl.fwd, tmpl, table, dev)
+        keep.append(pdev)
+        for p, (off, stride, nbytes) in zip(tmpl.fwd, layout):
+            _lib.check(lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),
+                                                      pdev.data_ptr() + off, stride, nbytes, p.count, stream), lib)
         lam = torch.empty_like(psi)
         for b in range(nb):
             _lib.check(lib.qsim_apply_pauli_sum(psi[b].data_ptr(), n, code, len(c), xm.ctypes.data, zm.ctypes.data,
                                                 ny.ctypes.data, c.ctypes.data, lam[b].data_ptr(), stream), lib)
             energies[b0 + b] = backend.expectation_pauli(psi[b], n, xm, zm, ny, c) + hamiltonian.constant
-        gsub = grads[sel]
-        for kind, payload in blocking.partition(sub(rev), n, low, singles=False):
-            if kind != "block":
-                raise _Unsupported("single")
-            bits, pops = payload
-            progs, count, slots = blocking.compile_pass_batched(bits, pops, n, cdt, nb)
-            pdev, stride, nbytes = _upload_programs(progs, dev)
-            npart = max(1, lib.qsim_adjoint_partials_count(n, nb, len(slots)))
+        gsub = grads[b0:b1]
+        pdev, layout = _upload_programs(tmpl.rev, tmpl, table, dev)
+        keep.append(pdev)
+        for p, (off, stride, nbytes) in zip(tmpl.rev, layout):
+            npart = max(1, lib.qsim_adjoint_partials_count(n, nb, len(p.slots)))
             part = torch.empty(npart, dtype=torch.float64, device=dev)
-            _lib.check(lib.qsim_adjoint_blocked(psi.data_ptr(), lam.data_ptr(), nb, n, code, _lib.int_array(bits),
-                                                pdev.data_ptr(), stride, nbytes, count, len(slots),
+            _lib.check(lib.qsim_adjoint_blocked(psi.data_ptr(), lam.data_ptr(), nb, n, code, _lib.int_array(p.bits),
+                                                pdev.data_ptr() + off, stride, nbytes, p.count, len(p.slots),
                                                 part.data_ptr(), npart, stream), lib)
-            if slots:
-                pof = torch.tensor(slots, dtype=torch.int32).pin_memory().to(dev, non_blocking=True)
-                _lib.check(lib.qsim_adjoint_reduce(part.data_ptr(), nb, n, len(slots), pof.data_ptr(), nparams,
-                                                   gsub.data_ptr(), stream), lib)
-                keep.append(pof)
-            keep.append(pdev)
             keep.append(part)
+            if p.slots:
+                pof = p.dev_slots.get(dev)
+                if pof is None:
+                    pof = p.dev_slots[dev] = torch.tensor(p.slots, dtype=torch.int32).to(dev)
+                _lib.check(lib.qsim_adjoint_reduce(part.data_ptr(), nb, n, len(p.slots), pof.data_ptr(), nparams,
+                                                   gsub.data_ptr(), stream), lib)
         for t in keep:
             t.record_stream(torch.cuda.current_stream(dev))
     return energies, grads

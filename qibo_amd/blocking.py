@@ -507,27 +507,33 @@ class BatchOp:
 
 
 def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
-                         nbatch: int) -> Tuple[np.ndarray, int, list]:
+                         nbatch: int, entry_map: list = None) -> Tuple[np.ndarray, int, list]:
     """Records of one blocked pass for a batch of parameter sets.
 
     Returns (programs [B, nbytes] uint8, record count, gradient slot of each
     INNER record in program order). Every set's program has the same records
     (same layout and headers), only the entries differ. One record per
     operation: DENSE1_REAL (when the 2x2 is real for every set), DENSE1,
-    DIAG, LANE_PHASE, INNER1, INNER_DIAG and MAP."""
+    DIAG, LANE_PHASE, INNER1, INNER_DIAG and MAP. With ``entry_map`` (a list)
+    it receives (byte offset of the entries in a program, index of the
+    operation in ``ops``, entry count) for every record that has entries:
+    the entries are the operation's data row, flattened."""
     local = {b: i for i, b in enumerate(tile_bits)}
     items = [_Item(op, nqubits, i) for i, op in enumerate(ops)]
     npred, succ = _dependencies(items, False)
     ready = [i for i in range(len(items)) if npred[i] == 0]
     isz = np.dtype(cdtype).itemsize
     headers, entries = [], []  # per record: header bytes, [B, nent] entries or None
+    sources = []  # per record: index of the operation in ``ops``
     slots = []
     cur = 0
+    src = -1
 
     def emit(op, arg, cmask, nt, bits, data, rreq=0):
         nent = 0 if data is None else data.shape[-1]
         b = list(bits) + [0] * (8 - len(bits))
         headers.append(_HDR_STRUCT.pack(op, arg, nt, rreq, nent, cmask, *b, 0))
+        sources.append(src)
         if data is None:
             entries.append(None)
         else:
@@ -565,6 +571,7 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
             cur = arg
             continue
         op = items[arg].op
+        src = arg
         rb = reg_bits(cur)
         if op.kind in ("diag", "innerd"):
             tb = [nqubits - 1 - q for q in op.targets]
@@ -597,12 +604,17 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
             emit(OP_DENSE1_REAL if real else OP_DENSE1, j, cm, 0, (), m, rreq)
     # pack: the same headers for every set, entries per set, records padded to 16
     cols = []
-    for hdr, ent in zip(headers, entries):
+    off = 0
+    for hdr, ent, i in zip(headers, entries, sources):
         cols.append(np.broadcast_to(np.frombuffer(hdr, dtype=np.uint8), (nbatch, HEADER_BYTES)))
+        off += HEADER_BYTES
         if ent is not None:
             raw = np.ascontiguousarray(ent).view(np.uint8).reshape(nbatch, -1)
             pad = (-raw.shape[1]) % 16
             cols.append(raw)
+            if entry_map is not None:
+                entry_map.append((off, i, ent.shape[-1]))
+            off += raw.shape[1] + pad
             if pad:
                 cols.append(np.zeros((nbatch, pad), dtype=np.uint8))
     prog = np.concatenate(cols, axis=1) if cols else np.zeros((nbatch, 0), dtype=np.uint8)

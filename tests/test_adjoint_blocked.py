@@ -12,7 +12,7 @@ import pytest
 
 from oracle import OracleBackend
 from qibo_amd import Circuit, blocking, gates, hamiltonians, models
-from qibo_amd.adjoint import _batched_ops
+from qibo_amd.adjoint import SweepTemplate, _batched_ops
 from qibo_amd.backend import GatePlanner
 
 from tests.test_blocking import GROUPS, decode
@@ -78,27 +78,52 @@ def _pauli_apply(ham, psi, n):
 
 
 def emulate_blocked_adjoint(circuit, ham, params):
+    """Runs the programs the GPU sweep would launch: the circuit's
+    SweepTemplate filled with the sets' data table."""
     n = circuit.nqubits
     nb, npar = params.shape
-    gp = GatePlanner()
-    fwd, rev, _ = _batched_ops(gp, circuit, params)
+    tmpl = SweepTemplate(GatePlanner(), circuit, np.complex128)
+    table = tmpl.data(params)
     psi = np.zeros((nb, 2**n), dtype=np.complex128)
     psi[:, 0] = 1
     grads = np.zeros((nb, npar))
-    for kind, (bits, pops) in blocking.partition(fwd, n, 3, singles=False):
-        progs, count, slots = blocking.compile_pass_batched(bits, pops, n, np.complex128, nb)
-        assert not slots
+    for p in tmpl.fwd:
+        progs = tmpl.programs(p, table)
+        assert not p.slots
         for b in range(nb):
-            recs = decode(progs[b].tobytes(), 0, count, 16)
-            _apply((psi[b], np.zeros_like(psi[b])), n, bits, recs, grads, slots, b)
+            recs = decode(progs[b].tobytes(), 0, p.count, 16)
+            _apply((psi[b], np.zeros_like(psi[b])), n, p.bits, recs, grads, p.slots, b)
     lam = _pauli_apply(ham, psi, n)
     energies = np.real(np.sum(np.conj(psi) * lam, axis=1)) + ham.constant
-    for kind, (bits, pops) in blocking.partition(rev, n, 3, singles=False):
-        progs, count, slots = blocking.compile_pass_batched(bits, pops, n, np.complex128, nb)
+    for p in tmpl.rev:
+        progs = tmpl.programs(p, table)
         for b in range(nb):
-            recs = decode(progs[b].tobytes(), 0, count, 16)
-            _apply((psi[b], lam[b]), n, bits, recs, grads, slots, b)
+            recs = decode(progs[b].tobytes(), 0, p.count, 16)
+            _apply((psi[b], lam[b]), n, p.bits, recs, grads, p.slots, b)
     return energies, grads
+
+
+@pytest.mark.parametrize("cdt", [np.complex128, np.complex64])
+def test_template_programs_equal_direct_compilation(cdt):
+    """The template's programs (records compiled once, entries scattered per
+    set) are byte-identical to compiling every pass from the sets' own data."""
+    n = 13
+    c = _ansatz(n, 2)
+    params = np.random.default_rng(5).uniform(-7, 7, (3, len(c.get_parameters())))
+    params[1, :4] = 0.0  # X rotations that are real in one set keep the complex record
+    tmpl = SweepTemplate(GatePlanner(), c, cdt)
+    gp = GatePlanner(dtype="complex128" if cdt == np.complex128 else "complex64")
+    fwd, rev, npar = _batched_ops(gp, c, params)
+    assert npar == tmpl.nparams
+    table = tmpl.data(params)
+    low = 3 if cdt == np.complex128 else 4
+    for ops, passes in ((fwd, tmpl.fwd), (rev, tmpl.rev)):
+        direct = list(blocking.partition(ops, n, low, singles=False))
+        assert len(direct) == len(passes)
+        for (kind, (bits, pops)), p in zip(direct, passes):
+            progs, count, slots = blocking.compile_pass_batched(bits, pops, n, cdt, params.shape[0])
+            assert list(bits) == p.bits and count == p.count and list(slots) == p.slots
+            assert np.array_equal(tmpl.programs(p, table), progs)
 
 
 def _ansatz(n, layers):

@@ -344,6 +344,29 @@ def test_errors(backend):
         backend.sample_shots(np.array([0.5, 0.6]), 10)
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_probability_checks(backend, dtype):
+    """The checks np.random.choice makes before drawing, on device vectors of
+    2^20 entries: NaN, a negative entry, a sum off by more than sqrt(eps);
+    no uniform is drawn when a check fails."""
+    n = 1 << 20
+    p = np.full(n, 1.0 / n)
+    bad = [(np.nan, "NaN"), (-1e-9, "non-negative")]
+    for value, msg in bad:
+        q = p.copy()
+        q[n // 3] = value
+        state = np.random.get_state()
+        with pytest.raises(ValueError, match=msg):
+            backend.sample_shots(torch.as_tensor(q.astype(dtype), device=backend.torch_device), 5)
+        after = np.random.get_state()
+        assert state[2] == after[2] and np.array_equal(state[1], after[1])
+    with pytest.raises(ValueError, match="sum to 1"):
+        backend.sample_shots(torch.as_tensor((p * 1.001).astype(dtype), device=backend.torch_device), 5)
+    # a sum within tolerance passes
+    got = backend.sample_shots(torch.as_tensor((p * (1 + 1e-9)).astype(dtype), device=backend.torch_device), 5)
+    assert got.shape[0] == 5
+
+
 @pytest.mark.parametrize("m", [0, 3, 7, 8, 13, 14, 20, 22])
 def test_np_sum_bitwise(backend, rng, m):
     p = rng.random(2**m) * 10.0 ** rng.integers(-6, 6, size=2**m)
