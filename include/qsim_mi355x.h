@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 4
+#define QSIM_ABI_VERSION 5
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -255,6 +255,49 @@ int qsim_apply_pauli_sum(const void* state, int nqubits, int dtype, int nterms,
 int qsim_inner_matrix(const void* bra, const void* ket, int nqubits, int dtype, const int* targets,
                       int ntargets, const int* controls, int ncontrols, const void* matrix,
                       double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- density matrices ----------------------------------------------------
+ * A density matrix of n qubits (1..17) is 2^(2n) complex values, row-major:
+ * element (r, c) at r * 2^n + c, i.e. the state vector of 2n qubits whose
+ * qubit q is row qubit q and qubit n + q is column qubit q. Unitary gates run
+ * through the state-vector entry points on that 2n-qubit vector (U on the row
+ * qubits, conj(U) on the column qubits), as the reference's einsum pair in
+ * apply_gate (abstract.py:2322-2361) and _apply_gate_controlled_by_density_matrix
+ * (abstract.py:3199-3235). Every product is rounded before the sum it enters,
+ * as numpy evaluates the reference's expressions. */
+
+/* y = a * x (accumulate = 0), y = y + a * x (accumulate = 1) or y = x / a
+ * (accumulate = 2) over nelem complex values: the terms of
+ * Backend.apply_channel on density matrices, (1 - sum p) rho + sum_k p_k U_k
+ * rho U_k^dagger (abstract.py:2297-2311), and the mean of the final states of
+ * repeated execution (abstract.py:2603-2609). y and x must not overlap. */
+int qsim_dm_axpy(void* y, const void* x, uint64_t nelem, int dtype, double a, int accumulate, void* stream);
+
+/* In place: rho <- (1 - lam) rho + lam Tr_T(rho) (x) I / 2^k on the k = 1..6
+ * `targets` (depolarizing_error_density_matrix, abstract.py:2095-2131). */
+int qsim_dm_depolarize(void* rho, int nqubits, int dtype, const int* targets, int ntargets, double lam,
+                       void* stream);
+
+/* In place: rho <- (1 - p0 - p1) rho + Tr_q(rho) (x) (p0 |0><0| + p1 |1><1|)
+ * on `qubit` (reset_error_density_matrix, abstract.py:2212-2232). */
+int qsim_dm_reset(void* rho, int nqubits, int dtype, int qubit, double p0, double p1, void* stream);
+
+/* probs[o] = |sum over the unmeasured qubits of rho[i, i]| for the outcomes o
+ * of `qubits` (requested order, first = most significant): the density-matrix
+ * branch of calculate_probabilities (abstract.py:2742-2750). F64 output for
+ * complex128, F32 for complex64; with every qubit measured it equals numpy's
+ * abs of the diagonal bit for bit. */
+int qsim_dm_probabilities(const void* rho, int nqubits, int dtype, const int* qubits, int nmeasured, void* probs,
+                          void* stream);
+
+/* out[0] = sum_t coeffs[t] * Re Tr(P_t rho) with P_t = i^ny[t] X^xmasks[t]
+ * Z^zmasks[t] (the same tables as qsim_expval_pauli), FP64 accumulation: the
+ * density-matrix branch of exp_value_observable_symbolic (abstract.py:2979-3005).
+ * `workspace` must hold qsim_dm_expval_workspace(nterms) bytes. */
+size_t qsim_dm_expval_workspace(int nterms);
+int qsim_dm_expval_pauli(const void* rho, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
+                         const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

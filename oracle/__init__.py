@@ -14,3 +14,4 @@ listed in ``tests/test_oracle_pinning.py``.
 """
 
 from .numpy_backend import OracleBackend, OracleMatrices  # noqa: F401
+from .density import OracleDensity  # noqa: F401,E402

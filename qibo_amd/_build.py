@@ -28,6 +28,7 @@ SOURCES = {
     "measure.hip": ["-ffp-contract=off"],
     "expval.hip": [],
     "blocked.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"],
+    "density.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
@@ -57,6 +57,12 @@ SIGNATURES = {
     "qsim_adjoint_partials_count": (_sz, [_i, _i, _i]),
     "qsim_adjoint_blocked": (_i, [_vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _i, _vp, _sz, _vp]),
     "qsim_adjoint_reduce": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
+    "qsim_dm_axpy": (_i, [_vp, _vp, ctypes.c_uint64, _i, ctypes.c_double, _i, _vp]),
+    "qsim_dm_depolarize": (_i, [_vp, _i, _i, _c_int_p, _i, ctypes.c_double, _vp]),
+    "qsim_dm_reset": (_i, [_vp, _i, _i, _i, ctypes.c_double, ctypes.c_double, _vp]),
+    "qsim_dm_probabilities": (_i, [_vp, _i, _i, _c_int_p, _i, _vp, _vp]),
+    "qsim_dm_expval_workspace": (_sz, [_i]),
+    "qsim_dm_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
 }
 
 _lock = threading.Lock()

@@ -408,6 +408,17 @@ class MI355XBackend(GatePlanner):
     def _validate_nqubits(self, nqubits, density_matrix=False):
         if not isinstance(nqubits, int) or nqubits < 1:
             raise_error(ValueError, f"nqubits must be a positive integer, but got {nqubits}.")
+        if density_matrix:
+            if nqubits > _MAX_DM_QUBITS:
+                raise_error(ValueError, f"qibo_amd supports density matrices of at most {_MAX_DM_QUBITS} qubits.")
+            if config.MAX_QUBITS_DM != -1 and nqubits > config.MAX_QUBITS_DM:
+                mem_gib = 4**nqubits * 16 / 1024**3
+                raise_error(
+                    ValueError,
+                    f"Requested {nqubits} qubits for density matrix allocation, which would require "
+                    f"~{mem_gib:.1f} GiB of memory. The maximum allowed is {config.MAX_QUBITS_DM} qubits.",
+                )
+            return
         if nqubits > _MAX_QUBITS:
             raise_error(ValueError, f"qibo_amd supports at most {_MAX_QUBITS} qubits.")
         if config.MAX_QUBITS != -1 and nqubits > config.MAX_QUBITS:
@@ -420,7 +431,12 @@ class MI355XBackend(GatePlanner):
 
     def zero_state(self, nqubits: int, density_matrix: bool = False, dtype=None):
         if density_matrix:
-            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+            # |0><0|: the zero state of the 2n-qubit vectorized matrix
+            self._validate_nqubits(nqubits, density_matrix=True)
+            state = self._alloc_state(2 * nqubits, dtype)
+            _lib.check(self.lib.qsim_zero_state(state.data_ptr(), 2 * nqubits, self._dtype_code(state),
+                                                self._stream(state)), self.lib)
+            return state.view(2**nqubits, 2**nqubits)
         self._validate_nqubits(nqubits)
         state = self._alloc_state(nqubits, dtype)
         _lib.check(self.lib.qsim_zero_state(state.data_ptr(), nqubits, self._dtype_code(state),
@@ -496,29 +512,118 @@ class MI355XBackend(GatePlanner):
         return state
 
     def apply_gate(self, gate, state, nqubits):
+        if state.dim() == 2:
+            self._dm_check(state, nqubits, "apply_gate")
+            flat = state.view(-1)
+            for op in _dm_pair(self.plan_gate(gate), nqubits):
+                self.apply_op(op, flat, 2 * nqubits)
+            return state
         if state.dim() != 1:
-            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
+            raise_error(ValueError, f"State of shape {tuple(state.shape)} is neither a vector nor a matrix.")
         return self.apply_op(self.plan_gate(gate), state, nqubits)
+
+    # -- density matrices --------------------------------------------------------
+    # A density matrix is a (2^n, 2^n) tensor whose row-major storage is the
+    # state vector of 2n qubits (row qubit q = qubit q, column qubit q = qubit
+    # n + q): a gate is U on the row qubits and conj(U) on the column qubits,
+    # as the reference's pair of einsums (abstract.py:2341-2348, 3199-3235).
+    @staticmethod
+    def _dm_check(state, nqubits: int, what: str):
+        """The kernels read and write 4^n elements: refuse anything that is not a
+        contiguous (2^n, 2^n) device matrix before a launch."""
+        dim = 2**nqubits
+        if not isinstance(state, torch.Tensor) or state.dim() != 2 or tuple(state.shape) != (dim, dim):
+            raise_error(ValueError, f"{what} needs a ({dim}, {dim}) density matrix, got shape "
+                                    f"{tuple(getattr(state, 'shape', ()))}.")
+        if not state.is_contiguous():
+            raise_error(ValueError, f"{what} needs a contiguous density matrix.")
+
+    def apply_gate_half_density_matrix(self, gate, state, nqubits):
+        """U rho: the gate on the row qubits only (abstract.py:2363-2383)."""
+        if gate.is_controlled_by:
+            raise_error(NotImplementedError, "Gate density matrix half call is not implemented for "
+                                             "``controlled_by``gates.")
+        state = self.cast(state, dtype=state.dtype, copy=True).contiguous()
+        self._dm_check(state, nqubits, "apply_gate_half_density_matrix")
+        self.apply_op(self.plan_gate(gate), state.view(-1), 2 * nqubits)
+        return state
+
+    def dm_axpy(self, y: torch.Tensor, x: torch.Tensor, a: float, accumulate: bool = True):
+        """y = y + a x (or y = a x), each product rounded before the sum."""
+        if y.shape != x.shape or y.dtype != x.dtype:
+            raise_error(ValueError, "dm_axpy: operands differ in shape or dtype.")
+        if y.data_ptr() == x.data_ptr():
+            raise_error(ValueError, "dm_axpy: y and x must be different tensors.")
+        _lib.check(self.lib.qsim_dm_axpy(y.data_ptr(), x.data_ptr(), x.numel(), self._dtype_code(x), float(a),
+                                         1 if accumulate else 0, self._stream(x)), self.lib)
+        return y
+
+    def apply_channel(self, channel, state, nqubits):
+        """Density matrices: (1 - sum p) rho + sum_k p_k G_k rho G_k^dagger, in
+        the reference's order of terms. State vectors: one of the channel's
+        gates (or none) drawn with its probability from numpy's global
+        generator (abstract.py:2297-2320)."""
+        if state.dim() == 2:
+            self._dm_check(state, nqubits, "apply_channel")
+            new = torch.empty_like(state)
+            self.dm_axpy(new, state, 1 - channel.coefficient_sum, accumulate=False)
+            tmp = torch.empty_like(state)
+            for coeff, gate in zip(channel.coefficients, channel.gates):
+                tmp.copy_(state)
+                self.apply_gate(gate, tmp, nqubits)
+                self.dm_axpy(new, tmp, coeff)
+            return new
+        probabilities = tuple(channel.coefficients) + (1 - sum(channel.coefficients),)
+        index = int(self.sample_shots(np.asarray(probabilities, dtype=np.float64), 1)[0])
+        if index != len(channel.gates):
+            state = self.apply_gate(channel.gates[index], state, nqubits)
+        return state
+
+    def depolarizing_error_density_matrix(self, gate, state, nqubits):
+        """(1 - lam) rho + lam Tr_T(rho) (x) I / 2^k, in place (abstract.py:2095-2131)."""
+        self._dm_check(state, nqubits, "depolarizing_error_density_matrix")
+        targets = [int(q) for q in gate.target_qubits]
+        _lib.check(self.lib.qsim_dm_depolarize(state.data_ptr(), nqubits, self._dtype_code(state),
+                                               _lib.int_array(targets), len(targets),
+                                               float(gate.init_kwargs["lam"]), self._stream(state)), self.lib)
+        return state
+
+    def reset_error_density_matrix(self, gate, state, nqubits):
+        """(1 - p0 - p1) rho + Tr_q(rho) (x) (p0 |0><0| + p1 |1><1|), in place
+        (abstract.py:2212-2232)."""
+        self._dm_check(state, nqubits, "reset_error_density_matrix")
+        qubit = int(gate.target_qubits[0])
+        _lib.check(self.lib.qsim_dm_reset(state.data_ptr(), nqubits, self._dtype_code(state), qubit,
+                                          float(gate.init_kwargs["p_0"]), float(gate.init_kwargs["p_1"]),
+                                          self._stream(state)), self.lib)
+        return state
+
+    def thermal_error_density_matrix(self, gate, state, nqubits):
+        """``gate`` (a 4x4 map on row qubit q and column qubit q) applied to the
+        vectorized matrix as a 2n-qubit state (abstract.py:2234-2241)."""
+        self._dm_check(state, nqubits, "thermal_error_density_matrix")
+        self.apply_op(self.plan_gate(gate), state.view(-1), 2 * nqubits)
+        return state
 
     # -- circuit execution -----------------------------------------------------
     def execute_circuit(self, circuit, initial_state=None, nshots: int = 1000):
         from .circuit import Circuit
 
         nqubits = circuit.nqubits
-        if circuit.density_matrix:
-            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
-        self._validate_nqubits(nqubits)
+        dm = bool(circuit.density_matrix)
+        self._validate_nqubits(nqubits, density_matrix=dm)
         if isinstance(initial_state, Circuit):
             return self.execute_circuit(initial_state + circuit, None, nshots)
         if initial_state is not None:
             initial_state = self.cast(initial_state, dtype=self.dtype, copy=True)
-            if tuple(initial_state.shape) != (2**nqubits,):
+            valid = (2**nqubits, 2**nqubits) if dm else (2**nqubits,)
+            if tuple(initial_state.shape) != valid:
                 raise_error(
                     ValueError,
                     f"Given initial state has shape {tuple(initial_state.shape)}instead of the "
-                    f"expected {(2**nqubits,)}.",
+                    f"expected {valid}.",
                 )
-        if circuit.has_collapse or circuit.repeated_execution:
+        if circuit.repeated_execution:
             if not circuit.measurements and not circuit.has_collapse:
                 raise_error(
                     RuntimeError,
@@ -541,7 +646,10 @@ class MI355XBackend(GatePlanner):
         from .result import CircuitResult, QuantumState
 
         nqubits = circuit.nqubits
-        if initial_state is None and self.blocking and nqubits >= blocking.TILE_BITS:
+        if circuit.density_matrix:
+            state = self.zero_state(nqubits, density_matrix=True) if initial_state is None else initial_state
+            state = self.run_queue_density_matrix(circuit.queue, state, nqubits)
+        elif initial_state is None and self.blocking and nqubits >= blocking.TILE_BITS:
             # |0...0> is produced by the first blocked pass itself (no zero fill)
             self._validate_nqubits(nqubits)
             state = self._alloc_state(nqubits)  # budget check and MemoryError as zero_state
@@ -564,9 +672,10 @@ class MI355XBackend(GatePlanner):
         collapses the state; the final measurements are sampled once per run.
         The uniforms are drawn in the reference's order, so for the same seed
         the samples are the reference's."""
-        from .result import MeasurementOutcomes
+        from .result import MeasurementOutcomes, QuantumState
 
-        if circuit.has_collapse and not circuit.measurements:
+        dm = bool(circuit.density_matrix)
+        if circuit.has_collapse and not circuit.measurements and not dm:
             raise_error(
                 RuntimeError,
                 "The circuit contains only collapsing measurements (`collapse=True`) but "
@@ -576,18 +685,41 @@ class MI355XBackend(GatePlanner):
         from .result import CircuitResult
 
         nqubits = circuit.nqubits
-        state_copy = self.zero_state(nqubits) if initial_state is None else self.cast(initial_state, copy=True)
+        state_copy = (self.zero_state(nqubits, density_matrix=dm) if initial_state is None
+                      else self.cast(initial_state, copy=True))
         results, samples = [], []
+        total = None  # density matrices: running sum of the final states, in shot order
         for _ in range(nshots):
             state = state_copy.clone()
-            state = self.run_queue(circuit.queue, state, nqubits)
+            if dm:
+                state = self.run_queue_density_matrix(circuit.queue, state, nqubits)
+                if total is None:
+                    total = self.dm_axpy(torch.empty_like(state), state, 1.0, accumulate=False)
+                else:
+                    self.dm_axpy(total, state, 1.0)
+            else:
+                state = self.run_queue(circuit.queue, state, nqubits)
             if circuit.measurements:
                 result = CircuitResult(state, circuit.measurements, backend=self, nshots=1)
                 sample = result.samples()[0]
                 results.append(sample)
-                samples.append("".join(str(int(x)) for x in sample.tolist()))
+                if not dm:
+                    samples.append("".join(str(int(x)) for x in sample.tolist()))
                 for gate in circuit.measurements:
                     gate.result.reset()
+        if dm:
+            # the mean of the final states (abstract.py:2603-2623): sum / nshots
+            final_state = torch.empty_like(total)
+            _lib.check(self.lib.qsim_dm_axpy(final_state.data_ptr(), total.data_ptr(), total.numel(),
+                                             self._dtype_code(total), float(nshots), 2, self._stream(total)),
+                       self.lib)
+            if circuit.measurements:
+                final = CircuitResult(final_state, circuit.measurements, backend=self,
+                                      samples=self.aggregate_shots(results), nshots=nshots)
+            else:
+                final = QuantumState(final_state, backend=self)
+            circuit._final_state = final
+            return final
         final_result = MeasurementOutcomes(
             circuit.measurements, backend=self, samples=self.aggregate_shots(results), nshots=nshots
         )
@@ -600,10 +732,19 @@ class MI355XBackend(GatePlanner):
         and renormalise it (abstract.py:2424-2440, 3279-3304): the other
         amplitudes become zero, the kept ones are divided by the square root of
         the outcome's probability."""
-        if density_matrix:
-            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
         qubits = [int(q) for q in qubits]
         outcome = int(shot[0]) if hasattr(shot, "__len__") else int(shot)
+        if density_matrix:
+            # keep the (outcome, outcome) block of rows and columns and divide it by
+            # its trace (_collapse_density_matrix, abstract.py:3249-3277)
+            self._dm_check(state, nqubits, "collapse_state")
+            m = len(qubits)
+            norm = float(self.calculate_probabilities(state, qubits, nqubits, True)[outcome]) if normalize else 1.0
+            both = qubits + [q + nqubits for q in qubits]
+            _lib.check(self.lib.qsim_collapse(state.data_ptr(), 2 * nqubits, self._dtype_code(state),
+                                              _lib.int_array(both), 2 * m, (outcome << m) | outcome, norm,
+                                              self._stream(state)), self.lib)
+            return state
         if normalize:
             p = self.calculate_probabilities(state, qubits, nqubits)
             norm = math.sqrt(float(p[outcome]))
@@ -628,6 +769,7 @@ class MI355XBackend(GatePlanner):
         stand for |0...0>: a first blocked pass produces it itself, anything
         else fills it first.
         """
+        from .channels import Channel
         from .measurements_gate import M
         from .special import FusedGate
 
@@ -665,7 +807,7 @@ class MI355XBackend(GatePlanner):
                         continue
                     stop.append(gate)
                     return
-                if isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate):
+                if isinstance(gate, (gates_mod.SpecialGate, Channel)) and not isinstance(gate, FusedGate):
                     stop.append(gate)
                     return
                 yield self.plan_gate(gate)
@@ -708,8 +850,8 @@ class MI355XBackend(GatePlanner):
         if isinstance(gate, FusedGate):
             parts = tuple(self._gate_key(g) for g in gate.gates)
             return None if any(k is None for k in parts) else ("Fused", gate.target_qubits, parts)
-        if isinstance(gate, gates_mod.SpecialGate):
-            return None
+        if isinstance(gate, gates_mod.SpecialGate) or getattr(gate, "coefficients", None) is not None:
+            return None  # callbacks and channels run outside the compiled passes
         params = gate.parameters if isinstance(gate, gates_mod.ParametrizedGate) else ()
         pk = tuple(self._param_key(x) for x in params)
         if any(k is None for k in pk):
@@ -766,25 +908,68 @@ class MI355XBackend(GatePlanner):
         from .measurements_gate import M
         from .special import FusedGate
 
-        pad = blocking.TILE_BITS - nqubits
+        from .channels import Channel
+
         ops = []
         for gate in queue:
             if isinstance(gate, M) and not gate.collapse:
                 gate.result.backend = self
                 continue
-            if isinstance(gate, M) or (isinstance(gate, gates_mod.SpecialGate) and not isinstance(gate, FusedGate)):
+            if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
+                                                  and not isinstance(gate, FusedGate)):
                 for g in queue:  # callbacks need the real state between gates
                     state = g.apply(self, state, nqubits)
                 return state
-            op = self.plan_gate(gate)
-            ops.append(_Op(op.kind, tuple(q + pad for q in op.targets),
-                           tuple(q + pad for q in op.controls), op.data))
+            ops.append(self.plan_gate(gate))
+        return self._run_ops(ops, state, nqubits)
+
+    def _run_ops(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        """Planned ops on an nqubits register, in blocked passes; below 12
+        qubits the register is the low end of a 12-qubit buffer whose extra
+        (most significant) qubits stay |0> (qubit q becomes qubit q + p)."""
         if not ops:
             return state
+        if not self.blocking:
+            for op in ops:
+                self.apply_op(op, state, nqubits)
+            return state
+        if nqubits >= blocking.TILE_BITS:
+            return self._run_blocked(ops, state, nqubits)
+        pad = blocking.TILE_BITS - nqubits
+        shifted = [_Op(op.kind, tuple(q + pad for q in op.targets), tuple(q + pad for q in op.controls), op.data)
+                   for op in ops]
         big = torch.zeros(2**blocking.TILE_BITS, dtype=state.dtype, device=state.device)
         big[: state.numel()].copy_(state)
-        big = self._run_blocked(ops, big, blocking.TILE_BITS)
+        big = self._run_blocked(shifted, big, blocking.TILE_BITS)
         state.copy_(big[: state.numel()])
+        return state
+
+    def run_queue_density_matrix(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
+        """Apply ``queue`` to a density matrix (the loop of abstract.py:3321-3322).
+
+        Unitary gates become pairs of planned ops on the 2n-qubit vectorized
+        matrix (U on the row qubits, conj(U) on the column qubits), grouped into
+        blocked passes; channels, collapsing measurements and other special
+        gates run in order between the passes through ``gate.apply``."""
+        from .channels import Channel
+        from .measurements_gate import M
+        from .special import FusedGate
+
+        state = state.contiguous()
+        self._dm_check(state, nqubits, "run_queue_density_matrix")
+        ops = []
+        for gate in queue:
+            if isinstance(gate, M) and not gate.collapse:
+                gate.result.backend = self
+                continue
+            if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
+                                                  and not isinstance(gate, FusedGate)):
+                self._run_ops(ops, state.view(-1), 2 * nqubits)
+                ops = []
+                state = gate.apply(self, state, nqubits).contiguous()
+                continue
+            ops.extend(_dm_pair(self.plan_gate(gate), nqubits))
+        self._run_ops(ops, state.view(-1), 2 * nqubits)
         return state
 
     def _fill_zero(self, state: torch.Tensor, nqubits: int):
@@ -849,8 +1034,6 @@ class MI355XBackend(GatePlanner):
 
     # -- probabilities and sampling -------------------------------------------
     def calculate_probabilities(self, state, qubits, nqubits, density_matrix=False):
-        if density_matrix:
-            raise_error(NotImplementedError, "qibo_amd simulates state vectors only.")
         if not isinstance(state, torch.Tensor) or state.device.type != "cuda":
             state = self.cast(state, dtype=_np_cdtype(state))
         qubits = [int(q) for q in qubits]
@@ -858,6 +1041,14 @@ class MI355XBackend(GatePlanner):
         rdt = torch.float64 if code == _lib.C128 else torch.float32
         m = len(qubits)
         out = torch.empty(2**m, dtype=rdt, device=state.device)
+        if density_matrix:
+            # |sum of the diagonal over the unmeasured qubits| (abstract.py:2742-2750)
+            state = state.contiguous()
+            self._dm_check(state, nqubits, "calculate_probabilities")
+            rc = self.lib.qsim_dm_probabilities(state.data_ptr(), nqubits, code, _lib.int_array(qubits), m,
+                                                out.data_ptr(), self._stream(state))
+            _lib.check(rc, self.lib)
+            return out
         wsb = self.lib.qsim_probabilities_workspace(nqubits, m)
         ws = self._workspace(wsb, "probs", state.device) if wsb else None
         rc = self.lib.qsim_probabilities(state.data_ptr(), nqubits, code, _lib.int_array(qubits), m,
@@ -1008,9 +1199,24 @@ class MI355XBackend(GatePlanner):
 
     # -- expectation values ------------------------------------------------------
     def expectation_pauli(self, state: torch.Tensor, nqubits: int, xm, zm, ny, coeffs) -> torch.Tensor:
-        """Device float64 scalar ``sum_t c_t <psi|P_t|psi>`` (no host sync)."""
+        """Device float64 scalar ``sum_t c_t <psi|P_t|psi>`` (no host sync); for a
+        density matrix ``sum_t c_t Re Tr(P_t rho)``."""
         nterms = len(coeffs)
         out = torch.empty(1, dtype=torch.float64, device=state.device)
+        if state.dim() == 2:
+            state = state.contiguous()
+            self._dm_check(state, nqubits, "expectation")
+            wsb = self.lib.qsim_dm_expval_workspace(nterms)
+            ws = self._workspace(wsb, "dm_expval", state.device)
+            c = np.ascontiguousarray(coeffs, dtype=np.float64)
+            xm = np.ascontiguousarray(xm, dtype=np.uint64)
+            zm = np.ascontiguousarray(zm, dtype=np.uint64)
+            ny = np.ascontiguousarray(ny, dtype=np.int32)
+            rc = self.lib.qsim_dm_expval_pauli(state.data_ptr(), nqubits, self._dtype_code(state), nterms,
+                                               xm.ctypes.data, zm.ctypes.data, ny.ctypes.data, c.ctypes.data,
+                                               out.data_ptr(), ws.data_ptr(), wsb, self._stream(state))
+            _lib.check(rc, self.lib)
+            return out[0]
         wsb = self.lib.qsim_expval_workspace(nqubits, nterms)
         ws = self._workspace(wsb, "expval", state.device)
         c = np.ascontiguousarray(coeffs, dtype=np.float64)
@@ -1061,6 +1267,17 @@ class MI355XBackend(GatePlanner):
 
 
 _MAX_QUBITS = 36
+_MAX_DM_QUBITS = 17  # 2^34 elements: 256 GiB in complex128
+
+
+def _dm_pair(op: _Op, nqubits: int) -> List[_Op]:
+    """The planned gate on the row qubits and its complex conjugate on the
+    column qubits (qubit q + n) of a vectorized density matrix."""
+    if op.kind == "noop":
+        return []
+    data = None if op.data is None else np.conj(op.data)
+    col = _Op(op.kind, tuple(q + nqubits for q in op.targets), tuple(q + nqubits for q in op.controls), data)
+    return [op, col]
 _BLOCK_OP = _Op("block", (), (), None)
 
 

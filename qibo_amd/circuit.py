@@ -117,7 +117,12 @@ class Circuit:
         self.has_unitary_channel = False
         self.density_matrix = density_matrix
         self.accelerators = None
-        self.repeated_execution = False
+
+    @property
+    def repeated_execution(self) -> bool:
+        """Run once per shot: collapsing measurements, or unitary channels on
+        state vectors (circuit.py:320-324)."""
+        return self.has_collapse or (self.has_unitary_channel and not self.density_matrix)
 
     # -- composition -------------------------------------------------------
     def __add__(self, circuit):
@@ -141,6 +146,7 @@ class Circuit:
         new.trainable_gates = self.trainable_gates
         new.measurements = self.measurements
         new.has_collapse = self.has_collapse
+        new.has_unitary_channel = self.has_unitary_channel
         return new
 
     def copy(self, deep: bool = False):
@@ -205,6 +211,8 @@ class Circuit:
                 measurement.collapse = True
                 self.has_collapse = True
                 self.measurements.remove(measurement)
+        if isinstance(gate, gates.UnitaryChannel):
+            self.has_unitary_channel = True
         if isinstance(gate, gates.ParametrizedGate):
             self.parametrized_gates.append(gate)
             if gate.trainable:

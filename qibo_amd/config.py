@@ -17,6 +17,9 @@ SHOT_BATCH_SIZE = 2**18
 # Largest state vector allowed; -1 disables the check (qibo config.py:22-28).
 MAX_QUBITS = int(os.environ.get("QIBO_MAX_QUBITS", "-1"))
 
+# Largest density matrix allowed; -1 disables the check (qibo config.py:26-28).
+MAX_QUBITS_DM = int(os.environ.get("QIBO_MAX_QUBITS_DM", "-1"))
+
 # Tolerance used by unitary checks (qibo config.py:31).
 PRECISION_TOL = 1e-8
 
@@ -62,6 +65,14 @@ def set_max_qubits(n: int) -> None:
     if not isinstance(n, int) or (n < 1 and n != -1):
         raise_error(ValueError, f"max qubits must be a positive integer or -1, got {n}.")
     MAX_QUBITS = n
+
+
+def set_max_qubits_dm(n: int) -> None:
+    """Set the largest allowed density matrix (-1 disables the check)."""
+    global MAX_QUBITS_DM
+    if not isinstance(n, int) or (n < 1 and n != -1):
+        raise_error(ValueError, f"max qubits must be a positive integer or -1, got {n}.")
+    MAX_QUBITS_DM = n
 
 
 def set_hbm_budget(nbytes) -> None:

@@ -865,3 +865,15 @@ CCX = TOFFOLI
 # special and measurement gates live in their own modules; re-exported here
 from .special import CallbackGate, FusedGate  # noqa: E402  (re-export)
 from .measurements_gate import M  # noqa: E402  (re-export)
+from .channels import (  # noqa: E402  (re-export)
+    AmplitudeDampingChannel,
+    Channel,
+    DepolarizingChannel,
+    KrausChannel,
+    PauliNoiseChannel,
+    PhaseDampingChannel,
+    ReadoutErrorChannel,
+    ResetChannel,
+    ThermalRelaxationChannel,
+    UnitaryChannel,
+)

@@ -91,9 +91,10 @@ class M(_g.Gate):
         if not self.collapse:
             return state
         qubits = sorted(self.target_qubits)
-        probs = backend.calculate_probabilities(state, qubits, nqubits)
+        dm = len(state.shape) == 2  # density matrix (measurements.py:205-217)
+        probs = backend.calculate_probabilities(state, qubits, nqubits, density_matrix=dm)
         shot = self.result.add_shot(probs, backend=backend)
-        return backend.collapse_state(state, qubits, shot, nqubits)
+        return backend.collapse_state(state, qubits, shot, nqubits, density_matrix=dm)
 
 
 def _basis_rotation(cls, qubit):

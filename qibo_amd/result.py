@@ -45,6 +45,15 @@ class QuantumState:
     def symbolic(self, decimals: int = 5, cutoff: float = 1e-10, max_terms: int = 20):
         state = self.backend.to_numpy(self._state)
         terms = []
+        if self.density_matrix:  # abstract.py:2394-2407
+            for j, k in zip(*np.nonzero(state)):
+                if abs(state[j, k]) >= cutoff:
+                    x = np.round(state[j, k], decimals)
+                    terms.append(f"{x}|{int(j):0{self.nqubits}b}><{int(k):0{self.nqubits}b}|")
+                if len(terms) >= max_terms:
+                    terms.append("...")
+                    break
+            return " + ".join(terms)
         for j in np.flatnonzero(state):
             if abs(state[j]) >= cutoff:
                 terms.append(f"{np.round(state[j], decimals)}|{int(j):0{self.nqubits}b}>")
