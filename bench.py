@@ -30,7 +30,11 @@ the metric, each on its own bounded workload:
   U[10, 50], half complex64, half <sum Z> / shots) through the continuous-
   batching server: circuits/s, p50 and p99 latency;
 * "adjoint": adjoint gradients of the TFIM energy for the 20-qubit, 10-layer
-  RY/RZ + CNOT-ladder ansatz over 64 parameter sets, gradients/s.
+  RY/RZ + CNOT-ladder ansatz over 64 parameter sets, gradients/s;
+* "density": the layered circuit with a depolarizing channel on every qubit
+  after every layer on a 14-qubit density matrix (4 GiB, complex128):
+  operations/s, the depolarizing pass in GB/s, and the oracle beside the GPU
+  on the same circuit at 9 qubits.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--nqubits 30] [--no-cpu] [--no-extras]
 """
@@ -296,6 +300,83 @@ def bench_serving(seed, nrequests=48, qps=16.0):
     return out
 
 
+def _noisy_layered(n, nlayers, seed, lam=0.01):
+    """The layered RY/RZ/CZ circuit on a density matrix with a one-qubit
+    DepolarizingChannel on every qubit after every layer."""
+    from qibo_amd import Circuit, gates, models
+
+    angles = models.layered_angles(n, nlayers, seed)
+    c = Circuit(n, density_matrix=True)
+    for layer in range(nlayers):
+        for q in range(n):
+            c.add(gates.RY(q, float(angles[layer, 0, q])))
+        for q in range(n):
+            c.add(gates.RZ(q, float(angles[layer, 1, q])))
+        for q in range(layer % 2, n - 1, 2):
+            c.add(gates.CZ(q, q + 1))
+        for q in range(n):
+            c.add(gates.DepolarizingChannel((q,), lam))
+    return c
+
+
+def bench_density(seed, n=14, nlayers=6, n_cpu=9, reps=3):
+    """Noisy layered circuit on an n-qubit density matrix (a 2n-qubit vector:
+    4 GiB at n = 14 in complex128). Unitary gates run as blocked passes of
+    conj pairs, each DepolarizingChannel as one qsim_dm_depolarize pass.
+    The oracle (numpy restatement of the reference's density path) is timed on
+    the same circuit at n_cpu qubits, with the GPU beside it at that size."""
+    import numpy as np
+    import torch
+
+    from oracle import OracleDensity
+    from qibo_amd.backend import MI355XBackend
+
+    backend = MI355XBackend(dtype="complex128")
+    out = {"nqubits": n, "layers": nlayers}
+    for size, key in ((n, "gpu"), (n_cpu, "gpu_small")):
+        c = _noisy_layered(size, nlayers, seed)
+        ngates = sum(1 for g in c.queue if not hasattr(g, "coefficients"))
+        nch = len(c.queue) - ngates
+        backend.execute_circuit(c)  # warm-up
+        best = None
+        for _ in range(reps):
+            c._final_state = None
+            _, dt = _timed(lambda: backend.execute_circuit(c))
+            best = dt if best is None else min(best, dt)
+        out[key] = {"nqubits": size, "gates": ngates, "channels": nch, "s": best,
+                    "operations_per_s": (ngates + nch) / best}
+        del c
+        torch.cuda.empty_cache()
+    # one depolarizing pass over the 4^n elements: read + write of the matrix
+    rho = backend.zero_state(n, density_matrix=True)
+    from qibo_amd import gates as g_
+
+    ch = g_.DepolarizingChannel((n // 2,), 0.01)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    backend.depolarizing_error_density_matrix(ch, rho, n)
+    ev[0].record(stream)
+    for _ in range(10):
+        backend.depolarizing_error_density_matrix(ch, rho, n)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 10
+    nbytes = 2 * (4**n) * 16
+    out["depolarize_pass"] = {"ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}
+    del rho
+    torch.cuda.empty_cache()
+    # the oracle on the small size, same circuit
+    c = _noisy_layered(n_cpu, nlayers, seed)
+    t0 = time.perf_counter()
+    want = OracleDensity().execute_dm(c)
+    cpu_s = time.perf_counter() - t0
+    got = backend.execute_circuit(c).state(numpy=True)
+    out["cpu_oracle_small"] = {"nqubits": n_cpu, "s": cpu_s, "cores": 1,
+                               "max_abs_err_gpu_vs_oracle": float(np.max(np.abs(got - want)))}
+    out["gpu_small_speedup_vs_cpu"] = cpu_s / out["gpu_small"]["s"]
+    return out
+
+
 def bench_adjoint(seed, n=20, nlayers=10, nsets=64):
     import numpy as np
     import torch
@@ -517,7 +598,8 @@ def main():
     }
     if not args.no_extras:
         for key, fn in (("layered28", bench_layered28), ("qft", bench_qft), ("qft15", bench_qft15),
-                        ("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint)):
+                        ("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint),
+                        ("density", bench_density)):
             try:
                 out[key] = fn(args.seed)
             except Exception as exc:  # noqa: BLE001 - an extra must not hide the headline line

@@ -227,6 +227,40 @@ class GatePlanner:
             return plan_matrix(gate.matrix(self), gate.target_qubits, gate.control_qubits)
         return plan_matrix(gate.matrix(self), gate.qubits, ())
 
+    def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:
+        """The 4x4 map of a one-qubit channel on (row bit, column bit) of its
+        qubit, or None when the channel is not of that form.
+
+        A term p U rho U^dagger maps rho[r', c'] to rho[r, c] with weight
+        p U[r, r'] conj(U[c, c']), i.e. kron(U, conj(U)); the channel is
+        (1 - sum p) I + sum_k p_k kron(U_k, conj(U_k)) (apply_channel,
+        abstract.py:2297-2311). That covers Kraus, unitary, Pauli-noise,
+        depolarizing (its Pauli decomposition), reset (its Kraus operators),
+        damping and readout channels. ThermalRelaxationChannel uses the map
+        its apply writes (channels.py:678-699), or the Kraus form on qubit 0
+        where the reference's reset branch puts its Z."""
+        from .channels import ThermalRelaxationChannel
+        from .special import FusedGate
+
+        if len(channel.target_qubits) != 1:
+            return None
+        q = channel.target_qubits[0]
+        if isinstance(channel, ThermalRelaxationChannel):
+            if channel.t_1 < channel.t_2:
+                p0, p1, e = (channel.init_kwargs[k] for k in ("p_0", "p_1", "e_t2"))
+                return np.array([[1 - p1, 0, 0, p0], [0, e, 0, 0], [0, 0, e, 0], [p1, 0, 0, 1 - p0]],
+                                dtype=np.complex128)
+            if q != 0:
+                return None
+        sop = (1 - channel.coefficient_sum) * np.eye(4, dtype=np.complex128)
+        for coeff, gate in zip(channel.coefficients, channel.gates):
+            if gate.is_controlled_by or tuple(gate.target_qubits) != (q,) or gate.control_qubits:
+                return None
+            u = self.matrix_fused(gate) if isinstance(gate, FusedGate) else gate.matrix(self)
+            u = np.asarray(u, dtype=np.complex128)
+            sop = sop + coeff * np.kron(u, np.conj(u))
+        return sop
+
     # -- Pauli strings (host) ---------------------------------------------
     def pauli_tables(self, terms: Sequence[str], term_qubits, nqubits: int):
         """X/Z bit masks and Y counts of Pauli strings over amplitude bits."""
@@ -285,6 +319,10 @@ class MI355XBackend(GatePlanner):
         self.block_stats = collections.deque(maxlen=1 << 16)  # gates per blocked pass (diagnostics)
         # compiled blocked passes of recently executed gate queues, keyed by the
         # gates' kinds, qubits and exact parameter values (see run_queue)
+        # one-qubit channels inside density-matrix circuits run as their 4x4
+        # transfer matrix on (row qubit, column qubit), inside the blocked
+        # passes; False applies them term by term in the reference's order
+        self.dm_superoperators = True
         self.program_cache = True
         self.program_cache_size = 16
         self._programs = collections.OrderedDict()
@@ -962,6 +1000,12 @@ class MI355XBackend(GatePlanner):
             if isinstance(gate, M) and not gate.collapse:
                 gate.result.backend = self
                 continue
+            if isinstance(gate, Channel) and self.dm_superoperators:
+                sop = self.channel_transfer_matrix(gate)
+                if sop is not None:
+                    q = gate.target_qubits[0]
+                    ops.append(plan_matrix(sop, (q, q + nqubits), ()))
+                    continue
             if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
                                                   and not isinstance(gate, FusedGate)):
                 self._run_ops(ops, state.view(-1), 2 * nqubits)

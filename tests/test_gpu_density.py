@@ -5,7 +5,8 @@ path (oracle/density.py, pinned in test_oracle_density.py):
 * noisy circuits (every channel class, controlled gates, fused gates) from
   |0><0| and from a mixed initial state, complex128 to 1e-12 and complex64 to
   2e-5, on 3..7 qubits (7 qubits = a 14-qubit vectorized matrix, which runs
-  as blocked passes);
+  as blocked passes), with one-qubit channels as transfer matrices inside the
+  passes and term by term;
 * probabilities of the same matrix: all qubits bit for bit, marginals to 1e-14;
 * Pauli-sum expectations to 1e-12;
 * the collapse of a matrix is the normalised (outcome, outcome) block;
@@ -64,11 +65,18 @@ def _noisy_circuit(n, rng):
     return c
 
 
+@pytest.mark.parametrize("transfer", [True, False])
 @pytest.mark.parametrize("n", [3, 5, 6, 7])
-def test_noisy_circuit_matches_oracle(backend, n, rng):
+def test_noisy_circuit_matches_oracle(backend, n, transfer, rng):
+    """One-qubit channels as 4x4 transfer matrices inside the blocked passes
+    (the default) and term by term in the reference's order."""
     c = _noisy_circuit(n, rng)
     want = OracleDensity().execute_dm(c)
-    got = _np(backend.execute_circuit(c).state())
+    backend.dm_superoperators = transfer
+    try:
+        got = _np(backend.execute_circuit(c).state())
+    finally:
+        backend.dm_superoperators = True
     assert got.shape == (2**n, 2**n)
     assert np.max(np.abs(got - want)) < 1e-12
 

@@ -181,3 +181,31 @@ def test_channels_preserve_trace_and_hermiticity(k, rng):
     assert abs(np.trace(out) - 1) < 1e-14
     assert np.max(np.abs(out - out.conj().T)) < 1e-15
     assert np.min(np.linalg.eigvalsh(out)) > -1e-14
+
+
+@pytest.mark.parametrize("k", range(10))
+def test_channel_transfer_matrices(k, rng):
+    """The 4x4 maps that density-matrix circuits run for one-qubit channels
+    (GatePlanner.channel_transfer_matrix) equal the oracle's channel
+    application: the map applied by the state-vector oracle to the 2n-qubit
+    vectorized matrix on (row qubit q, column qubit q)."""
+    from oracle.density import _Matrix
+    from qibo_amd.backend import GatePlanner
+
+    n = 3
+    ch = _channels(n)[k]
+    sop = GatePlanner().channel_transfer_matrix(ch)
+    if len(ch.target_qubits) != 1:
+        assert sop is None
+        return
+    q = ch.target_qubits[0]
+    rho = _mixed_state(n, rng)
+    od = OracleDensity()
+    got = od.apply_gate(_Matrix(sop, (q, q + n)), rho.ravel().copy(), 2 * n).reshape(rho.shape)
+    assert np.max(np.abs(got - od.apply_dm(ch, rho.copy(), n))) < 1e-15
+
+
+def test_thermal_reset_branch_off_qubit_zero_has_no_transfer_matrix():
+    from qibo_amd.backend import GatePlanner
+
+    assert GatePlanner().channel_transfer_matrix(gates.ThermalRelaxationChannel(1, [1.0, 0.5, 0.4])) is None
