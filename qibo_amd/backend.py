@@ -62,6 +62,20 @@ class _Op:
         self.data = data          # numpy matrix / diagonal, or None
 
 
+def _embed(u: np.ndarray, qubits: Tuple[int, ...], targets: Tuple[int, ...]) -> np.ndarray:
+    """``u`` acting on ``qubits`` (first = most significant) as a matrix on
+    ``targets`` (a superset, same convention), identity on the others."""
+    if qubits == targets:
+        return u
+    rest = tuple(q for q in targets if q not in qubits)
+    full = np.kron(u, np.eye(2 ** len(rest), dtype=u.dtype))  # on qubits + rest
+    order = qubits + rest
+    k = len(targets)
+    perm = [order.index(q) for q in targets]
+    full = full.reshape((2,) * (2 * k)).transpose(perm + [p + k for p in perm])
+    return full.reshape(2**k, 2**k)
+
+
 def _peel_controls(m: np.ndarray, qubits: Sequence[int]) -> Tuple[np.ndarray, list, list]:
     """Move qubits on which ``m`` is ``[[I, 0], [0, U]]`` into the control list.
 
@@ -228,36 +242,40 @@ class GatePlanner:
         return plan_matrix(gate.matrix(self), gate.qubits, ())
 
     def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:
-        """The 4x4 map of a one-qubit channel on (row bit, column bit) of its
-        qubit, or None when the channel is not of that form.
+        """The map of a channel on one or two qubits T as a 4^k x 4^k matrix
+        on (row qubits T, column qubits T) of the vectorized density matrix,
+        or None when the channel is not of that form.
 
         A term p U rho U^dagger maps rho[r', c'] to rho[r, c] with weight
         p U[r, r'] conj(U[c, c']), i.e. kron(U, conj(U)); the channel is
         (1 - sum p) I + sum_k p_k kron(U_k, conj(U_k)) (apply_channel,
-        abstract.py:2297-2311). That covers Kraus, unitary, Pauli-noise,
-        depolarizing (its Pauli decomposition), reset (its Kraus operators),
-        damping and readout channels. ThermalRelaxationChannel uses the map
-        its apply writes (channels.py:678-699), or the Kraus form on qubit 0
-        where the reference's reset branch puts its Z."""
+        abstract.py:2297-2311), each U_k embedded on T. That covers Kraus,
+        unitary, Pauli-noise, depolarizing (its Pauli decomposition), reset
+        (its Kraus operators), damping and readout channels.
+        ThermalRelaxationChannel uses the map its apply writes
+        (channels.py:678-699), or the Kraus form on qubit 0 where the
+        reference's reset branch puts its Z."""
         from .channels import ThermalRelaxationChannel
         from .special import FusedGate
 
-        if len(channel.target_qubits) != 1:
+        targets = tuple(channel.target_qubits)
+        k = len(targets)
+        if k not in (1, 2) or len(set(targets)) != k:
             return None
-        q = channel.target_qubits[0]
         if isinstance(channel, ThermalRelaxationChannel):
             if channel.t_1 < channel.t_2:
-                p0, p1, e = (channel.init_kwargs[k] for k in ("p_0", "p_1", "e_t2"))
+                p0, p1, e = (channel.init_kwargs[key] for key in ("p_0", "p_1", "e_t2"))
                 return np.array([[1 - p1, 0, 0, p0], [0, e, 0, 0], [0, 0, e, 0], [p1, 0, 0, 1 - p0]],
                                 dtype=np.complex128)
-            if q != 0:
+            if targets[0] != 0:
                 return None
-        sop = (1 - channel.coefficient_sum) * np.eye(4, dtype=np.complex128)
+        dim = 2**k
+        sop = (1 - channel.coefficient_sum) * np.eye(dim * dim, dtype=np.complex128)
         for coeff, gate in zip(channel.coefficients, channel.gates):
-            if gate.is_controlled_by or tuple(gate.target_qubits) != (q,) or gate.control_qubits:
+            if gate.is_controlled_by or gate.control_qubits or not set(gate.target_qubits) <= set(targets):
                 return None
             u = self.matrix_fused(gate) if isinstance(gate, FusedGate) else gate.matrix(self)
-            u = np.asarray(u, dtype=np.complex128)
+            u = _embed(np.asarray(u, dtype=np.complex128), tuple(gate.target_qubits), targets)
             sop = sop + coeff * np.kron(u, np.conj(u))
         return sop
 
@@ -1003,8 +1021,8 @@ class MI355XBackend(GatePlanner):
             if isinstance(gate, Channel) and self.dm_superoperators:
                 sop = self.channel_transfer_matrix(gate)
                 if sop is not None:
-                    q = gate.target_qubits[0]
-                    ops.append(plan_matrix(sop, (q, q + nqubits), ()))
+                    t = tuple(gate.target_qubits)
+                    ops.append(plan_matrix(sop, t + tuple(q + nqubits for q in t), ()))
                     continue
             if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
                                                   and not isinstance(gate, FusedGate)):

@@ -62,6 +62,8 @@ def _noisy_circuit(n, rng):
     for q in range(n):
         c.add(gates.RZ(q, rng.uniform(0, 6)))
     c.add(gates.DepolarizingChannel((n - 1,), 0.1))
+    c.add(gates.PauliNoiseChannel((n - 1, 1), [("XZ", 0.1), ("YY", 0.05), ("IX", 0.2)]))
+    c.add(gates.DepolarizingChannel((0, 1, 2), 0.05))
     return c
 
 

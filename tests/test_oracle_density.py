@@ -195,17 +195,35 @@ def test_channel_transfer_matrices(k, rng):
     n = 3
     ch = _channels(n)[k]
     sop = GatePlanner().channel_transfer_matrix(ch)
-    if len(ch.target_qubits) != 1:
+    t = tuple(ch.target_qubits)
+    if len(t) > 2:  # three-qubit channels run term by term
         assert sop is None
         return
-    q = ch.target_qubits[0]
+    assert sop.shape == (4 ** len(t), 4 ** len(t))
     rho = _mixed_state(n, rng)
     od = OracleDensity()
-    got = od.apply_gate(_Matrix(sop, (q, q + n)), rho.ravel().copy(), 2 * n).reshape(rho.shape)
+    got = od.apply_gate(_Matrix(sop, t + tuple(q + n for q in t)), rho.ravel().copy(), 2 * n).reshape(rho.shape)
     assert np.max(np.abs(got - od.apply_dm(ch, rho.copy(), n))) < 1e-15
 
 
-def test_thermal_reset_branch_off_qubit_zero_has_no_transfer_matrix():
+def test_channels_without_transfer_matrix():
     from qibo_amd.backend import GatePlanner
 
-    assert GatePlanner().channel_transfer_matrix(gates.ThermalRelaxationChannel(1, [1.0, 0.5, 0.4])) is None
+    gp = GatePlanner()
+    assert gp.channel_transfer_matrix(gates.ThermalRelaxationChannel(1, [1.0, 0.5, 0.4])) is None
+    assert gp.channel_transfer_matrix(gates.DepolarizingChannel((0, 1, 2), 0.1)) is None
+
+
+@pytest.mark.parametrize("targets", [(2, 0), (1, 3)])
+def test_two_qubit_pauli_channel_transfer_matrix(targets, rng):
+    from oracle.density import _Matrix
+    from qibo_amd.backend import GatePlanner
+
+    n = 4
+    ch = gates.PauliNoiseChannel(targets, [("XZ", 0.1), ("YY", 0.05), ("IX", 0.2), ("ZI", 0.15)])
+    sop = GatePlanner().channel_transfer_matrix(ch)
+    rho = _mixed_state(n, rng)
+    od = OracleDensity()
+    t = tuple(ch.target_qubits)
+    got = od.apply_gate(_Matrix(sop, t + tuple(q + n for q in t)), rho.ravel().copy(), 2 * n).reshape(rho.shape)
+    assert np.max(np.abs(got - od.apply_dm(ch, rho.copy(), n))) < 1e-15
