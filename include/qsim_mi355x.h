@@ -175,7 +175,12 @@ int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nsl
  * (abstract.py:2734-2758, 3371-3381). Output dtype is F64 for C128 states and
  * F32 for C64 states (the state's real dtype, as `rtype` there). When all
  * qubits are measured the values are bitwise those of numpy's
- * `np.abs(state)**2`; marginals use a fixed-order FP64 reduction.
+ * `np.abs(state)**2`; marginals are bitwise numpy's
+ * `np.sum(probs, axis=unmeasured)` too (its order: pairwise sums of the
+ * trailing unmeasured runs in 8192-element buffers, then C order; elementwise
+ * C order when the last qubit is measured), in the probabilities' dtype,
+ * except when an output would need more than 2^24 serial additions, where a
+ * parallel fixed-order FP64 reduction is used.
  * `workspace` must hold qsim_probabilities_workspace(nqubits, nmeasured) bytes. */
 size_t qsim_probabilities_workspace(int nqubits, int nmeasured);
 int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qubits, int nmeasured,

@@ -31,6 +31,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "qsim_common.h"
@@ -170,6 +171,169 @@ __global__ __launch_bounds__(kT) void k_marg_final(const double* __restrict__ pa
   out[o] = R(acc);
 }
 
+// ---------------------------------------------------------------------------
+// marginals in numpy's order: np.sum(probs, axis=unmeasured) on the (2,)*n
+// array (calculate_probabilities, abstract.py:2739-2741)
+// ---------------------------------------------------------------------------
+// numpy reduces in C order after merging adjacent axes of the same kind
+// (unmeasured / measured), in the probability dtype, from 0:
+// * last qubit unmeasured: the innermost merged axis is a run of L = 2^t
+//   unmeasured elements; each output adds, in C order of the outer unmeasured
+//   qubits, the pairwise sums of its runs taken in buffers of at most 8192
+//   elements (the pairwise rule of np.sum, see k_np_buffer_sums);
+// * last qubit measured: each output adds its elements one by one in C order
+//   of the unmeasured qubits.
+// A serial chain of more than kMaxSerial additions per output would take
+// seconds, so such marginals keep the parallel fixed-order FP64 sum.
+constexpr uint64_t kMargBuf = 8192;
+constexpr uint64_t kMaxSerial = uint64_t(1) << 24;
+
+template <typename R>
+__device__ R np_leaf_sum_psi(const cplx<R>* a, int n) {
+  if (n < 8) {
+    R res = R(0);
+    for (int i = 0; i < n; ++i) res = res + np_abs2(a[i]);
+    return res;
+  }
+  R r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = np_abs2(a[j]);
+  int i = 8;
+  for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = r[j] + np_abs2(a[i + j]);
+  }
+  R res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res = res + np_abs2(a[i]);
+  return res;
+}
+
+// Pairwise sums of consecutive runs of lc probabilities (lc a power of two,
+// 128 <= lc <= 8192): one 64-thread block per run, 128-element leaves
+// combined as a balanced tree (the pairwise split of a power of two).
+template <typename R>
+__global__ __launch_bounds__(64) void k_run_sums(const cplx<R>* __restrict__ psi, uint64_t lc,
+                                                 R* __restrict__ runsum) {
+  __shared__ R leaf[kMargBuf / 128];
+  const uint64_t run = block_id();
+  const cplx<R>* a = psi + run * lc;
+  const int nleaf = int(lc / 128);
+  if (int(threadIdx.x) < nleaf) leaf[threadIdx.x] = np_leaf_sum_psi(a + uint64_t(threadIdx.x) * 128, 128);
+  __syncthreads();
+  for (int w = nleaf; w > 1; w >>= 1) {
+    R v = R(0);
+    if (int(threadIdx.x) < w / 2) v = leaf[2 * threadIdx.x] + leaf[2 * threadIdx.x + 1];
+    __syncthreads();
+    if (int(threadIdx.x) < w / 2) leaf[threadIdx.x] = v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) runsum[run] = leaf[0];
+}
+
+struct NpMargArgs {
+  uint64_t nout;   // 2^m outputs, requested order
+  uint64_t nred;   // outer unmeasured combinations (runs case) or all unmeasured combinations
+  uint64_t nsub;   // run sums per outer index (runs case) / run length (short runs)
+  int m;
+  uint8_t obit[kMaxFixed];  // index bit of requested output bit k (k = 0 most significant)
+  uint8_t mpos[kMaxFixed];  // the same bits, ascending
+};
+
+__device__ __forceinline__ uint64_t np_out_bits(uint64_t o, const NpMargArgs& a) {
+  uint64_t ob = 0;
+  for (int k = 0; k < a.m; ++k) ob |= ((o >> (a.m - 1 - k)) & 1) << a.obit[k];
+  return ob;
+}
+
+// Runs of >= 128 elements: add the run sums of each output in C order.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_np_runs(const R* __restrict__ runsum, R* __restrict__ out,
+                                                     const NpMargArgs a) {
+  const uint64_t o = block_id() * kT + threadIdx.x;
+  if (o >= a.nout) return;
+  const uint64_t ob = np_out_bits(o, a);
+  R acc = R(0);
+  for (uint64_t r = 0; r < a.nred; ++r) {
+    const R* sub = runsum + (insert_zeros(r, a.mpos, a.m) | ob) * a.nsub;
+    for (uint64_t c = 0; c < a.nsub; ++c) acc = acc + sub[c];
+  }
+  out[o] = acc;
+}
+
+// Runs of fewer than 128 elements (nsub = run length, index = outer << t).
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_np_short(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                      const NpMargArgs a) {
+  const uint64_t o = block_id() * kT + threadIdx.x;
+  if (o >= a.nout) return;
+  const uint64_t ob = np_out_bits(o, a);
+  R acc = R(0);
+  for (uint64_t r = 0; r < a.nred; ++r)
+    acc = acc + np_leaf_sum_psi(psi + (insert_zeros(r, a.mpos, a.m) | ob) * a.nsub, int(a.nsub));
+  out[o] = acc;
+}
+
+// Last qubit measured: every element in C order of the unmeasured qubits.
+template <typename R>
+__global__ __launch_bounds__(kT) void k_marg_np_seq(const cplx<R>* __restrict__ psi, R* __restrict__ out,
+                                                    const NpMargArgs a) {
+  const uint64_t o = block_id() * kT + threadIdx.x;
+  if (o >= a.nout) return;
+  const uint64_t ob = np_out_bits(o, a);
+  R acc = R(0);
+  for (uint64_t v = 0; v < a.nred; ++v) acc = acc + np_abs2(psi[insert_zeros(v, a.mpos, a.m) | ob]);
+  out[o] = acc;
+}
+
+// Launches numpy's order for a marginal; returns 1 if the serial chain is too
+// long (the caller then sums in parallel), 0 on launch, <0 on error.
+template <typename R>
+static int marg_numpy_order(const cplx<R>* psi, int n, const int* qubits, int m, R* out, void* ws, size_t ws_bytes,
+                            hipStream_t s) {
+  bool measured[64] = {false};
+  for (int k = 0; k < m; ++k) measured[qubits[k]] = true;
+  NpMargArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.m = m;
+  a.nout = uint64_t(1) << m;
+  if (measured[n - 1]) {
+    a.nred = uint64_t(1) << (n - m);
+    if (a.nred > kMaxSerial) return 1;
+    for (int k = 0; k < m; ++k) a.obit[k] = uint8_t(n - 1 - qubits[k]);
+    sorted_positions(n, qubits, m, nullptr, 0, a.mpos);
+    hipLaunchKernelGGL(k_marg_np_seq<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    QSIM_CHECK_LAUNCH();
+    return 0;
+  }
+  int t = 0;
+  while (t < n && !measured[n - 1 - t]) ++t;  // trailing unmeasured run (qubits n-t .. n-1)
+  const uint64_t L = uint64_t(1) << t;
+  const int outer_bits = n - t;
+  a.nred = uint64_t(1) << (outer_bits - m);
+  // positions in the outer index (full index >> t)
+  for (int k = 0; k < m; ++k) a.obit[k] = uint8_t(n - 1 - qubits[k] - t);
+  for (int k = 0; k < m; ++k) a.mpos[k] = a.obit[k];
+  std::sort(a.mpos, a.mpos + m);
+  if (L < 128) {
+    if (a.nred * L > kMaxSerial) return 1;
+    a.nsub = L;
+    hipLaunchKernelGGL(k_marg_np_short<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, psi, out, a);
+    QSIM_CHECK_LAUNCH();
+    return 0;
+  }
+  const uint64_t lc = L < kMargBuf ? L : kMargBuf;
+  a.nsub = L / lc;
+  if (a.nred * a.nsub > kMaxSerial) return 1;
+  const uint64_t nruns = (uint64_t(1) << n) / lc;
+  QSIM_CHECK_ARG(ws != nullptr && ws_bytes >= nruns * sizeof(R), "qsim_probabilities: workspace too small");
+  R* runsum = reinterpret_cast<R*>(ws);
+  hipLaunchKernelGGL(k_run_sums<R>, launch_grid(nruns), dim3(64), 0, s, psi, lc, runsum);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_marg_np_runs<R>, launch_grid(ceil_div_u(a.nout, kT)), dim3(kT), 0, s, runsum, out, a);
+  QSIM_CHECK_LAUNCH();
+  return 0;
+}
+
 static void marg_plan(int n, int m, uint64_t* nchunks, bool* per_thread) {
   const uint64_t nout = uint64_t(1) << m, nin = uint64_t(1) << (n - m);
   if (nin <= 64) {
@@ -206,6 +370,8 @@ static int probabilities_impl(const void* state, int n, const int* qubits, int m
     QSIM_CHECK_LAUNCH();
     return QSIM_OK;
   }
+  const int rc = marg_numpy_order<R>(psi, n, qubits, m, out, ws, ws_bytes, s);
+  if (rc <= 0) return rc;
   MargArgs a;
   std::memset(&a, 0, sizeof(a));
   a.m = m;
@@ -896,8 +1062,10 @@ size_t qsim_probabilities_workspace(int nqubits, int nmeasured) {
   uint64_t nchunks;
   bool per_thread;
   marg_plan(nqubits, nmeasured, &nchunks, &per_thread);
-  if (per_thread || nchunks == 1) return 0;
-  return size_t((uint64_t(1) << nmeasured) * nchunks * sizeof(double));
+  const size_t fixed_order = (per_thread || nchunks == 1) ? 0 : size_t((uint64_t(1) << nmeasured) * nchunks * sizeof(double));
+  // run sums of numpy's order: at most 2^n / 128 values (double for complex128)
+  const size_t runs = nqubits >= 7 ? size_t(uint64_t(1) << (nqubits - 7)) * sizeof(double) : 0;
+  return fixed_order > runs ? fixed_order : runs;
 }
 
 int qsim_probabilities(const void* state, int nqubits, int dtype, const int* qubits, int nmeasured,
