@@ -514,7 +514,8 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
     INNER record in program order). Every set's program has the same records
     (same layout and headers), only the entries differ. One record per
     operation: DENSE1_REAL (when the 2x2 is real for every set), DENSE1,
-    DIAG, LANE_PHASE, INNER1, INNER_DIAG and MAP. With ``entry_map`` (a list)
+    DIAG, NEG (a -1 on a controlled subspace for every set), LANE_PHASE,
+    INNER1, INNER_DIAG and MAP. With ``entry_map`` (a list)
     it receives (byte offset of the entries in a program, index of the
     operation in ``ops``, entry count) for every record that has entries:
     the entries are the operation's data row, flattened."""
@@ -583,7 +584,9 @@ def compile_pass_batched(tile_bits: Sequence[int], ops: Sequence, nqubits: int, 
                 continue
             cm, rreq = split_cond(op, cur)
             enc = [(REG_FLAG | rb.index(b)) if b in rb else b for b in tb]
-            if op.kind == "diag":
+            if op.kind == "diag" and nt == 0 and np.all(d == -1):
+                emit(OP_NEG, 0, cm, 0, (), None, rreq)  # CZ-type: a sign flip, no multiply
+            elif op.kind == "diag":
                 emit(OP_DIAG, 0, cm, nt, enc, d, rreq)
             else:
                 enc = enc + [0] * (6 - nt) + [op.slot & 255, op.slot >> 8]

@@ -954,6 +954,20 @@ __global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const 
         diag_regs(v, h, m, lane_idx, lane_ok);
         diag_regs(w, h, m, lane_idx, lane_ok);
         break;
+      case kOpNeg: {
+        // -1 on the controlled subspace of both states: sign-bit flips
+        const uint32_t rreq = h.rreq;
+        const uint32_t sign = lane_ok ? 0x80000000u : 0u;
+#pragma clang loop unroll(full)
+        for (int r = 0; r < kBReg; ++r) {
+          if (!reg_ok(r, rreq)) continue;
+          flip_sign(v[r].re, sign);
+          flip_sign(v[r].im, sign);
+          flip_sign(w[r].re, sign);
+          flip_sign(w[r].im, sign);
+        }
+        break;
+      }
       case kOpLanePhase: {
         uint32_t rm[4];
         const uint32_t di = diag_lane_part(h, lane_idx, rm);

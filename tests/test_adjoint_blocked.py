@@ -32,6 +32,10 @@ def _apply(states, n, tile_bits, recs, grads_out, slots, b):
         if op == blocking.OP_MAP:
             cur = arg
             continue
+        if op == blocking.OP_NEG:
+            psi[:] = np.where(active, -psi, psi)
+            lam[:] = np.where(active, -lam, lam)
+            continue
         if op in (blocking.OP_DIAG, blocking.OP_LANE_PHASE, blocking.OP_INNER_DIAG):
             di = np.zeros_like(idx)
             for tb in bits[:nt]:
@@ -138,6 +142,8 @@ def _ansatz(n, layers):
         c.add(gates.CU1(n - 2, 1, 0.4))
         c.add(gates.RZZ(2, 5, 0.5))
         c.add(gates.RX(3, 0.6).controlled_by(7))
+        c.add(gates.CZ(4, 9))
+        c.add(gates.CZ(0, 1))
     return c
 
 
@@ -165,6 +171,16 @@ def test_blocked_adjoint_matches_central_differences(n):
             tm[k] -= eps
             fd = (energy(tp) - energy(tm)) / (2 * eps)
             assert abs(grads[b, k] - fd) < 1e-7, (b, k)
+
+
+def test_cz_type_gates_become_sign_flips():
+    n = 13
+    c = _ansatz(n, 1)
+    tmpl = SweepTemplate(GatePlanner(), c, np.complex128)
+    kinds = []
+    for p in tmpl.fwd + tmpl.rev:
+        kinds += [r[0] for r in decode(p.row.tobytes(), 0, p.count, 16)]
+    assert blocking.OP_NEG in kinds
 
 
 def test_layered_circuit_compiles_to_few_passes():

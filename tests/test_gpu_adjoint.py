@@ -119,6 +119,9 @@ def test_blocked_batched_matches_per_set_sweep(backend, n, batch):
     c.add(gates.CU1(n - 2, 1, 0.4))
     c.add(gates.RZZ(2, 5, 0.5))
     c.add(gates.RX(3, 0.6).controlled_by(7))
+    c.add(gates.CZ(4, 9))  # a sign flip record on both states
+    c.add(gates.RY(9, 0.2))
+    c.add(gates.CZ(0, n - 1))
     ham = hamiltonians.TFIM(n, h=0.7, closed_boundary=False, backend=backend)
     params = rng.uniform(0, 2 * np.pi, (batch, len(c.get_parameters())))
     e, g = blocked_expectation_and_gradient(backend, c, ham, params)
