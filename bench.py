@@ -37,6 +37,7 @@ the metric, each on its own bounded workload:
   on the same circuit at 9 qubits.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--nqubits 30] [--no-cpu] [--no-extras]
+                    [--extras NAME,NAME,...]
 """
 
 import argparse
@@ -427,6 +428,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default=None, help="comma-separated subset of the extras to run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -597,9 +599,12 @@ def main():
         },
     }
     if not args.no_extras:
+        chosen = None if args.extras is None else set(args.extras.split(","))
         for key, fn in (("layered28", bench_layered28), ("qft", bench_qft), ("qft15", bench_qft15),
                         ("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint),
                         ("density", bench_density)):
+            if chosen is not None and key not in chosen:
+                continue
             try:
                 out[key] = fn(args.seed)
             except Exception as exc:  # noqa: BLE001 - an extra must not hide the headline line
