@@ -900,15 +900,23 @@ __global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const 
   P.re = R(1);
   P.im = R(0);
   bool pdirty = false;
-  const uint64_t nbytes = a.nbytes;
-  uint64_t off = 0;
+  // 32-bit offsets and a header read ahead, as in k_blocked: the record
+  // decode stays on the scalar unit and overlaps the previous record
+  const uint32_t nbytes = __builtin_amdgcn_readfirstlane(uint32_t(a.nbytes));
+  const uint64_t tb_lo = __builtin_amdgcn_readfirstlane(uint32_t(tile_base));
+  const uint64_t tb_hi = __builtin_amdgcn_readfirstlane(uint32_t(tile_base >> 32));
+  const uint64_t tile_base_s = tb_lo | (tb_hi << 32);
+  uint32_t off = 0;
   int slot = 0;
-  for (int k = 0; k < a.ninstr && off + sizeof(BlkHdr) <= nbytes; ++k) {
-    const Hdr h = load_hdr(prog + off);
+  Hdr h = {};
+  if (a.ninstr > 0) h = load_hdr(prog);
+  for (int k = 0; k < a.ninstr && off + uint32_t(sizeof(BlkHdr)) <= nbytes; ++k) {
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
-    off += sizeof(BlkHdr) + ((uint64_t(h.nent) * sizeof(cplx<R>) + 15) & ~uint64_t(15));
-    const bool tile_ok = (tile_base & h.tmask) == h.tmask;
-    const bool lane_ok = tile_ok && (lane_idx & h.cmask) == h.cmask;
+    const uint32_t nent = h.nent < kMaxEntries ? h.nent : kMaxEntries;
+    off += uint32_t(sizeof(BlkHdr)) + ((nent * uint32_t(sizeof(cplx<R>)) + 15u) & ~15u);
+    const uint32_t next = off < nbytes ? off : nbytes;
+    const Hdr hn = load_hdr(prog + next);
+    const bool tile_ok = (tile_base_s & h.tmask) == h.tmask;
     if (h.op == kOpInner1 || h.op == kOpInnerDiag) {
       // every INNER record writes its partial, zero where the condition fails
       double acc = 0.0;
@@ -925,15 +933,20 @@ __global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const 
         }
         // a pending lane factor P (a product of diagonal unitaries, applied
         // to both states) drops out: conj(P w) M (P v) = |P|^2 conj(w) M v
+        acc = ((lane_idx & h.cmask) == h.cmask) ? acc : 0.0;
       }
-      acc = lane_ok ? acc : 0.0;
       acc = wave_sum(acc);
       if (lane == 0 && slot < a.nslots)
         part[((bz * uint64_t(a.nslots) + uint64_t(slot)) * a.ntiles + tile) * 4 + wave] = acc;
       ++slot;
+      h = hn;
       continue;
     }
-    if (!tile_ok) continue;
+    if (!tile_ok) {
+      h = hn;
+      continue;
+    }
+    const bool lane_ok = (lane_idx & h.cmask) == h.cmask;
     switch (h.op) {
       case kOpDense1Real:
       case kOpDense1: {
@@ -987,6 +1000,7 @@ __global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const 
       default:
         break;
     }
+    h = hn;
   }
   flush_lane_phase2(v, w, P, pdirty);
   if (map != 0) {
