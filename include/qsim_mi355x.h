@@ -304,6 +304,19 @@ int qsim_dm_expval_pauli(const void* rho, int nqubits, int dtype, int nterms, co
                          const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- host planner ----------------------------------------------------------
+ * Pass selection of the blocked execution of the circuit loop
+ * (abstract.py:3321-3322; qibo_amd.blocking.partition), host code: among the
+ * nitems pending gates (bit masks of their non-diagonal targets and of all
+ * their bits, and whether a pass can hold them), the greedy scan from the
+ * seed of the low_bits lowest bits and (unless `ordered`) from every seed of
+ * those plus tile_bits - low_bits consecutive bits; the first seed taking the
+ * most gates wins. Writes taken[i] = 1 for the gates it takes and the
+ * resulting tile bits; returns the number taken (>= 0) or a negative error.
+ * No device work. */
+int qsim_plan_pass(int nitems, const uint64_t* nd_masks, const uint64_t* all_masks, const uint8_t* blockable,
+                   int nqubits, int low_bits, int tile_bits, int ordered, uint8_t* taken, uint64_t* tile_mask);
+
 #ifdef __cplusplus
 }
 #endif

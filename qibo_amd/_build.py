@@ -29,6 +29,7 @@ SOURCES = {
     "expval.hip": [],
     "blocked.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"],
     "density.hip": ["-ffp-contract=off"],
+    "planner.hip": [],
 }
 
 COMMON = [

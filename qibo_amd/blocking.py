@@ -60,6 +60,7 @@ lane, ``rreq`` the register bits (as a 4-bit mask over register positions);
 DENSE_LDS keeps all condition bits in ``cmask``.
 """
 
+import ctypes
 import struct
 from typing import Iterable, Iterator, Sequence, Tuple
 
@@ -134,8 +135,30 @@ def _scan(items, tile, ordered):
     return chosen, deferred, {b for b in range(tmask.bit_length()) if (tmask >> b) & 1}
 
 
+def _scan_native(items, nqubits, low_bits, ordered):
+    """_scan from every seed of ``partition`` in one native call
+    (qsim_plan_pass): the same choice as the Python loop, item for item."""
+    from . import _lib
+
+    lib = _lib.load()
+    nitems = len(items)
+    nd = np.fromiter((it.nd_mask for it in items), dtype=np.uint64, count=nitems)
+    al = np.fromiter((it.all_mask for it in items), dtype=np.uint64, count=nitems)
+    ok = np.fromiter((it.ok for it in items), dtype=np.uint8, count=nitems)
+    taken = np.empty(nitems, dtype=np.uint8)
+    tile = ctypes.c_uint64(0)
+    rc = lib.qsim_plan_pass(nitems, nd.ctypes.data, al.ctypes.data, ok.ctypes.data, nqubits, low_bits, TILE_BITS,
+                            1 if ordered else 0, taken.ctypes.data, ctypes.byref(tile))
+    if rc < 0:
+        _lib.check(rc, lib)
+    chosen = [it for it, t in zip(items, taken.tolist()) if t]
+    deferred = [it for it, t in zip(items, taken.tolist()) if not t]
+    tm = tile.value
+    return chosen, deferred, {b for b in range(tm.bit_length()) if (tm >> b) & 1}
+
+
 def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
-              window: int = WINDOW, singles: bool = True) -> Iterator[Tuple[str, object]]:
+              window: int = WINDOW, singles: bool = True, native: bool = True) -> Iterator[Tuple[str, object]]:
     """Split ``ops`` (planned gates in circuit order, any iterable) into passes.
 
     Yields ("single", op) and ("block", (tile_bits, [ops])) items whose
@@ -145,6 +168,8 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
     first ``window`` pending gates, so ``ops`` is consumed lazily: gates
     further on are planned while earlier passes run. With ``ordered`` a pass
     ends at the first gate that does not fit (no gate moves ahead).
+    ``native`` runs the pass selection in the library (qsim_plan_pass);
+    False runs the same rule in Python (``_scan``).
     """
     source = iter(ops)
     items = []
@@ -168,8 +193,11 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
             del items[0]
             fill()
             continue
-        chosen, deferred, tile = _scan(items, set(range(low_bits)), ordered)
-        if not ordered:
+        if native:
+            chosen, deferred, tile = _scan_native(items, nqubits, low_bits, ordered)
+        else:
+            chosen, deferred, tile = _scan(items, set(range(low_bits)), ordered)
+        if not ordered and not native:
             # also try tiles of low bits + a window of consecutive bits, and
             # keep the tile that takes the most gates: on circuits of
             # neighbouring gates a contiguous block of qubits can advance
