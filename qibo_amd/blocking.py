@@ -61,6 +61,7 @@ DENSE_LDS keeps all condition bits in ``cmask``.
 """
 
 import ctypes
+import functools
 import struct
 from typing import Iterable, Iterator, Sequence, Tuple
 
@@ -257,17 +258,17 @@ def _pack(buf: bytearray, op: int, arg: int, cmask: int, nt: int, bits: Sequence
           data, cdtype, rreq: int = 0):
     """Append one variable-length record: header (u8 op, u8 arg, u8 nt, u8 rreq,
     u32 nent, u64 cmask, u8 bits[8], u64 0) + nent complex entries."""
-    mat = np.zeros(0, dtype=cdtype) if data is None else np.asarray(data, dtype=cdtype).ravel()
-    hdr = np.zeros(1, dtype=_HDR)
-    hdr["op"], hdr["arg"], hdr["nt"], hdr["rreq"], hdr["nent"], hdr["cmask"] = op, arg, nt, rreq, mat.size, cmask
-    b = np.zeros(8, dtype=np.uint8)
-    b[: len(bits)] = bits
-    hdr["bits"] = b
-    size = record_bytes(mat.size, np.dtype(cdtype).itemsize)
-    rec = bytearray(size)
-    rec[:HEADER_BYTES] = hdr.tobytes()
-    rec[HEADER_BYTES: HEADER_BYTES + mat.nbytes] = mat.tobytes()
-    buf.extend(rec)
+    if data is None:
+        nent, raw = 0, b""
+    else:
+        mat = np.asarray(data, dtype=cdtype).ravel()
+        nent, raw = mat.size, mat.tobytes()
+    b = list(bits) + [0] * (8 - len(bits))
+    buf += _HDR_STRUCT.pack(op, arg, nt, rreq, nent, cmask, *b, 0)
+    buf += raw
+    pad = (-len(raw)) % 16
+    if pad:
+        buf += bytes(pad)
 
 
 def _dependencies(items, ordered):
@@ -311,6 +312,23 @@ def _dependencies(items, ordered):
 
 
 _RVALS = np.array([[(r >> j) & 1 for r in range(16)] for j in range(4)], dtype=np.int64)  # [j, r]
+_ONES16 = np.ones(16, dtype=np.complex128)
+
+
+@functools.lru_cache(maxsize=None)
+def _reg_phase_pattern(tregs: Tuple[int, ...], cregs: Tuple[int, ...]):
+    """For a diagonal on register bits ``tregs`` (first = most significant
+    entry bit) under register controls ``cregs``: which of the 16 register
+    entries it touches and the diagonal entry each one takes."""
+    ok = np.ones(16, dtype=bool)
+    for c in cregs:
+        ok &= _RVALS[c] == 1
+    di = np.zeros(16, dtype=np.int64)
+    for t in tregs:
+        di = (di << 1) | _RVALS[t]
+    ok.setflags(write=False)
+    di.setflags(write=False)
+    return ok, di
 _RFLIP = [np.arange(16) ^ (1 << j) for j in range(4)]  # register r -> r with bit j flipped
 
 
@@ -441,13 +459,8 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
                 emit(OP_LANE_PHASE, 0, cm, nt, tb, d)
                 return
             if all(b in rb for b in tb + cb):
-                table = np.ones(16, dtype=np.complex128) if pending is None else pending
-                ok = np.ones(16, dtype=bool)
-                for c in cb:
-                    ok &= _RVALS[rb.index(c)] == 1
-                di = np.zeros(16, dtype=np.int64)
-                for b in tb:
-                    di = (di << 1) | _RVALS[rb.index(b)]
+                table = _ONES16 if pending is None else pending
+                ok, di = _reg_phase_pattern(tuple(rb.index(b) for b in tb), tuple(rb.index(c) for c in cb))
                 pending = np.where(ok, table * d[di], table)
                 return
         cm, rreq = split_cond(op, cur)
