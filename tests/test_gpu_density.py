@@ -217,3 +217,23 @@ def test_kraus_channel_rejects_state_vectors(backend):
     c.add(gates.AmplitudeDampingChannel(0, 0.2))
     with pytest.raises(NotImplementedError):
         backend.execute_circuit(c)
+
+
+def test_half_density_matrix_and_direct_channel_calls(backend, rng):
+    """apply_gate_half_density_matrix (U rho) and the backend's channel
+    entry points called directly, against numpy on the same matrix."""
+    n = 4
+    rho = _mixed_state(n, rng)
+    g = gates.U3(2, *rng.uniform(0, 6, 3))
+    u = OracleBackend().matrix(g)
+    full = np.kron(np.kron(np.eye(4), u), np.eye(2))  # qubit 2 of 4
+    got = _np(backend.apply_gate_half_density_matrix(g, backend.cast(rho), n))
+    assert np.max(np.abs(got - full @ rho)) < 1e-14
+    od = OracleDensity()
+    for ch in (gates.DepolarizingChannel((1, 3), 0.3), gates.ResetChannel(0, [0.2, 0.5]),
+               gates.AmplitudeDampingChannel(2, 0.4), gates.ThermalRelaxationChannel(3, [1.0, 1.5, 0.3])):
+        want = od.apply_dm(ch, rho.copy(), n)
+        got = _np(ch.apply(backend, backend.cast(rho), n))
+        assert np.max(np.abs(got - want)) < 1e-14, ch.__class__.__name__
+    with pytest.raises(ValueError):
+        backend.depolarizing_error_density_matrix(gates.DepolarizingChannel((0,), 0.1), backend.cast(rho[0]), n)
