@@ -1000,6 +1000,53 @@ class MI355XBackend(GatePlanner):
         state.copy_(big[: state.numel()])
         return state
 
+    def _run_dm_cached(self, queue, state: torch.Tensor, nqubits: int) -> bool:
+        """Run a density-matrix queue whose every element is a unitary gate, a
+        non-collapsing measurement or a channel with a transfer matrix as one
+        op list, reusing the compiled passes of an identical op list (the
+        program cache of run_queue, keyed by every op's kind, qubits and exact
+        data). Returns False, having done nothing, for other queues."""
+        from .channels import Channel
+        from .measurements_gate import M
+        from .special import FusedGate
+
+        if not (self.program_cache and self.blocking and self.dm_superoperators) or 2 * nqubits < blocking.TILE_BITS:
+            return False
+        ops, mgates = [], []
+        for gate in queue:
+            if isinstance(gate, M) and not gate.collapse:
+                mgates.append(gate)
+                continue
+            if isinstance(gate, Channel):
+                sop = self.channel_transfer_matrix(gate)
+                if sop is None:
+                    return False
+                t = tuple(gate.target_qubits)
+                ops.append(plan_matrix(sop, t + tuple(q + nqubits for q in t), ()))
+                continue
+            if isinstance(gate, (M, gates_mod.SpecialGate)) and not isinstance(gate, FusedGate):
+                return False
+            ops.extend(_dm_pair(self.plan_gate(gate), nqubits))
+        for gate in mgates:
+            gate.result.backend = self
+        key = ("density", nqubits, str(state.dtype), str(state.device), bool(self.blocking_ordered),
+               tuple((op.kind, op.targets, op.controls,
+                      None if op.data is None else (op.data.dtype.str, op.data.shape, op.data.tobytes()))
+                     for op in ops if op.kind != "noop"))
+        flat = state.view(-1)
+        actions = self._programs.get(key)
+        if actions is not None:
+            self._programs.move_to_end(key)
+            self.program_cache_hits += 1
+            self._replay(actions, [], flat, 2 * nqubits, [False])
+            return True
+        record = []
+        self._run_blocked(ops, flat, 2 * nqubits, None, record)
+        self._programs[key] = record
+        while len(self._programs) > self.program_cache_size:
+            self._programs.popitem(last=False)
+        return True
+
     def run_queue_density_matrix(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
         """Apply ``queue`` to a density matrix (the loop of abstract.py:3321-3322).
 
@@ -1013,6 +1060,8 @@ class MI355XBackend(GatePlanner):
 
         state = state.contiguous()
         self._dm_check(state, nqubits, "run_queue_density_matrix")
+        if self._run_dm_cached(queue, state, nqubits):
+            return state
         ops = []
         for gate in queue:
             if isinstance(gate, M) and not gate.collapse:

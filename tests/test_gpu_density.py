@@ -237,3 +237,22 @@ def test_half_density_matrix_and_direct_channel_calls(backend, rng):
         assert np.max(np.abs(got - want)) < 1e-14, ch.__class__.__name__
     with pytest.raises(ValueError):
         backend.depolarizing_error_density_matrix(gates.DepolarizingChannel((0,), 0.1), backend.cast(rho[0]), n)
+
+
+def test_program_cache_replays_density_circuits(backend, rng):
+    """A second execution of the same noisy circuit replays the compiled
+    passes (one cache hit) and gives the same matrix bit for bit."""
+    n = 7
+    c = Circuit(n, density_matrix=True)
+    for q in range(n):
+        c.add(gates.RY(q, float(rng.uniform(0, 6))))
+        c.add(gates.DepolarizingChannel((q,), 0.02))
+    c.add(gates.CZ(0, 6))
+    c.add(gates.PauliNoiseChannel((1, 4), [("XZ", 0.1), ("YY", 0.05)]))
+    first = _np(backend.execute_circuit(c).state())
+    hits = backend.program_cache_hits
+    c._final_state = None
+    second = _np(backend.execute_circuit(c).state())
+    assert backend.program_cache_hits == hits + 1
+    assert np.array_equal(first, second)
+    assert np.max(np.abs(first - OracleDensity().execute_dm(c))) < 1e-12
