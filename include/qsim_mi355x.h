@@ -228,6 +228,14 @@ int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int
 int qsim_samples_to_binary(const int64_t* samples, int64_t nshots, int nbits, int64_t* out,
                            void* stream);
 
+/* Readout bitflips on binary samples [nshots, nbits] (int64 0/1, row-major):
+ * out = s + (1 - s) [u < p0[col]] - s [u < p1[col]] with `uniforms` a device
+ * array of nshots * nbits doubles in the same order, drawn on the host by
+ * np.random.random_sample((nshots, nbits)); p0/p1 are host arrays of nbits
+ * probabilities. Replaces Backend.apply_bitflips (abstract.py:2279-2295). */
+int qsim_apply_bitflips(const int64_t* samples, int64_t nshots, int nbits, const double* uniforms,
+                        const double* p0, const double* p1, int64_t* out, void* stream);
+
 /* counts[v] += number of s with samples[s] == v, for v < nbins (int64 counts),
  * the accumulation of Backend.update_frequencies (abstract.py:2795-2801). */
 int qsim_histogram(const int64_t* samples, int64_t nshots, int64_t* counts, int64_t nbins,

@@ -459,6 +459,15 @@ class OracleBackend:
         return frequencies
 
     @staticmethod
+    def apply_bitflips(noiseless_samples, bitflip_probabilities):
+        """abstract.py:2279-2295: one uniform per sample bit, then the flips."""
+        sprobs = np.random.random(noiseless_samples.shape).astype(np.float64)
+        flip_0 = (sprobs < bitflip_probabilities[0]).astype(noiseless_samples.dtype)
+        flip_1 = (sprobs < bitflip_probabilities[1]).astype(noiseless_samples.dtype)
+        noisy_samples = noiseless_samples + (1 - noiseless_samples) * flip_0
+        return noisy_samples - noiseless_samples * flip_1
+
+    @staticmethod
     def samples_to_binary(samples, nqubits):
         qrange = np.arange(nqubits - 1, -1, -1, dtype=np.int32)
         return np.right_shift(samples[:, None], qrange) % 2

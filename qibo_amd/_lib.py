@@ -64,6 +64,7 @@ SIGNATURES = {
     "qsim_dm_expval_workspace": (_sz, [_i]),
     "qsim_dm_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "qsim_plan_pass": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp]),
+    "qsim_apply_bitflips": (_i, [_vp, _i64, _i, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

@@ -1289,6 +1289,29 @@ class MI355XBackend(GatePlanner):
                                                    self._stream(s)), self.lib)
         return out
 
+    def apply_bitflips(self, noiseless_samples, bitflip_probabilities):
+        """Readout bitflips (abstract.py:2279-2295): one uniform per sample bit
+        from numpy's global generator, in the samples' row-major order, then
+        s + (1 - s) [u < p0] - s [u < p1] on the device."""
+        s = noiseless_samples if isinstance(noiseless_samples, torch.Tensor) else \
+            torch.as_tensor(np.asarray(noiseless_samples))
+        s = s.to(device=self.torch_device, dtype=torch.int64).contiguous()
+        if s.dim() != 2:
+            raise_error(ValueError, "apply_bitflips expects samples of shape (nshots, nbits).")
+        nshots, nbits = s.shape
+        probs = np.asarray(self.to_numpy(bitflip_probabilities), dtype=np.float64).reshape(2, -1)
+        if probs.shape[1] != nbits:
+            raise_error(ValueError, f"{nbits} measured bits but {probs.shape[1]} bitflip probabilities.")
+        u = np.random.random_sample((nshots, nbits))
+        ud = torch.from_numpy(np.ascontiguousarray(u)).pin_memory().to(s.device, non_blocking=True)
+        p0 = np.ascontiguousarray(probs[0])
+        p1 = np.ascontiguousarray(probs[1])
+        out = torch.empty_like(s)
+        _lib.check(self.lib.qsim_apply_bitflips(s.data_ptr(), nshots, nbits, ud.data_ptr(), p0.ctypes.data,
+                                                p1.ctypes.data, out.data_ptr(), self._stream(s)), self.lib)
+        _keep_alive(ud, out)
+        return out
+
     def samples_to_decimal(self, samples, nqubits):
         s = samples if isinstance(samples, torch.Tensor) else torch.as_tensor(np.asarray(samples))
         s = s.to(device=self.torch_device, dtype=torch.int64)

@@ -1050,6 +1050,28 @@ __global__ __launch_bounds__(kT) void k_prob_stats_final(const double* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// readout bitflips on binary samples (Backend.apply_bitflips,
+// abstract.py:2279-2295): with u uniform per sample bit,
+// noisy = s + (1 - s) [u < p0] - s [u < p1], in integers
+// ---------------------------------------------------------------------------
+struct BitflipArgs {
+  int64_t n;     // nshots * nbits
+  int nbits;
+  double p0[kMaxFixed], p1[kMaxFixed];
+};
+
+__global__ __launch_bounds__(kT) void k_bitflips(const int64_t* __restrict__ s, const double* __restrict__ u,
+                                                 const BitflipArgs a, int64_t* __restrict__ out) {
+  const int64_t i = int64_t(block_id()) * kT + threadIdx.x;
+  if (i >= a.n) return;
+  const int col = int(i % a.nbits);
+  const int64_t x = s[i];
+  const double r = u[i];
+  const int64_t f0 = r < a.p0[col] ? 1 : 0, f1 = r < a.p1[col] ? 1 : 0;
+  out[i] = (x + (1 - x) * f0) - x * f1;
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -1198,6 +1220,28 @@ int qsim_histogram(const int64_t* samples, int64_t nshots, int64_t* counts, int6
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_histogram, launch_grid(ceil_div_u(uint64_t(nshots), kT)), dim3(kT), 0, s, samples, nshots,
                      reinterpret_cast<unsigned long long*>(counts), nbins);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_apply_bitflips(const int64_t* samples, int64_t nshots, int nbits, const double* uniforms,
+                        const double* p0, const double* p1, int64_t* out, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(samples != nullptr && uniforms != nullptr && out != nullptr && p0 != nullptr && p1 != nullptr,
+                 "qsim_apply_bitflips: null pointer");
+  QSIM_CHECK_ARG(nbits >= 1 && nbits <= kMaxFixed && nshots >= 0 && nshots <= (int64_t(1) << 40),
+                 "qsim_apply_bitflips: bad sizes");
+  BitflipArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = nshots * nbits;
+  a.nbits = nbits;
+  for (int k = 0; k < nbits; ++k) {
+    a.p0[k] = p0[k];
+    a.p1[k] = p1[k];
+  }
+  if (a.n == 0) return QSIM_OK;
+  hipLaunchKernelGGL(k_bitflips, launch_grid(ceil_div_u(uint64_t(a.n), kT)), dim3(kT), 0,
+                     reinterpret_cast<hipStream_t>(stream), samples, uniforms, a, out);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }

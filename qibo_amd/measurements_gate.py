@@ -57,9 +57,13 @@ class M(_g.Gate):
             raise_error(
                 NotImplementedError, "Bitflip measurement noise is not available when collapsing."
             )
-        if p0 is not None or p1 is not None:
-            raise_error(NotImplementedError, "Bitflip measurement noise is not supported.")
-        self.bitflip_map = ({q_: 0 for q_ in self.target_qubits}, {q_: 0 for q_ in self.target_qubits})
+        # bitflip probabilities per qubit (measurements.py:92-97): p0 flips a
+        # measured 0, p1 a measured 1; one given stands for both
+        if p1 is None:
+            p1 = p0
+        if p0 is None:
+            p0 = p1
+        self.bitflip_map = (self._get_bitflip_map(p0), self._get_bitflip_map(p1))
         # basis change gates added by the circuit before this measurement
         self.basis = []
         for qubit, cls in zip(self.target_qubits, self.basis_gates):
@@ -67,8 +71,33 @@ class M(_g.Gate):
             if rot is not None:
                 self.basis.append(rot)
 
+    @staticmethod
+    def _get_bitflip_tuple(qubits, probs):
+        """Per-qubit probabilities from a float, a list/tuple (one per qubit)
+        or a dict {qubit: p} (measurements.py:121-147)."""
+        if isinstance(probs, float):
+            if probs < 0 or probs > 1:
+                raise_error(ValueError, f"Invalid bitflip probability {probs}.")
+            return len(qubits) * (probs,)
+        if isinstance(probs, (tuple, list)):
+            if len(probs) != len(qubits):
+                raise_error(ValueError, f"{len(qubits)} qubits were measured but the given bitflip "
+                                        f"probability list contains {len(probs)} values.")
+            return tuple(probs)
+        if isinstance(probs, dict):
+            diff = set(probs.keys()) - set(qubits)
+            if diff:
+                raise_error(KeyError, f"Bitflip map contains {diff} qubits that are not measured.")
+            return tuple(probs[q] if q in probs else 0.0 for q in qubits)
+        raise_error(TypeError, f"Invalid type {probs} of bitflip map.")
+
+    def _get_bitflip_map(self, p=None):
+        if p is None:
+            return {q: 0 for q in self.qubits}
+        return dict(zip(self.qubits, self._get_bitflip_tuple(self.qubits, p)))
+
     def has_bitflip_noise(self):
-        return False
+        return sum(self.bitflip_map[0].values()) > 0 or sum(self.bitflip_map[1].values()) > 0
 
     def add(self, gate):
         """Merge another measurement's qubits (used for the global measurement gate)."""

@@ -95,6 +95,8 @@ class MeasurementOutcomes:
                     kwargs.pop("p0", None)
                     kwargs.pop("p1", None)
                     self._measurement_gate = M(*gate.init_args, **kwargs)
+                    self._measurement_gate.bitflip_map[0].update(gate.bitflip_map[0])
+                    self._measurement_gate.bitflip_map[1].update(gate.bitflip_map[1])
                 else:
                     self._measurement_gate.add(gate)
         return self._measurement_gate
@@ -109,6 +111,8 @@ class MeasurementOutcomes:
                 return self._repeated_execution_frequencies
             return collections.Counter({int(k, 2): v for k, v in self._repeated_execution_frequencies.items()})
         if self._frequencies is None:
+            if self.measurement_gate.has_bitflip_noise() and not self.has_samples():
+                self._samples = self.samples()  # frequencies of the noisy samples (result.py:244-245)
             if not self.has_samples():
                 self._frequencies = self.backend.sample_frequencies(self._probs, self.nshots)
                 qubit_map = {q: i for i, q in enumerate(qubits)}
@@ -151,6 +155,10 @@ class MeasurementOutcomes:
                 else:
                     samples = self.backend.sample_shots(self._probs, self.nshots)
                 samples = self.backend.samples_to_binary(samples, len(qubits))
+                if self.measurement_gate.has_bitflip_noise():  # result.py:412-423
+                    p0, p1 = self.measurement_gate.bitflip_map
+                    probs = np.array([[p0.get(q) for q in qubits], [p1.get(q) for q in qubits]], dtype=np.float64)
+                    samples = self.backend.apply_bitflips(samples, probs)
                 qubit_map = dict(zip(qubits, range(len(qubits))))
                 self._samples = samples
                 for gate in self.measurements:
@@ -164,6 +172,14 @@ class MeasurementOutcomes:
         if binary:
             return self._samples
         return self.backend.samples_to_decimal(self._samples, len(qubits))
+
+    def apply_bitflips(self, p0, p1=None):
+        """The samples with bitflips of probabilities ``p0`` (and ``p1``) drawn
+        on top (result.py:460-467, measurements.py:19-30)."""
+        gate = self.measurement_gate
+        t0 = gate._get_bitflip_tuple(gate.qubits, p0)
+        t1 = t0 if p1 is None else gate._get_bitflip_tuple(gate.qubits, p1)
+        return self.backend.apply_bitflips(self.samples(), np.array([t0, t1], dtype=np.float64))
 
 
 class CircuitResult(QuantumState, MeasurementOutcomes):
