@@ -408,10 +408,14 @@ class OracleBackend:
                     qubits = sorted(gate.target_qubits)
                     probs = self.calculate_probabilities(state, qubits, nqubits)
                     shot = self.sample_shots(probs, 1)
-                    records[k].append(self.samples_to_binary(shot, len(qubits))[0])
+                    bits = self.samples_to_binary(shot, len(qubits))[0]
+                    records[k].append(bits)
+                    gate.result.register_samples(np.asarray([bits]))  # the outcome symbols read
                     k += 1
                     state = self.collapse_state(state, qubits, shot, nqubits)
                     continue
+                if gate.symbolic_parameters:  # abstract.py:2590-2591
+                    gate.substitute_symbols()
                 state = self.apply_gate(gate, state, nqubits)
             if final_qubits:
                 probs = self.calculate_probabilities(state, final_qubits, nqubits)

@@ -866,6 +866,8 @@ class MI355XBackend(GatePlanner):
                 if isinstance(gate, (gates_mod.SpecialGate, Channel)) and not isinstance(gate, FusedGate):
                     stop.append(gate)
                     return
+                if gate.symbolic_parameters:  # conditioned on an earlier collapse
+                    gate.substitute_symbols()
                 yield self.plan_gate(gate)
 
         while True:
@@ -974,8 +976,12 @@ class MI355XBackend(GatePlanner):
             if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
                                                   and not isinstance(gate, FusedGate)):
                 for g in queue:  # callbacks need the real state between gates
+                    if g.symbolic_parameters:
+                        g.substitute_symbols()
                     state = g.apply(self, state, nqubits)
                 return state
+            if gate.symbolic_parameters:
+                gate.substitute_symbols()
             ops.append(self.plan_gate(gate))
         return self._run_ops(ops, state, nqubits)
 
@@ -1025,6 +1031,8 @@ class MI355XBackend(GatePlanner):
                 ops.append(plan_matrix(sop, t + tuple(q + nqubits for q in t), ()))
                 continue
             if isinstance(gate, (M, gates_mod.SpecialGate)) and not isinstance(gate, FusedGate):
+                return False
+            if gate.symbolic_parameters:
                 return False
             ops.extend(_dm_pair(self.plan_gate(gate), nqubits))
         for gate in mgates:
@@ -1079,6 +1087,8 @@ class MI355XBackend(GatePlanner):
                 ops = []
                 state = gate.apply(self, state, nqubits).contiguous()
                 continue
+            if gate.symbolic_parameters:  # conditioned on an earlier collapse
+                gate.substitute_symbols()
             ops.extend(_dm_pair(self.plan_gate(gate), nqubits))
         self._run_ops(ops, state.view(-1), 2 * nqubits)
         return state

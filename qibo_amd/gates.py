@@ -193,10 +193,22 @@ class ParametrizedGate(Gate):
                 f"Parametrized gate has {nparams} parameters but {len(x)} update values "
                 "were given.",
             )
+        for i, v in enumerate(x):
+            if type(v).__module__.startswith("sympy"):  # measurement-conditioned (abstract.py:733-735)
+                self.symbolic_parameters[i] = v
         self._parameters = tuple(x)
         self.init_kwargs.update(
             {n: v for n, v in zip(names, self._parameters) if n in self.init_kwargs}
         )
+
+    def substitute_symbols(self):
+        """Replace measurement symbols by the latest outcomes (abstract.py:746-752)."""
+        params = list(self._parameters)
+        for i, param in self.symbolic_parameters.items():
+            for symbol in param.free_symbols:
+                param = symbol.evaluate(param)
+            params[i] = float(param)
+        self.parameters = tuple(params)
 
     def matrix(self, backend=None):
         backend = _check_backend(backend)

@@ -16,6 +16,40 @@ def frequencies_to_binary(frequencies, nqubits):
     return collections.Counter({f"{k:0{nqubits}b}": v for k, v in frequencies.items()})
 
 
+def _measurement_symbol_class():
+    """sympy.Symbol tied to one bit of a measurement's latest outcome
+    (measurements.py:33-73); sympy is imported on first use."""
+    global _MeasurementSymbol
+    if _MeasurementSymbol is None:
+        import sympy
+
+        class MeasurementSymbol(sympy.Symbol):
+            _counter = 0
+
+            def __new__(cls, *args, **kwargs):
+                name = f"m{cls._counter}"
+                cls._counter += 1
+                return super().__new__(cls=cls, name=name)
+
+            def __init__(self, index, result):
+                self.index = index
+                self.result = result
+
+            def outcome(self):
+                res = self.result
+                samples = res._samples if res._samples is not None else res.samples(binary=True)
+                return int(_to_numpy(samples)[-1][self.index])
+
+            def evaluate(self, expr):
+                return expr.subs(self, self.outcome())
+
+        _MeasurementSymbol = MeasurementSymbol
+    return _MeasurementSymbol
+
+
+_MeasurementSymbol = None
+
+
 class MeasurementResult:
     def __init__(self, target_qubits):
         self.target_qubits = tuple(target_qubits)
@@ -23,6 +57,16 @@ class MeasurementResult:
         self.backend = None
         self._samples = None
         self._frequencies = None
+        self._symbols = None
+
+    @property
+    def symbols(self):
+        """One sympy symbol per measured qubit, for gate parameters conditioned
+        on this (collapsing) measurement's outcome (measurements.py:147-158)."""
+        if self._symbols is None:
+            cls = _measurement_symbol_class()
+            self._symbols = [cls(i, self) for i in range(len(self.target_qubits))]
+        return self._symbols
 
     def has_samples(self):
         return self._samples is not None

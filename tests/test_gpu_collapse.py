@@ -94,3 +94,45 @@ def test_measurement_collapse(backend, n, targets, rng):
     c.add(gates.M(*targets))
     outcome = backend.execute_circuit(c, backend.cast(_random_state(n, rng), copy=True), nshots=1)
     assert np.array_equal(_np(r.samples())[0], _np(outcome.samples())[0])
+
+
+@pytest.mark.parametrize("n", [3, 13])
+def test_measurement_conditioned_gates_match_oracle(backend, n):
+    """Gates whose angles depend on a collapse outcome (r.symbols): every shot
+    equals the oracle's for the same seed, on small (one-pass) and blocked
+    registers."""
+    def circuit():
+        c = Circuit(n)
+        for q in range(3):
+            c.add(gates.H(q))
+        r = c.add(gates.M(0, 2, collapse=True))
+        c.add(gates.RX(1, theta=np.pi * r.symbols[0] / 3))
+        c.add(gates.RY(n - 1, theta=np.pi * (r.symbols[0] + 2 * r.symbols[1]) / 5))
+        c.add(gates.CNOT(1, n - 1))
+        c.add(gates.M(0, 1, n - 1))
+        return c
+
+    np.random.seed(31)
+    want, _ = OracleBackend().execute_repeated(circuit(), 60)
+    np.random.seed(31)
+    got = backend.execute_circuit(circuit(), nshots=60).samples()
+    got = got.cpu().numpy() if isinstance(got, torch.Tensor) else np.asarray(got)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("effect", [False, True])
+def test_measurement_result_parameters_density(backend, effect):
+    """The reference's test_measurement_result_parameters (density matrices):
+    X(0) then a collapse of qubit 0 conditions RX(pi/4) on qubit 1."""
+    c = Circuit(4, density_matrix=True)
+    if effect:
+        c.add(gates.X(0))
+    r = c.add(gates.M(0, collapse=True))
+    c.add(gates.RX(1, theta=np.pi * r.symbols[0] / 4))
+    t = Circuit(4, density_matrix=True)
+    if effect:
+        t.add(gates.X(0))
+        t.add(gates.RX(1, theta=np.pi / 4))
+    got = backend.execute_circuit(c, nshots=1).state().cpu().numpy()
+    want = backend.execute_circuit(t).state().cpu().numpy()
+    assert np.max(np.abs(got - want)) < 1e-14

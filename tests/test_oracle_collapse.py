@@ -65,3 +65,20 @@ def test_repeated_execution_draw_order():
     expect = (u[0::2] >= 0.5).astype(np.int64)
     assert np.array_equal(collapsed[:, 0], expect)
     assert np.array_equal(final[:, 0], expect) and not final[:, 1].any()
+
+
+def test_measurement_conditioned_flip():
+    """RX(pi * outcome) on qubit 1 after collapsing qubit 0 of |+0>: qubit 1
+    copies qubit 0 in every shot (the feedback of
+    tests/test_measurements_collapse.py:100-123 of the reference)."""
+    c = Circuit(3)
+    c.add(gates.H(0))
+    r = c.add(gates.M(0, collapse=True))
+    c.add(gates.RX(1, theta=np.pi * r.symbols[0]))
+    c.add(gates.M(0, 1))
+    np.random.seed(5)
+    samples, records = OracleBackend().execute_repeated(c, 40)
+    assert samples.shape == (40, 2)
+    assert np.array_equal(samples[:, 0], samples[:, 1])
+    assert np.array_equal(records[0][:, 0], samples[:, 0])
+    assert 0 < samples[:, 0].sum() < 40
