@@ -900,21 +900,29 @@ class MI355XBackend(GatePlanner):
         return None  # unknown parameter type: not cacheable
 
     def _gate_key(self, gate):
-        from .measurements_gate import M
-        from .special import FusedGate
-
-        if isinstance(gate, M):
+        if isinstance(gate, _M):
             return None if gate.collapse else ("M",)
-        if isinstance(gate, FusedGate):
+        if isinstance(gate, _FusedGate):
             parts = tuple(self._gate_key(g) for g in gate.gates)
             return None if any(k is None for k in parts) else ("Fused", gate.target_qubits, parts)
         if isinstance(gate, gates_mod.SpecialGate) or getattr(gate, "coefficients", None) is not None:
             return None  # callbacks and channels run outside the compiled passes
+        # the key is kept on the gate while its parameter and qubit tuples are the
+        # same objects (every setter assigns new ones); array parameters, which
+        # can change in place, are described anew each time
+        fp = (gate._parameters, gate._target_qubits, gate._control_qubits, gate.is_controlled_by)
+        cached = gate.__dict__.get("_qkey")
+        if (cached is not None and cached[0] is fp[0] and cached[1] is fp[1] and cached[2] is fp[2]
+                and cached[3] == fp[3]):
+            return cached[4]
         params = gate.parameters if isinstance(gate, gates_mod.ParametrizedGate) else ()
         pk = tuple(self._param_key(x) for x in params)
         if any(k is None for k in pk):
             return None
-        return (type(gate).__name__, gate.target_qubits, gate.control_qubits, gate.is_controlled_by, pk)
+        key = (type(gate).__name__, gate.target_qubits, gate.control_qubits, gate.is_controlled_by, pk)
+        if not any(isinstance(x, (np.ndarray, torch.Tensor)) for x in params):
+            gate.__dict__["_qkey"] = fp + (key,)
+        return key
 
     def _queue_key(self, queue, nqubits, state, zero_pending):
         """Exact description of what run_queue would compile for ``queue``:
@@ -1452,3 +1460,7 @@ def _np_cdtype(array):
 def _keep_alive(tmp: torch.Tensor, user: torch.Tensor):
     """Keep a temporary device buffer alive until the stream has consumed it."""
     tmp.record_stream(torch.cuda.current_stream(user.device))
+
+
+from .measurements_gate import M as _M  # noqa: E402  (after the backend classes: no import cycle)
+from .special import FusedGate as _FusedGate  # noqa: E402

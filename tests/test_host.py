@@ -249,3 +249,29 @@ def test_diagonal_planning_matches_general_peeling():
             assert (got.kind, got.targets, got.controls) == (kind, tuple(tg), tuple([25] + extra))
             if kind == "diag":
                 assert np.array_equal(got.data, want_d)
+
+
+def test_program_cache_keys_follow_gate_changes():
+    """The per-gate program-cache key kept on a gate changes whenever its
+    parameters or qubits do (setters assign new tuples), and is not kept for
+    array parameters that could change in place."""
+    import numpy as np
+
+    from qibo_amd import gates, models
+    from qibo_amd.backend import MI355XBackend
+
+    be = MI355XBackend.__new__(MI355XBackend)  # host methods only: no device
+    c = models.random_layered_circuit(6, 2, seed=1, measure=False)
+    before = [be._gate_key(g) for g in c.queue]
+    assert [be._gate_key(g) for g in c.queue] == before
+    c.set_parameters(np.array(c.get_parameters()).ravel() + 0.1)
+    after = [be._gate_key(g) for g in c.queue]
+    assert sum(a != b for a, b in zip(before, after)) == sum(1 for g in c.queue if g.__class__.__name__ in ("RY", "RZ"))
+    h = gates.H(1)
+    k1 = be._gate_key(h)
+    h.controlled_by(0, 2)
+    assert be._gate_key(h) == ("H", (1,), (0, 2), True, ())
+    assert k1 != be._gate_key(h)
+    u = gates.Unitary(np.eye(2), 0)
+    be._gate_key(u)
+    assert "_qkey" not in u.__dict__
