@@ -256,3 +256,26 @@ def test_program_cache_replays_density_circuits(backend, rng):
     assert backend.program_cache_hits == hits + 1
     assert np.array_equal(first, second)
     assert np.max(np.abs(first - OracleDensity().execute_dm(c))) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_smallest_density_matrices(backend, n, rng):
+    """One- and two-qubit density matrices (a 2- or 4-qubit vectorized matrix,
+    run inside a padded 12-qubit pass), empty circuits and measurements."""
+    empty = Circuit(n, density_matrix=True)
+    want = OracleDensity().execute_dm(empty)
+    assert np.array_equal(_np(backend.execute_circuit(empty).state()), want)
+    c = Circuit(n, density_matrix=True)
+    c.add(gates.H(0))
+    c.add(gates.AmplitudeDampingChannel(0, 0.3))
+    c.add(gates.DepolarizingChannel((n - 1,), 0.2))
+    if n == 2:
+        c.add(gates.CNOT(0, 1))
+        c.add(gates.PauliNoiseChannel((0, 1), [("XY", 0.1), ("ZZ", 0.2)]))
+    c.add(gates.M(*range(n)))
+    want = OracleDensity().execute_dm(c)
+    result = backend.execute_circuit(c, nshots=100)
+    assert np.max(np.abs(_np(result.state()) - want)) < 1e-14
+    p = _np(result.probabilities())
+    assert np.max(np.abs(p - np.real(np.diag(want)))) < 1e-14
+    assert _np(result.samples()).shape == (100, n)
