@@ -1246,9 +1246,17 @@ class MI355XBackend(GatePlanner):
 
     def sample_shots(self, probabilities, nshots: int):
         p = self._as_prob_tensor(probabilities)
-        self._validate_probabilities(p)
-        cdf = self.exact_cdf(p)
+        # the uniforms are drawn while the device still computes p; if p fails
+        # the checks the generator is put back, so a failed call consumes
+        # nothing, as np.random.choice's checks run before its draws
+        rng_state = np.random.get_state()
         u = np.random.random_sample(int(nshots))
+        try:
+            self._validate_probabilities(p)
+        except ValueError:
+            np.random.set_state(rng_state)
+            raise
+        cdf = self.exact_cdf(p)
         return self.search_cdf(cdf, u)
 
     def np_sum(self, p: torch.Tensor) -> torch.Tensor:
