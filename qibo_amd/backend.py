@@ -233,13 +233,13 @@ class GatePlanner:
 
     # -- gate application ----------------------------------------------------
     def plan_gate(self, gate) -> _Op:
-        from .special import FusedGate
-
-        if isinstance(gate, FusedGate):
+        if isinstance(gate, _FusedGate):
             return plan_matrix(self.matrix_fused(gate), gate.target_qubits, ())
+        # gates that keep the base matrix() go straight to the backend formulas
+        m = self.matrix(gate) if type(gate).matrix in _BASE_MATRIX else gate.matrix(self)
         if gate.is_controlled_by:
-            return plan_matrix(gate.matrix(self), gate.target_qubits, gate.control_qubits)
-        return plan_matrix(gate.matrix(self), gate.qubits, ())
+            return plan_matrix(m, gate.target_qubits, gate.control_qubits)
+        return plan_matrix(m, gate.qubits, ())
 
     def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:
         """The map of a channel on one or two qubits T as a 4^k x 4^k matrix
@@ -1472,3 +1472,4 @@ def _keep_alive(tmp: torch.Tensor, user: torch.Tensor):
 
 from .measurements_gate import M as _M  # noqa: E402  (after the backend classes: no import cycle)
 from .special import FusedGate as _FusedGate  # noqa: E402
+_BASE_MATRIX = (gates_mod.Gate.matrix, gates_mod.ParametrizedGate.matrix)
