@@ -235,15 +235,15 @@ assert _HDR_STRUCT.size == HEADER_BYTES
 def rotation_form(m: np.ndarray):
     """(mode, t, s, negated) when the 2x2 ``m`` is exactly one of the ROT
     forms, else None. negated: m = -(the form with c >= 0 that is encoded)."""
-    m = np.asarray(m, dtype=np.complex128).reshape(2, 2)
-    re, im = m.real, m.imag
-    if not np.any(im) and re[0, 0] == re[1, 1] and re[0, 1] == -re[1, 0]:
-        mode, c, s = ROT_RY, re[0, 0], re[1, 0]
-    elif not np.any(im) and re[0, 0] == -re[1, 1] and re[0, 1] == re[1, 0]:
-        mode, c, s = ROT_H, re[0, 0], re[1, 0]
-    elif (re[0, 0] == re[1, 1] and im[0, 0] == 0 and im[1, 1] == 0 and re[0, 1] == 0
-          and re[1, 0] == 0 and im[0, 1] == im[1, 0]):
-        mode, c, s = ROT_RX, re[0, 0], -im[0, 1]
+    a, b, cc, d = np.asarray(m, dtype=np.complex128).ravel().tolist()  # Python complex: the same values
+    real = a.imag == 0 and b.imag == 0 and cc.imag == 0 and d.imag == 0
+    if real and a.real == d.real and b.real == -cc.real:
+        mode, c, s = ROT_RY, a.real, cc.real
+    elif real and a.real == -d.real and b.real == cc.real:
+        mode, c, s = ROT_H, a.real, cc.real
+    elif (a.real == d.real and a.imag == 0 and d.imag == 0 and b.real == 0
+          and cc.real == 0 and b.imag == cc.imag):
+        mode, c, s = ROT_RX, a.real, -b.imag
     else:
         return None
     if abs(c * c + s * s - 1.0) > 1e-14:  # the shears reproduce only an exact rotation
