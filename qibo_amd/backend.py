@@ -334,6 +334,9 @@ class MI355XBackend(GatePlanner):
         # keep circuit order inside blocked passes (bitwise equal to the per-gate
         # path); the default lets commuting gates move ahead, giving fewer passes
         self.blocking_ordered = False
+        # index bits every pass keeps in its tile (consecutive amplitudes per
+        # memory segment); None: 3 for complex128, 4 for complex64 (128 bytes)
+        self.blocking_low_bits = None
         self.block_stats = collections.deque(maxlen=1 << 16)  # gates per blocked pass (diagnostics)
         # compiled blocked passes of recently executed gate queues, keyed by the
         # gates' kinds, qubits and exact parameter values (see run_queue)
@@ -1114,7 +1117,7 @@ class MI355XBackend(GatePlanner):
         ``record`` (a list) receives the launches for the program cache."""
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
-        low = 3 if code == _lib.C128 else 4
+        low = self.blocking_low_bits or (3 if code == _lib.C128 else 4)
         ordered = self.blocking_ordered
         stream = self._stream(state)
         cur = torch.cuda.current_stream(state.device)
