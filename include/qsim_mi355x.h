@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 5
+#define QSIM_ABI_VERSION 6
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -163,10 +163,12 @@ int qsim_adjoint_blocked(void* psi, void* lam, int nbatch, int nqubits, int dtyp
                          int nslots, double* partials, size_t partials_count, void* stream);
 
 /* grads[b * nparams + param_of[s]] = 2 * (sum of the partials of INNER
- * record s of batch entry b), summed in a fixed order. `param_of_dev` is a
- * DEVICE int array of nslots entries (-1: not stored). */
-int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nslots, const int* param_of_dev,
-                        int nparams, double* grads, void* stream);
+ * record s of batch entry b), summed in a fixed order. `partials` holds
+ * `partials_count` doubles, at least qsim_adjoint_partials_count(nqubits,
+ * nbatch, nslots) (the buffer qsim_adjoint_blocked filled). `param_of_dev` is
+ * a DEVICE int array of nslots entries (-1: not stored). */
+int qsim_adjoint_reduce(const double* partials, size_t partials_count, int nbatch, int nqubits, int nslots,
+                        const int* param_of_dev, int nparams, double* grads, void* stream);
 
 /* Probabilities of measuring `qubits` (m = nmeasured, in the given order, the
  * first being the most significant bit of the output index):

@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
@@ -56,7 +56,7 @@ SIGNATURES = {
     "qsim_apply_blocked_batched": (_i, [_vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
     "qsim_adjoint_partials_count": (_sz, [_i, _i, _i]),
     "qsim_adjoint_blocked": (_i, [_vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _i, _vp, _sz, _vp]),
-    "qsim_adjoint_reduce": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
+    "qsim_adjoint_reduce": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp]),
     "qsim_dm_axpy": (_i, [_vp, _vp, ctypes.c_uint64, _i, ctypes.c_double, _i, _vp]),
     "qsim_dm_depolarize": (_i, [_vp, _i, _i, _c_int_p, _i, ctypes.c_double, _vp]),
     "qsim_dm_reset": (_i, [_vp, _i, _i, _i, ctypes.c_double, ctypes.c_double, _vp]),

@@ -30,7 +30,7 @@ from typing import List, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import _lib, blocking, gates
+from . import _lib, blocking, gates, kinds
 from .backend import plan_matrix
 from .blocking import BatchOp
 from .config import raise_error
@@ -97,7 +97,7 @@ def expectation_and_gradient(backend, circuit, hamiltonian, initial_state=None):
         raise_error(NotImplementedError, "adjoint gradients are for state vectors")
     n = circuit.nqubits
     xm, zm, ny, coeffs = hamiltonian.pauli_tables()
-    queue = [g for g in circuit.queue if not isinstance(g, gates.M)]
+    queue = [g for g in circuit.queue if not kinds.is_measurement(g)]
     if initial_state is None:
         psi = backend.zero_state(n)
     else:
@@ -209,7 +209,7 @@ def _batched_ops(backend, circuit, params: np.ndarray, sources=None, groups=None
     With ``sources`` and ``groups`` (dicts) it also records, for the ops of
     trainable gates, id(op) -> (gate name, dagger, index among that name's
     gates) and, per gate name, the parameter slots in circuit order."""
-    queue = [g for g in circuit.queue if not isinstance(g, gates.M)]
+    queue = [g for g in circuit.queue if not kinds.is_measurement(g)]
     nb = params.shape[0]
     slot_of, k = {}, 0
     for g in circuit.trainable_gates:
@@ -223,7 +223,7 @@ def _batched_ops(backend, circuit, params: np.ndarray, sources=None, groups=None
     fwd, rev = [], []
     fixed = {}  # plans of parameter-free gates, by matrix and qubits
     for g in queue:
-        if not isinstance(g, gates.Gate) or isinstance(g, gates.SpecialGate):
+        if kinds.runs_outside_passes(g) or kinds.is_fused(g):
             raise _Unsupported("special gate")
         targets, controls = _gate_qubits(g)
         qubits = tuple(g.qubits) if not g.is_controlled_by else tuple(g.target_qubits)
@@ -421,7 +421,7 @@ def _structure_key(backend, circuit):
     slot_of = {id(g): k for k, g in enumerate(circuit.trainable_gates)}
     parts = []
     for g in circuit.queue:
-        if isinstance(g, gates.M):
+        if kinds.is_measurement(g):
             continue
         if id(g) in slot_of and g.trainable:
             parts.append((type(g).__name__, g.target_qubits, g.control_qubits, g.is_controlled_by, slot_of[id(g)]))
@@ -517,7 +517,7 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
                 pof = p.dev_slots.get(dev)
                 if pof is None:
                     pof = p.dev_slots[dev] = torch.tensor(p.slots, dtype=torch.int32).to(dev)
-                _lib.check(lib.qsim_adjoint_reduce(part.data_ptr(), nb, n, len(p.slots), pof.data_ptr(), nparams,
+                _lib.check(lib.qsim_adjoint_reduce(part.data_ptr(), npart, nb, n, len(p.slots), pof.data_ptr(), nparams,
                                                    gsub.data_ptr(), stream), lib)
         for t in keep:
             t.record_stream(torch.cuda.current_stream(dev))

@@ -30,6 +30,7 @@ the terms multiplied by an exact zero.
 import collections
 import math
 import os
+import threading
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -39,6 +40,7 @@ from . import _lib
 from . import blocking
 from . import config
 from . import gates as gates_mod
+from . import kinds
 from .config import SHOT_BATCH_SIZE, log, raise_error
 from .matrices import GateMatrices
 
@@ -233,7 +235,9 @@ class GatePlanner:
 
     # -- gate application ----------------------------------------------------
     def plan_gate(self, gate) -> _Op:
-        if isinstance(gate, _FusedGate):
+        if kinds.is_measurement(gate):
+            raise_error(ValueError, "measurement gates have no matrix and are not planned as gates")
+        if kinds.is_fused(gate):
             return plan_matrix(self.matrix_fused(gate), gate.target_qubits, ())
         # gates that keep the base matrix() go straight to the backend formulas
         m = self.matrix(gate) if type(gate).matrix in _BASE_MATRIX else gate.matrix(self)
@@ -255,14 +259,11 @@ class GatePlanner:
         ThermalRelaxationChannel uses the map its apply writes
         (channels.py:678-699), or the Kraus form on qubit 0 where the
         reference's reset branch puts its Z."""
-        from .channels import ThermalRelaxationChannel
-        from .special import FusedGate
-
         targets = tuple(channel.target_qubits)
         k = len(targets)
         if k not in (1, 2) or len(set(targets)) != k:
             return None
-        if isinstance(channel, ThermalRelaxationChannel):
+        if type(channel).__name__ == "ThermalRelaxationChannel":
             if channel.t_1 < channel.t_2:
                 p0, p1, e = (channel.init_kwargs[key] for key in ("p_0", "p_1", "e_t2"))
                 return np.array([[1 - p1, 0, 0, p0], [0, e, 0, 0], [0, 0, e, 0], [p1, 0, 0, 1 - p0]],
@@ -274,7 +275,7 @@ class GatePlanner:
         for coeff, gate in zip(channel.coefficients, channel.gates):
             if gate.is_controlled_by or gate.control_qubits or not set(gate.target_qubits) <= set(targets):
                 return None
-            u = self.matrix_fused(gate) if isinstance(gate, FusedGate) else gate.matrix(self)
+            u = self.matrix_fused(gate) if kinds.is_fused(gate) else gate.matrix(self)
             u = _embed(np.asarray(u, dtype=np.complex128), tuple(gate.target_qubits), targets)
             sop = sop + coeff * np.kron(u, np.conj(u))
         return sop
@@ -347,6 +348,7 @@ class MI355XBackend(GatePlanner):
         self.program_cache = True
         self.program_cache_size = 16
         self._programs = collections.OrderedDict()
+        self._programs_lock = threading.Lock()
         self.program_cache_hits = 0
         rec = self.lib.qsim_blocked_header_bytes()
         if rec != blocking.HEADER_BYTES:
@@ -828,10 +830,6 @@ class MI355XBackend(GatePlanner):
         stand for |0...0>: a first blocked pass produces it itself, anything
         else fills it first.
         """
-        from .channels import Channel
-        from .measurements_gate import M
-        from .special import FusedGate
-
         zero = [zero_pending]  # still to be materialised
         if zero_pending and (not self.blocking or nqubits < blocking.TILE_BITS):
             self._fill_zero(state, nqubits)
@@ -844,10 +842,8 @@ class MI355XBackend(GatePlanner):
             return self._run_small(queue, state, nqubits)
         key = self._queue_key(queue, nqubits, state, zero_pending) if self.program_cache else None
         if key is not None:
-            actions = self._programs.get(key)
+            actions = self._cache_get(key)
             if actions is not None:
-                self._programs.move_to_end(key)
-                self.program_cache_hits += 1
                 return self._replay(actions, queue, state, nqubits, zero)
             record = []
         else:
@@ -860,13 +856,10 @@ class MI355XBackend(GatePlanner):
             # measurement, produced lazily so planning overlaps the passes
             # already launched
             for gate in gates_left:
-                if isinstance(gate, M):
-                    if not gate.collapse:
-                        gate.result.backend = self
-                        continue
-                    stop.append(gate)
-                    return
-                if isinstance(gate, (gates_mod.SpecialGate, Channel)) and not isinstance(gate, FusedGate):
+                if kinds.is_measurement(gate) and not gate.collapse:
+                    gate.result.backend = self
+                    continue
+                if kinds.runs_outside_passes(gate):
                     stop.append(gate)
                     return
                 if gate.symbolic_parameters:  # conditioned on an earlier collapse
@@ -880,9 +873,7 @@ class MI355XBackend(GatePlanner):
                 zero[0] = False
             if not stop:
                 if record is not None:
-                    self._programs[key] = record
-                    while len(self._programs) > self.program_cache_size:
-                        self._programs.popitem(last=False)
+                    self._cache_put(key, record)
                 return state
             state = stop.pop().apply(self, state, nqubits)
 
@@ -903,12 +894,16 @@ class MI355XBackend(GatePlanner):
         return None  # unknown parameter type: not cacheable
 
     def _gate_key(self, gate):
-        if isinstance(gate, _M):
+        if kinds.is_measurement(gate):
             return None if gate.collapse else ("M",)
-        if isinstance(gate, _FusedGate):
+        if getattr(gate, "symbolic_parameters", None):
+            # parameters that follow a measurement outcome are substituted at run
+            # time (abstract.py:2584-2587): the values seen here may be stale
+            return None
+        if kinds.is_fused(gate):
             parts = tuple(self._gate_key(g) for g in gate.gates)
             return None if any(k is None for k in parts) else ("Fused", gate.target_qubits, parts)
-        if isinstance(gate, gates_mod.SpecialGate) or getattr(gate, "coefficients", None) is not None:
+        if kinds.runs_outside_passes(gate):
             return None  # callbacks and channels run outside the compiled passes
         # the key is kept on the gate while its parameter and qubit tuples are the
         # same objects (every setter assigns new ones); array parameters, which
@@ -918,7 +913,7 @@ class MI355XBackend(GatePlanner):
         if (cached is not None and cached[0] is fp[0] and cached[1] is fp[1] and cached[2] is fp[2]
                 and cached[3] == fp[3]):
             return cached[4]
-        params = gate.parameters if isinstance(gate, gates_mod.ParametrizedGate) else ()
+        params = tuple(getattr(gate, "parameters", ()) or ())
         pk = tuple(self._param_key(x) for x in params)
         if any(k is None for k in pk):
             return None
@@ -940,15 +935,40 @@ class MI355XBackend(GatePlanner):
                 return None
             parts.append(k)
         return (tuple(parts), nqubits, str(state.dtype), str(state.device), self.dtype,
-                bool(zero_pending), bool(self.blocking_ordered))
+                bool(zero_pending), bool(self.blocking_ordered), self._low_bits(self._dtype_code(state)))
+
+    def _low_bits(self, code: int) -> int:
+        """Index bits every blocked pass keeps in its tile: ``blocking_low_bits``,
+        or 3 for complex128 and 4 for complex64 (128-byte segments) when None."""
+        low = self.blocking_low_bits
+        if low is None:
+            return 3 if code == _lib.C128 else 4
+        if isinstance(low, bool) or not isinstance(low, (int, np.integer)) or not 0 <= int(low) < blocking.TILE_BITS:
+            raise_error(ValueError, f"blocking_low_bits must be an integer in [0, {blocking.TILE_BITS}), got {low!r}")
+        return int(low)
+
+    def _cache_get(self, key):
+        """Compiled launches of ``key`` (marked most recently used), or None.
+        The cache is shared by every thread that executes on this backend."""
+        with self._programs_lock:
+            actions = self._programs.get(key)
+            if actions is not None:
+                self._programs.move_to_end(key)
+                self.program_cache_hits += 1
+            return actions
+
+    def _cache_put(self, key, record):
+        with self._programs_lock:
+            self._programs[key] = record
+            self._programs.move_to_end(key)
+            while len(self._programs) > self.program_cache_size:
+                self._programs.popitem(last=False)
 
     def _replay(self, actions, queue, state, nqubits, zero):
         """Run the recorded launches of an identical queue: the same blocked
         programs (still on the device) and one-gate kernels, in the same order."""
-        from .measurements_gate import M
-
         for gate in queue:
-            if isinstance(gate, M):
+            if kinds.is_measurement(gate):
                 gate.result.backend = self
         code = self._dtype_code(state)
         stream = self._stream(state)
@@ -974,18 +994,12 @@ class MI355XBackend(GatePlanner):
         The first 2^n amplitudes of the padded state are the n-qubit state, and
         qubit q becomes qubit q + p of the padded register, so the whole circuit
         is a single kernel launch. Same arithmetic as the per-gate kernels."""
-        from .measurements_gate import M
-        from .special import FusedGate
-
-        from .channels import Channel
-
         ops = []
         for gate in queue:
-            if isinstance(gate, M) and not gate.collapse:
+            if kinds.is_measurement(gate) and not gate.collapse:
                 gate.result.backend = self
                 continue
-            if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
-                                                  and not isinstance(gate, FusedGate)):
+            if kinds.runs_outside_passes(gate):
                 for g in queue:  # callbacks need the real state between gates
                     if g.symbolic_parameters:
                         g.substitute_symbols()
@@ -1023,47 +1037,38 @@ class MI355XBackend(GatePlanner):
         op list, reusing the compiled passes of an identical op list (the
         program cache of run_queue, keyed by every op's kind, qubits and exact
         data). Returns False, having done nothing, for other queues."""
-        from .channels import Channel
-        from .measurements_gate import M
-        from .special import FusedGate
-
         if not (self.program_cache and self.blocking and self.dm_superoperators) or 2 * nqubits < blocking.TILE_BITS:
             return False
         ops, mgates = [], []
         for gate in queue:
-            if isinstance(gate, M) and not gate.collapse:
+            if kinds.is_measurement(gate) and not gate.collapse:
                 mgates.append(gate)
                 continue
-            if isinstance(gate, Channel):
+            if kinds.is_channel(gate):
                 sop = self.channel_transfer_matrix(gate)
                 if sop is None:
                     return False
                 t = tuple(gate.target_qubits)
                 ops.append(plan_matrix(sop, t + tuple(q + nqubits for q in t), ()))
                 continue
-            if isinstance(gate, (M, gates_mod.SpecialGate)) and not isinstance(gate, FusedGate):
-                return False
-            if gate.symbolic_parameters:
+            if kinds.runs_outside_passes(gate) or gate.symbolic_parameters:
                 return False
             ops.extend(_dm_pair(self.plan_gate(gate), nqubits))
         for gate in mgates:
             gate.result.backend = self
         key = ("density", nqubits, str(state.dtype), str(state.device), bool(self.blocking_ordered),
+               self._low_bits(self._dtype_code(state)),
                tuple((op.kind, op.targets, op.controls,
                       None if op.data is None else (op.data.dtype.str, op.data.shape, op.data.tobytes()))
                      for op in ops if op.kind != "noop"))
         flat = state.view(-1)
-        actions = self._programs.get(key)
+        actions = self._cache_get(key)
         if actions is not None:
-            self._programs.move_to_end(key)
-            self.program_cache_hits += 1
             self._replay(actions, [], flat, 2 * nqubits, [False])
             return True
         record = []
         self._run_blocked(ops, flat, 2 * nqubits, None, record)
-        self._programs[key] = record
-        while len(self._programs) > self.program_cache_size:
-            self._programs.popitem(last=False)
+        self._cache_put(key, record)
         return True
 
     def run_queue_density_matrix(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
@@ -1073,27 +1078,22 @@ class MI355XBackend(GatePlanner):
         matrix (U on the row qubits, conj(U) on the column qubits), grouped into
         blocked passes; channels, collapsing measurements and other special
         gates run in order between the passes through ``gate.apply``."""
-        from .channels import Channel
-        from .measurements_gate import M
-        from .special import FusedGate
-
         state = state.contiguous()
         self._dm_check(state, nqubits, "run_queue_density_matrix")
         if self._run_dm_cached(queue, state, nqubits):
             return state
         ops = []
         for gate in queue:
-            if isinstance(gate, M) and not gate.collapse:
+            if kinds.is_measurement(gate) and not gate.collapse:
                 gate.result.backend = self
                 continue
-            if isinstance(gate, Channel) and self.dm_superoperators:
+            if kinds.is_channel(gate) and self.dm_superoperators:
                 sop = self.channel_transfer_matrix(gate)
                 if sop is not None:
                     t = tuple(gate.target_qubits)
                     ops.append(plan_matrix(sop, t + tuple(q + nqubits for q in t), ()))
                     continue
-            if isinstance(gate, (M, Channel)) or (isinstance(gate, gates_mod.SpecialGate)
-                                                  and not isinstance(gate, FusedGate)):
+            if kinds.runs_outside_passes(gate):
                 self._run_ops(ops, state.view(-1), 2 * nqubits)
                 ops = []
                 state = gate.apply(self, state, nqubits).contiguous()
@@ -1117,7 +1117,7 @@ class MI355XBackend(GatePlanner):
         ``record`` (a list) receives the launches for the program cache."""
         code = self._dtype_code(state)
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
-        low = self.blocking_low_bits or (3 if code == _lib.C128 else 4)
+        low = self._low_bits(code)
         ordered = self.blocking_ordered
         stream = self._stream(state)
         cur = torch.cuda.current_stream(state.device)
@@ -1473,6 +1473,4 @@ def _keep_alive(tmp: torch.Tensor, user: torch.Tensor):
     tmp.record_stream(torch.cuda.current_stream(user.device))
 
 
-from .measurements_gate import M as _M  # noqa: E402  (after the backend classes: no import cycle)
-from .special import FusedGate as _FusedGate  # noqa: E402
 _BASE_MATRIX = (gates_mod.Gate.matrix, gates_mod.ParametrizedGate.matrix)

@@ -1154,12 +1154,14 @@ int qsim_adjoint_blocked(void* psi, void* lam, int nbatch, int nqubits, int dtyp
                                program_stride, partials, nslots, s);
 }
 
-int qsim_adjoint_reduce(const double* partials, int nbatch, int nqubits, int nslots, const int* param_of_dev,
-                        int nparams, double* grads, void* stream) {
+int qsim_adjoint_reduce(const double* partials, size_t partials_count, int nbatch, int nqubits, int nslots,
+                        const int* param_of_dev, int nparams, double* grads, void* stream) {
   clear_error();
   QSIM_CHECK_ARG(partials != nullptr && param_of_dev != nullptr && grads != nullptr, "qsim_adjoint_reduce: null");
   QSIM_CHECK_ARG(nbatch >= 1 && nslots >= 1 && nparams >= 1, "qsim_adjoint_reduce: bad sizes");
   QSIM_CHECK_ARG(nqubits >= kBTileBits && nqubits <= QSIM_MAX_QUBITS, "qsim_adjoint_reduce: bad nqubits");
+  QSIM_CHECK_ARG(partials_count >= qsim_adjoint_partials_count(nqubits, nbatch, nslots),
+                 "qsim_adjoint_reduce: partials buffer smaller than nbatch * nslots tiles of partial sums");
   const uint64_t per_slot = (uint64_t(1) << (nqubits - kBTileBits)) * 4;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_adjoint_reduce, dim3(unsigned(uint64_t(nbatch) * uint64_t(nslots))), dim3(256), 0, s,

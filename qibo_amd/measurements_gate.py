@@ -101,7 +101,10 @@ class M(_g.Gate):
 
     def add(self, gate):
         """Merge another measurement's qubits (used for the global measurement gate)."""
-        assert isinstance(gate, self.__class__)
+        from .kinds import is_measurement
+
+        if not is_measurement(gate):
+            raise_error(TypeError, f"cannot merge {type(gate).__name__} into a measurement gate")
         self.target_qubits += gate.target_qubits
         self.bitflip_map[0].update(gate.bitflip_map[0])
         self.bitflip_map[1].update(gate.bitflip_map[1])

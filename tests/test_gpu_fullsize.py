@@ -7,9 +7,11 @@ Size-independent properties where the oracle cannot run the whole circuit:
 * 100 000 shots drawn from that state's 2^30 probabilities equal, bit for
   bit, numpy's np.random.choice on the same probabilities with the same seed
   (the reference's sample_shots, abstract.py:2774-2781), run on the host;
-* the 32-qubit complex64 circuit fused into 5-qubit blocks (32 GiB state, the
-  MFMA path) followed by the unfused inverse returns |0...0> to complex64
-  accuracy.
+* the 32-qubit complex64 layered circuit grouped by ``fuse(max_qubits=5)``
+  (32 GiB state) followed by the unfused inverse returns |0...0> to complex64
+  accuracy. On this circuit qibo's fusion forms 2- and 3-qubit groups only,
+  which run as dense records inside blocked passes; the dense 5-qubit block
+  kernels at 32 qubits are covered by tests/test_gpu_configs.py.
 """
 
 import numpy as np

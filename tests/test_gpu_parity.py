@@ -112,7 +112,8 @@ def test_fused_circuit_c128(backend, maxq):
 @pytest.mark.parametrize("n", [11, 14])
 @pytest.mark.parametrize("maxq", [4, 5])
 def test_fused_circuit_c64(backend64, maxq, n):
-    """n = 14 runs the 4- and 5-qubit blocks on MFMA (FP32 accumulation)."""
+    """The layered circuit grouped by fuse(4) / fuse(5): qibo's fusion forms 2-
+    and 3-qubit groups here (dense records in blocked passes, complex64)."""
     c = models.random_layered_circuit(n, 4, seed=8, measure=False)
     want = OracleBackend().execute(c)
     got = to_np(backend64.execute_circuit(c.fuse(max_qubits=maxq)).state())

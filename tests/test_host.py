@@ -275,3 +275,96 @@ def test_program_cache_keys_follow_gate_changes():
     u = gates.Unitary(np.eye(2), 0)
     be._gate_key(u)
     assert "_qkey" not in u.__dict__
+
+
+def test_program_cache_key_none_for_symbolic_gates():
+    """A gate whose angle follows a measurement outcome is never cached, also
+    after substitute_symbols turned its parameters into floats: the next run
+    substitutes a possibly different outcome (abstract.py:2584-2587)."""
+    import numpy as np
+
+    from qibo_amd import gates
+    from qibo_amd.backend import MI355XBackend
+
+    be = MI355XBackend.__new__(MI355XBackend)
+    m = gates.M(0, collapse=True)
+    g = gates.RX(1, theta=np.pi * m.result.symbols[0])
+    assert be._gate_key(g) is None
+    m.result.register_samples(np.array([[1]]))
+    g.substitute_symbols()
+    assert isinstance(g.parameters[0], float)
+    assert be._gate_key(g) is None
+
+
+def test_low_bits_resolved_and_checked():
+    import pytest
+
+    from qibo_amd import _lib, blocking
+    from qibo_amd.backend import MI355XBackend
+
+    be = MI355XBackend.__new__(MI355XBackend)
+    be.blocking_low_bits = None
+    assert be._low_bits(_lib.C128) == 3 and be._low_bits(_lib.C64) == 4
+    for ok in (0, 2, blocking.TILE_BITS - 1):
+        be.blocking_low_bits = ok
+        assert be._low_bits(_lib.C128) == ok
+    for bad in (-1, blocking.TILE_BITS, 2.0, True, "3"):
+        be.blocking_low_bits = bad
+        with pytest.raises(ValueError):
+            be._low_bits(_lib.C128)
+
+
+def test_standin_gates_planned_by_duck_type():
+    """Gates that are not qibo_amd classes but carry qibo's attributes (what
+    qibo itself passes after set_backend) are planned like the package's own;
+    a measurement is recognised and never asked for its matrix."""
+    import numpy as np
+    import pytest
+
+    from qibo_amd import gates, kinds
+    from qibo_amd.backend import GatePlanner, MI355XBackend
+    from tests import standins as si
+
+    gp = GatePlanner("complex128")
+    pairs = [(si.RY(2, 0.7), gates.RY(2, 0.7)), (si.RZ(0, 1.3), gates.RZ(0, 1.3)), (si.CZ(1, 3), gates.CZ(1, 3)),
+             (si.H(1), gates.H(1)), (si.CNOT(3, 0), gates.CNOT(3, 0)),
+             (si.H(2).controlled_by(0, 1), gates.H(2).controlled_by(0, 1))]
+    for foreign, own in pairs:
+        a, b = gp.plan_gate(foreign), gp.plan_gate(own)
+        assert (a.kind, a.targets, a.controls) == (b.kind, b.targets, b.controls)
+        assert np.array_equal(a.data, b.data)
+    m = si.M(0, 1, 2)
+    assert kinds.is_measurement(m) and not kinds.runs_outside_passes(m)
+    with pytest.raises(ValueError):
+        gp.plan_gate(m)
+    fg, own = si.FusedGate(0, 2), gates.FusedGate(0, 2)
+    for g, h in ((si.H(0), gates.H(0)), (si.CZ(0, 2), gates.CZ(0, 2)), (si.RY(2, 0.3), gates.RY(2, 0.3))):
+        fg.append(g)
+        own.append(h)
+    assert kinds.is_fused(fg) and not kinds.is_channel(fg)
+    assert np.array_equal(gp.matrix_fused(fg), gp.matrix_fused(own))
+    a = gp.plan_gate(fg)
+    assert a.kind == gp.plan_gate(own).kind and np.array_equal(a.data, gp.plan_gate(own).data)
+    be = MI355XBackend.__new__(MI355XBackend)
+    assert be._gate_key(m) == ("M",)
+    assert be._gate_key(si.M(0, collapse=True)) is None
+    assert be._gate_key(si.RY(2, 0.7)) == be._gate_key(gates.RY(2, 0.7))
+    assert be._gate_key(fg) is not None
+
+
+def test_standin_measurement_bookkeeping():
+    """Result bookkeeping on stand-in measurement gates: samples are
+    registered per gate in its qubit order and read back per register."""
+    import numpy as np
+
+    from qibo_amd.result import MeasurementOutcomes
+    from tests import standins as si
+
+    m1, m2 = si.M(0, 2, register_name="a"), si.M(1, register_name="b")
+    samples = np.array([[0, 1, 1], [1, 0, 0], [1, 1, 0]], dtype=np.int64)
+    out = MeasurementOutcomes([m1, m2], backend=None, samples=samples, nshots=3)
+    assert out.measurement_gate.qubits == (0, 2, 1)
+    assert np.array_equal(m1.result.samples(), samples[:, [0, 1]])
+    assert np.array_equal(m2.result.samples(), samples[:, [2]])
+    regs = out.samples(registers=True)
+    assert np.array_equal(regs["a"], samples[:, [0, 1]]) and np.array_equal(regs["b"], samples[:, [2]])
