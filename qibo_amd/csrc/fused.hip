@@ -274,6 +274,148 @@ static int launch_fused_mfma(void* state, int n, const int* targets, const void*
   return QSIM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// complex128 blocks of k = 4 or 5 qubits on FP64 MFMA (v_mfma_f64_16x16x4f64)
+// ---------------------------------------------------------------------------
+// Y = M X for 16 columns at a time (one wavefront), as three real products
+// (Gauss): P1 = Re M Re X, P2 = Im M Im X, P3 = (Re M + Im M)(Re X + Im X),
+// then Re Y = P1 - P2 and Im Y = (P3 - P1) - P2: 3 D x D real products
+// instead of the 4 of the real embedding. Each product is (D/16) row blocks
+// x (D/4) K-steps of 16x16x4 FP64 MFMA with FP64 accumulation.
+//
+// Operand layout of v_mfma_f64_16x16x4f64 (wave64):
+//   A (16 x 4):  lane l supplies A[l % 16][l / 16]
+//   B (4 x 16):  lane l supplies B[l / 16][l % 16]
+//   D (16 x 16): register r of lane l holds D[4 r + l / 16][l % 16]
+// (the f64 form interleaves the rows over the four lane groups, unlike the
+// f32 16x16x4 form). Lane (j = l % 16, q = l / 16) therefore loads the D/4
+// rows (D/4) q .. (D/4) q + D/4 - 1 of column j (K-step s uses row
+// (D/4) q + s) straight from HBM into registers, and stores rows
+// 16 mb + 4 r + q of the same column: no LDS staging of the state. The matrix fragments (A
+// operands) sit in LDS in lane order, read once per MFMA without bank
+// conflicts. A column group belongs to one wavefront, so the in-place update
+// needs no synchronisation: its loads complete before its stores.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+struct Mfma64Args {
+  uint64_t ngroups;        // groups of 16 columns: 2^(n-K) / 16
+  uint64_t off[32];        // index bits of matrix row r (row bit K-1-t <-> target t)
+  int nfixed;
+  uint8_t pos[kMaxFixed];  // target bit positions, ascending (for insert_zeros)
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, const c128* __restrict__ mat,
+                                                        c128* __restrict__ psi) {
+  constexpr int D = 1 << K;          // complex rows
+  constexpr int RPL = D / 4;         // rows per lane = K-steps per product
+  constexpr int MB = D / 16;         // 16-row output blocks per product
+  constexpr int NFRAG = 3 * MB * RPL;
+  __shared__ double frag[NFRAG * 64];
+  // fragment (p, mb, s), lane l: M_p[16 mb + l % 16][RPL (l / 16) + s] with
+  // M_0 = Re M, M_1 = Im M, M_2 = Re M + Im M
+  for (int e = threadIdx.x; e < NFRAG * 64; e += 256) {
+    const int l = e & 63, f = e >> 6;
+    const int st = f % RPL, mb = (f / RPL) % MB, p = f / (RPL * MB);
+    const c128 m = mat[(16 * mb + (l & 15)) * D + RPL * (l >> 4) + st];
+    frag[e] = p == 0 ? m.re : (p == 1 ? m.im : m.re + m.im);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 15, q = lane >> 4;
+  // row offsets are OR-linear in the row bits: off[RPL q + s] = off[RPL q] | off[s]
+  const uint64_t qin = a.off[RPL * q];
+  const uint64_t qout = a.off[q];
+  // XCD-contiguous work split: workgroup b runs on XCD b % 8; each XCD walks one
+  // contiguous eighth of the column groups, its wavefronts interleaved
+  const uint32_t nxcd = gridDim.x >= 8 ? 8u : 1u;
+  const uint32_t xcd = blockIdx.x % nxcd;
+  const uint64_t wpx = uint64_t(gridDim.x / nxcd) * 4;  // wavefronts per XCD
+  const uint64_t w = uint64_t(blockIdx.x / nxcd) * 4 + (threadIdx.x >> 6);
+  const uint64_t chunk = (a.ngroups + nxcd - 1) / nxcd;
+  const uint64_t gend0 = (uint64_t(xcd) + 1) * chunk;
+  const uint64_t gend = gend0 < a.ngroups ? gend0 : a.ngroups;
+  uint64_t g = uint64_t(xcd) * chunk + w;
+  c128 x[RPL];
+  if (g < gend) {
+    const uint64_t base = insert_zeros(16 * g + j, a.pos, a.nfixed) | qin;
+#pragma clang loop unroll(full)
+    for (int s = 0; s < RPL; ++s) x[s] = ld_nt(psi, base | a.off[s]);
+  }
+  while (g < gend) {
+    const uint64_t gn = g + wpx;
+    c128 xn[RPL];
+    if (gn < gend) {  // the next group's loads overlap this group's products
+      const uint64_t bn = insert_zeros(16 * gn + j, a.pos, a.nfixed) | qin;
+#pragma clang loop unroll(full)
+      for (int s = 0; s < RPL; ++s) xn[s] = ld_nt(psi, bn | a.off[s]);
+    }
+    doublex4 acc[3][MB];
+#pragma clang loop unroll(full)
+    for (int p = 0; p < 3; ++p)
+#pragma clang loop unroll(full)
+      for (int mb = 0; mb < MB; ++mb) acc[p][mb] = doublex4(0.0);
+#pragma clang loop unroll(full)
+    for (int s = 0; s < RPL; ++s) {
+      const double b0 = x[s].re, b1 = x[s].im, b2 = x[s].re + x[s].im;
+#pragma clang loop unroll(full)
+      for (int mb = 0; mb < MB; ++mb) {
+        acc[0][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((0 * MB + mb) * RPL + s) * 64 + lane], b0,
+                                                          acc[0][mb], 0, 0, 0);
+        acc[1][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((1 * MB + mb) * RPL + s) * 64 + lane], b1,
+                                                          acc[1][mb], 0, 0, 0);
+        acc[2][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((2 * MB + mb) * RPL + s) * 64 + lane], b2,
+                                                          acc[2][mb], 0, 0, 0);
+      }
+    }
+    const uint64_t obase = insert_zeros(16 * g + j, a.pos, a.nfixed) | qout;
+#pragma clang loop unroll(full)
+    for (int mb = 0; mb < MB; ++mb)
+#pragma clang loop unroll(full)
+      for (int r = 0; r < 4; ++r) {
+        c128 y;
+        y.re = acc[0][mb][r] - acc[1][mb][r];
+        y.im = (acc[2][mb][r] - acc[0][mb][r]) - acc[1][mb][r];
+        st_nt(psi, obase | a.off[16 * mb + 4 * r], y);
+      }
+#pragma clang loop unroll(full)
+    for (int s = 0; s < RPL; ++s) x[s] = xn[s];
+    g = gn;
+  }
+}
+
+template <int K>
+static int launch_fused_mfma_c128(void* state, int n, const int* targets, const void* mat, hipStream_t s) {
+  Mfma64Args a;
+  std::memset(&a, 0, sizeof(a));
+  a.nfixed = sorted_positions(n, targets, K, nullptr, 0, a.pos);
+  a.ngroups = (uint64_t(1) << (n - K)) / 16;
+  for (int r = 0; r < (1 << K); ++r) {
+    uint64_t off = 0;
+    for (int t = 0; t < K; ++t)
+      if ((r >> (K - 1 - t)) & 1) off |= uint64_t(1) << (n - 1 - targets[t]);
+    a.off[r] = off;
+  }
+  // persistent grid of the workgroups the device keeps resident (a multiple of 8)
+  static const unsigned resident = [] {
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_mfma_c128<K>, 256, 0) != hipSuccess || nb < 1)
+      nb = 2;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    return unsigned(nb) * unsigned(cus);
+  }();
+  const uint64_t want = (a.ngroups + 3) / 4;  // one group per wavefront at least
+  unsigned grid = unsigned(want < resident ? want : resident);
+  if (grid >= 8) grid &= ~7u;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_fused_mfma_c128<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c128*>(mat),
+                     reinterpret_cast<c128*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -287,7 +429,11 @@ extern "C" int qsim_apply_fused(void* state, int nqubits, int dtype, const int* 
   if (!validate_qubits(nqubits, targets, ntargets, nullptr, 0, "qsim_apply_fused")) return QSIM_EINVAL;
   QSIM_CHECK_ARG(nqubits >= ntargets, "qsim_apply_fused: more targets than qubits");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (dtype == QSIM_C128) return dispatch_fused<double>(state, nqubits, targets, ntargets, matrix_dev, s);
+  if (dtype == QSIM_C128) {
+    if (nqubits >= 9 && ntargets == 5) return launch_fused_mfma_c128<5>(state, nqubits, targets, matrix_dev, s);
+    if (nqubits >= 8 && ntargets == 4) return launch_fused_mfma_c128<4>(state, nqubits, targets, matrix_dev, s);
+    return dispatch_fused<double>(state, nqubits, targets, ntargets, matrix_dev, s);
+  }
   if (nqubits >= kMfmaTileBits && ntargets == 5) return launch_fused_mfma<5>(state, nqubits, targets, matrix_dev, s);
   if (nqubits >= kMfmaTileBits && ntargets == 4) return launch_fused_mfma<4>(state, nqubits, targets, matrix_dev, s);
   return dispatch_fused<float>(state, nqubits, targets, ntargets, matrix_dev, s);
