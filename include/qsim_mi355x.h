@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 6
+#define QSIM_ABI_VERSION 7
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -256,6 +256,23 @@ size_t qsim_expval_workspace(int nqubits, int nterms);
 int qsim_expval_pauli(const void* state, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
                       const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* qsim_expval_pauli without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The per-block partial sums go to a file-scope device
+ * array of qsim_expval_noalloc_max_groups() x 1024 doubles (64 groups,
+ * 512 KiB), one copy per GPU held until the process ends. Term tables that
+ * need more groups (a group = the terms of one X mask, 32 at most) are
+ * rejected with QSIM_EINVAL. The stream must belong to the current device.
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one
+ * GPU (other streams, other threads) race, so callers serialise them.
+ * Same kernels and summation order as qsim_expval_pauli: the results are
+ * bitwise equal. */
+int qsim_expval_pauli_noalloc(const void* state, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
+                              const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
+                              void* stream);
+int qsim_expval_noalloc_max_groups(void);
 
 /* out = sum_t coeffs[t] * i^{ny_t} X^{xmask_t} Z^{zmask_t} |psi>  (out of
  * place, out must not alias psi). The Hamiltonian-vector product that seeds the

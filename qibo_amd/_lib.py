@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
@@ -48,6 +48,8 @@ SIGNATURES = {
     "qsim_histogram": (_i, [_vp, _i64, _vp, _i64, _vp]),
     "qsim_expval_workspace": (_sz, [_i, _i]),
     "qsim_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "qsim_expval_pauli_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qsim_expval_noalloc_max_groups": (_i, []),
     "qsim_apply_pauli_sum": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "qsim_inner_matrix": (_i, [_vp, _vp, _i, _i, _c_int_p, _i, _c_int_p, _i, _vp, _vp, _vp, _sz, _vp]),
     "qsim_blocked_header_bytes": (_sz, []),

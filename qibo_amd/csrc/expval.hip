@@ -18,8 +18,8 @@
 
 #include <algorithm>
 #include <cstring>
-#include <map>
-#include <vector>
+#include <mutex>
+#include <string>
 
 #include "qsim_common.h"
 
@@ -113,37 +113,60 @@ static int grid_for(uint64_t nwork) {
   return int(b < kEBlocks ? (b < 1 ? 1 : b) : kEBlocks);
 }
 
-struct TermRef {
-  uint64_t z;
-  double c;
-  int ny;
-};
-
-// Split terms into launches: grouped by x mask, at most kMaxGroupTerms each.
-static void plan_groups(int nterms, const uint64_t* xm, const uint64_t* zm, const int* ny, const double* c,
-                        std::vector<std::pair<uint64_t, std::vector<TermRef>>>* out) {
-  std::map<uint64_t, std::vector<TermRef>> g;
-  for (int t = 0; t < nterms; ++t) g[xm[t]].push_back({zm[t], c[t], ny[t]});
-  for (auto& kv : g) {
-    for (size_t s = 0; s < kv.second.size(); s += kMaxGroupTerms) {
-      const size_t e = std::min(kv.second.size(), s + kMaxGroupTerms);
-      out->push_back({kv.first, std::vector<TermRef>(kv.second.begin() + s, kv.second.begin() + e)});
+// The launches of a term table, in the order every entry point uses: the
+// distinct x masks in ascending order, each split into chunks of at most
+// kMaxGroupTerms terms in table order. Found by repeated minimum scans, so no
+// container is allocated (the no-allocation entry point shares it).
+template <typename F>
+static void for_each_group(int nterms, const uint64_t* xm, const uint64_t* zm, const int* ny, const double* c,
+                           F&& visit) {
+  bool have = false;
+  uint64_t cur = 0;
+  for (int t = 0; t < nterms; ++t)
+    if (!have || xm[t] < cur) cur = xm[t], have = true;
+  while (have) {
+    PauliGroupArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = cur;
+    for (int t = 0; t < nterms; ++t) {
+      if (xm[t] != cur) continue;
+      a.z[a.nterms] = zm[t];
+      a.c[a.nterms] = c[t];
+      a.ny[a.nterms] = ny[t];
+      if (++a.nterms == kMaxGroupTerms) {
+        visit(a);
+        a.nterms = 0;
+      }
     }
+    if (a.nterms > 0) visit(a);
+    bool next = false;
+    uint64_t nxt = 0;
+    for (int t = 0; t < nterms; ++t)
+      if (xm[t] > cur && (!next || xm[t] < nxt)) nxt = xm[t], next = true;
+    have = next;
+    cur = nxt;
   }
 }
 
+// Launches the group passes into partial[0 ..) and the final fixed-order sum;
+// the partial buffer must hold ngroups * kEBlocks doubles (checked first).
 template <typename R>
 static int expval_impl(const void* state, int n, int nterms, const uint64_t* xm, const uint64_t* zm,
-                       const int* ny, const double* c, double* out, void* ws, size_t ws_bytes, hipStream_t s) {
-  std::vector<std::pair<uint64_t, std::vector<TermRef>>> groups;
-  plan_groups(nterms, xm, zm, ny, c, &groups);
-  QSIM_CHECK_ARG(groups.size() * kEBlocks * sizeof(double) <= ws_bytes, "qsim_expval_pauli: workspace too small");
+                       const int* ny, const double* c, double* out, void* ws, size_t ws_bytes, hipStream_t s,
+                       const char* who) {
+  size_t ngroups = 0;
+  for_each_group(nterms, xm, zm, ny, c, [&](const PauliGroupArgs&) { ++ngroups; });
+  if (ngroups * kEBlocks * sizeof(double) > ws_bytes) {
+    set_error(std::string(who) + ": workspace too small for " + std::to_string(ngroups) +
+              " term groups (distinct X masks, 32 terms per group)");
+    return QSIM_EINVAL;
+  }
   double* partial = reinterpret_cast<double*>(ws);
   int64_t used = 0;
-  for (auto& grp : groups) {
-    PauliGroupArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.x = grp.first;
+  int rc = QSIM_OK;
+  for_each_group(nterms, xm, zm, ny, c, [&](const PauliGroupArgs& g) {
+    if (rc != QSIM_OK) return;
+    PauliGroupArgs a = g;
     if (a.x == 0) {
       a.nwork = uint64_t(1) << n;
     } else {
@@ -151,18 +174,18 @@ static int expval_impl(const void* state, int n, int nterms, const uint64_t* xm,
       a.hbit = uint64_t(1) << a.hpos;
       a.nwork = uint64_t(1) << (n - 1);
     }
-    a.nterms = int(grp.second.size());
-    for (int t = 0; t < a.nterms; ++t) {
-      a.z[t] = grp.second[t].z;
-      a.c[t] = grp.second[t].c;
-      a.ny[t] = grp.second[t].ny;
-    }
     const int blocks = grid_for(a.nwork);
     hipLaunchKernelGGL(k_pauli_group<R>, dim3(blocks), dim3(kET), 0, s, reinterpret_cast<const cplx<R>*>(state), a,
                        partial + used);
-    QSIM_CHECK_LAUNCH();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error(std::string(who) + ": launch failed: " + hipGetErrorString(e));
+      rc = QSIM_EHIP;
+      return;
+    }
     used += blocks;
-  }
+  });
+  if (rc != QSIM_OK) return rc;
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kET), 0, s, partial, used, out);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -273,6 +296,58 @@ static int launch_inner(const void* bra, const void* ket, int n, const int* targ
   return QSIM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch for callers that cannot allocate (qsim_expval_pauli_noalloc): the
+// per-block partial sums of up to kNoAllocGroups term groups in a file-scope
+// device array. Each GPU holds its own copy (512 KiB) until the process ends;
+// its address on a device comes from hipGetSymbolAddress, taken once per
+// device with that device current. The array's name in host code is only a
+// handle for that call, never a device address.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocGroups = 64;
+constexpr int kNoAllocDevices = 64;
+__device__ __align__(256) double scratch[kNoAllocGroups * kEBlocks];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kNoAllocDevices];  // per device, set once under g_scratch_mu
+
+static int scratch_on_current_device(void* stream, double** out, const char* who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error(std::string(who) + ": hipGetDevice: " + hipGetErrorString(e));
+    return QSIM_EHIP;
+  }
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error(std::string(who) + ": device ordinal out of range");
+    return QSIM_EINVAL;
+  }
+  if (stream != nullptr) {
+    hipDevice_t sdev = -1;
+    e = hipStreamGetDevice(reinterpret_cast<hipStream_t>(stream), &sdev);
+    if (e != hipSuccess) {
+      set_error(std::string(who) + ": hipStreamGetDevice: " + hipGetErrorString(e));
+      return QSIM_EHIP;
+    }
+    if (int(sdev) != dev) {
+      set_error(std::string(who) + ": the stream belongs to another device than the current one");
+      return QSIM_EINVAL;
+    }
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string(who) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+      return QSIM_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = reinterpret_cast<double*>(g_scratch_ptr[dev]);
+  return QSIM_OK;
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -298,9 +373,44 @@ int qsim_expval_pauli(const void* state, int nqubits, int dtype, int nterms, con
     QSIM_CHECK_ARG((xmasks[t] & ~full) == 0 && (zmasks[t] & ~full) == 0, "qsim_expval_pauli: mask out of range");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128)
-    return expval_impl<double>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, workspace, workspace_bytes, s);
-  return expval_impl<float>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, workspace, workspace_bytes, s);
+    return expval_impl<double>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, workspace, workspace_bytes, s,
+                               "qsim_expval_pauli");
+  return expval_impl<float>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, workspace, workspace_bytes, s,
+                            "qsim_expval_pauli");
 }
+
+int qsim_expval_pauli_noalloc(const void* state, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
+                              const uint64_t* zmasks, const int* ny, const double* coeffs, double* out,
+                              void* stream) {
+  clear_error();
+  const char* who = "qsim_expval_pauli_noalloc";
+  QSIM_CHECK_ARG(state != nullptr && out != nullptr, "qsim_expval_pauli_noalloc: null pointer");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_expval_pauli_noalloc: bad dtype");
+  QSIM_CHECK_ARG(nqubits >= 1 && nqubits <= QSIM_MAX_QUBITS, "qsim_expval_pauli_noalloc: nqubits out of range");
+  QSIM_CHECK_ARG(nterms >= 1 && xmasks && zmasks && ny && coeffs, "qsim_expval_pauli_noalloc: bad term table");
+  const uint64_t full = (uint64_t(1) << nqubits) - 1;
+  for (int t = 0; t < nterms; ++t)
+    QSIM_CHECK_ARG((xmasks[t] & ~full) == 0 && (zmasks[t] & ~full) == 0,
+                   "qsim_expval_pauli_noalloc: mask out of range");
+  size_t ngroups = 0;
+  for_each_group(nterms, xmasks, zmasks, ny, coeffs, [&](const PauliGroupArgs&) { ++ngroups; });
+  if (ngroups > size_t(kNoAllocGroups)) {
+    set_error("qsim_expval_pauli_noalloc: " + std::to_string(ngroups) + " term groups (distinct X masks, 32 terms " +
+              "per group) exceed the maximum of " + std::to_string(kNoAllocGroups) +
+              "; use qsim_expval_pauli with a workspace");
+    return QSIM_EINVAL;
+  }
+  double* ptr = nullptr;
+  const int rc = scratch_on_current_device(stream, &ptr, who);
+  if (rc != QSIM_OK) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(double) * size_t(kNoAllocGroups) * size_t(kEBlocks);
+  if (dtype == QSIM_C128)
+    return expval_impl<double>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, ptr, bytes, s, who);
+  return expval_impl<float>(state, nqubits, nterms, xmasks, zmasks, ny, coeffs, out, ptr, bytes, s, who);
+}
+
+int qsim_expval_noalloc_max_groups(void) { return kNoAllocGroups; }
 
 int qsim_apply_pauli_sum(const void* state, int nqubits, int dtype, int nterms, const uint64_t* xmasks,
                          const uint64_t* zmasks, const int* ny, const double* coeffs, void* out, void* stream) {
