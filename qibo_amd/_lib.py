@@ -12,7 +12,10 @@ import os
 import threading
 
 ABI_VERSION = 7
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqsim_mi355x.so")
+# QIBO_AMD_LIB names another build of the library (an A/B comparison of two
+# builds on one device); by default the one next to this file
+LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                          "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
 F64, F32 = 0, 1
