@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 7
+#define QSIM_ABI_VERSION 8
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -345,6 +345,46 @@ int qsim_dm_expval_pauli(const void* rho, int nqubits, int dtype, int nterms, co
  * No device work. */
 int qsim_plan_pass(int nitems, const uint64_t* nd_masks, const uint64_t* all_masks, const uint8_t* blockable,
                    int nqubits, int low_bits, int tile_bits, int ordered, uint8_t* taken, uint64_t* tile_mask);
+
+/* Record compiler of one blocked pass, host code: the records that
+ * qsim_apply_blocked runs for `nops` planned gates on the 12 ascending
+ * `tile_bits` (the gate loop of Backend._execute_circuit, abstract.py:3321-3322,
+ * for one pass; qibo_amd.blocking.compile_pass is its Python restatement).
+ * Gate i is op_desc[44 i ..]: kind (0 identity, 1 diagonal, 2 SWAP, 3 dense),
+ * number of targets (<= 5; dense 1..3), number of controls (<= 35), unused,
+ * 5 target slots, 35 control slots (qibo qubit numbers, targets in matrix
+ * order); its complex128 entries (diagonal: 2^k, dense: 4^k row-major)
+ * start at op_data[2 data_offsets[i]] (-1: none). Records are written in the
+ * state dtype's precision to `out` (capacity >= qsim_compile_pass_capacity
+ * (nops, dtype) bytes): *out_bytes bytes, *count records, *switches mapping
+ * changes. With `ordered` only the exact per-gate forms are used. No device
+ * work. */
+size_t qsim_compile_pass_capacity(int nops, int dtype);
+int qsim_compile_pass(int nqubits, const int* tile_bits, int nops, const int* op_desc, const double* op_data,
+                      const int64_t* data_offsets, int dtype, int ordered, uint8_t* out, size_t capacity,
+                      size_t* out_bytes, int* count, int* switches);
+
+/* Neighbour-fusion planner of Circuit.fuse (circuit.py:954-1003, with
+ * _Queue.to_fused / from_fused :55-95 and FusedGate.can_fuse / fuse
+ * gates/special.py:81-153), host code. Gate i of the queue acts on the qubits
+ * of qubit_masks[i] (bit q = qibo qubit q); kind[i] = 0 for a gate, 1 for a
+ * measurement, 2 for a special gate (it blocks every qubit). Writes per gate
+ * status[i] (0: absorbed into another group, 1: kept as itself, 2: a fused
+ * group) and, for status 2, the group's gates in application order at
+ * members[member_offsets[i] .. member_offsets[i + 1]) (members holds ngates
+ * ints, member_offsets ngates + 1). No device work. */
+int qsim_fuse_plan(int nqubits, int ngates, const uint64_t* qubit_masks, const uint8_t* kind, int max_qubits,
+                   int* status, int* members, int* member_offsets);
+
+/* Product of a fused group's gate matrices on its `rank` sorted `qubits`
+ * (Backend.matrix_fused, abstract.py:2680-2717), host code: starting from
+ * the identity, each gate is applied in order to every column. Gate k is
+ * gate_desc[(2 + rank) k ..]: number of controlled_by controls, number of
+ * targets, the controls, the targets (qibo qubit numbers, targets in matrix
+ * order); its 2^nt x 2^nt complex128 matrix starts at mats[2 mat_offsets[k]].
+ * `out` receives the 2^rank x 2^rank complex128 product, row-major. */
+int qsim_matrix_fused(int rank, const int* qubits, int ngates, const int* gate_desc, const double* mats,
+                      const int64_t* mat_offsets, double* out);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,8 @@ SOURCES = {
     "blocked.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"],
     "density.hip": ["-ffp-contract=off"],
     "planner.hip": [],
+    "compile.hip": ["-ffp-contract=off"],
+    "fuse.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

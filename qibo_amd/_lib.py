@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 # QIBO_AMD_LIB names another build of the library (an A/B comparison of two
 # builds on one device); by default the one next to this file
 LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -69,6 +69,10 @@ SIGNATURES = {
     "qsim_dm_expval_workspace": (_sz, [_i]),
     "qsim_dm_expval_pauli": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "qsim_plan_pass": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp]),
+    "qsim_compile_pass_capacity": (_sz, [_i, _i]),
+    "qsim_fuse_plan": (_i, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "qsim_matrix_fused": (_i, [_i, _c_int_p, _i, _vp, _vp, _vp, _vp]),
+    "qsim_compile_pass": (_i, [_i, _c_int_p, _i, _vp, _vp, _vp, _i, _i, _vp, _sz, _vp, _vp, _vp]),
     "qsim_apply_bitflips": (_i, [_vp, _i64, _i, _vp, _vp, _vp, _vp, _vp]),
 }
 

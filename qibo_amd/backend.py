@@ -55,9 +55,12 @@ _SWAP4 = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
 class _Op:
     """A planned kernel call for one gate."""
 
-    __slots__ = ("kind", "targets", "controls", "data")
+    __slots__ = ("kind", "targets", "controls", "data", "native", "masks", "dev")
 
     def __init__(self, kind, targets, controls, data):
+        self.native = None  # flat description for the native record compiler (blocking._desc)
+        self.masks = None  # bit masks for the pass selection (blocking._masks)
+        self.dev = None  # (device, dtype, tensor): a dense block's matrix uploaded once
         self.kind = kind          # "noop" | "diag" | "swap" | "dense" | "fused"
         self.targets = tuple(targets)
         self.controls = tuple(controls)
@@ -205,14 +208,45 @@ class GatePlanner:
         return np.asarray(_matrix)
 
     def matrix_fused(self, fgate):
-        """Product of the group's matrices embedded on its sorted qubits.
+        """Product of the group's matrices on its sorted qubits
+        (abstract.py:2680-2717), formed in the library (qsim_matrix_fused, host
+        code): each gate matrix, with its ``controlled_by`` controls as a block
+        diagonal, is applied to every column of the running product. The gate
+        matrices are this planner's formulas; the product is FP64 and cast to
+        the backend dtype."""
+        from . import _lib
 
-        Same construction as qibo's matrix_fused (abstract.py:2680-2717): each
-        gate matrix (with its ``controlled_by`` controls as a block diagonal) is
-        extended by a Kronecker identity, its axes are permuted onto the group's
-        qubits, and the result left-multiplies the running product. Dense
-        numpy products replace the sparse ones (at most 32 x 32).
-        """
+        lib = _lib.load()
+        rank = len(fgate.target_qubits)
+        cdt = _DTYPES[self.dtype][1]
+        ng = len(fgate.gates)
+        desc = np.zeros((max(ng, 1), 2 + rank), dtype=np.int32)
+        mats, offs, off = [], np.zeros(max(ng, 1), dtype=np.int64), 0
+        for k, gate in enumerate(fgate.gates):
+            gm = np.ascontiguousarray(gate.matrix(self), dtype=np.complex128)
+            qubits = list(gate.qubits)
+            if len(gate.control_qubits) > 0 and len(gm) < 2 ** len(qubits):
+                ctrl, targ = list(gate.control_qubits), list(gate.target_qubits)
+            else:
+                ctrl, targ = [], qubits
+            if len(ctrl) + len(targ) > rank or len(gm) != 2 ** len(targ):
+                raise_error(ValueError, f"gate {gate} does not fit the fused group on {fgate.target_qubits}")
+            desc[k, 0], desc[k, 1] = len(ctrl), len(targ)
+            desc[k, 2: 2 + len(ctrl) + len(targ)] = ctrl + targ
+            mats.append(gm.ravel())
+            offs[k] = off
+            off += gm.size
+        data = np.concatenate(mats) if mats else np.zeros(1, dtype=np.complex128)
+        out = np.empty((2**rank, 2**rank), dtype=np.complex128)
+        _lib.check(lib.qsim_matrix_fused(rank, _lib.int_array(fgate.target_qubits), ng, desc.ctypes.data,
+                                         data.ctypes.data, offs.ctypes.data, out.ctypes.data), lib)
+        return out.astype(cdt)
+
+    def _matrix_fused_python(self, fgate):
+        """The reference's construction of matrix_fused in numpy: each gate
+        matrix (with its ``controlled_by`` controls as a block diagonal) is
+        extended by a Kronecker identity, its axes are permuted onto the
+        group's qubits, and the result left-multiplies the running product."""
         rank = len(fgate.target_qubits)
         cdt = _DTYPES[self.dtype][1]
         matrix = np.eye(2**rank, dtype=cdt)
@@ -239,11 +273,26 @@ class GatePlanner:
             raise_error(ValueError, "measurement gates have no matrix and are not planned as gates")
         if kinds.is_fused(gate):
             return plan_matrix(self.matrix_fused(gate), gate.target_qubits, ())
-        # gates that keep the base matrix() go straight to the backend formulas
-        m = self.matrix(gate) if type(gate).matrix in _BASE_MATRIX else gate.matrix(self)
+        # gates that keep the base matrix() go straight to the backend formulas;
+        # their plan is kept on the gate while its parameter and qubit tuples are
+        # the same objects (every setter assigns new ones) and no parameter is an
+        # array (which could change in place)
+        base = type(gate).matrix in _BASE_MATRIX
+        fp = None
+        if base:
+            fp = (gate._parameters, gate._target_qubits, gate._control_qubits, gate.is_controlled_by, self.dtype)
+            cached = gate.__dict__.get("_qplan")
+            if (cached is not None and cached[0] is fp[0] and cached[1] is fp[1] and cached[2] is fp[2]
+                    and cached[3] == fp[3] and cached[4] == fp[4]):
+                return cached[5]
+        m = self.matrix(gate) if base else gate.matrix(self)
         if gate.is_controlled_by:
-            return plan_matrix(m, gate.target_qubits, gate.control_qubits)
-        return plan_matrix(m, gate.qubits, ())
+            op = plan_matrix(m, gate.target_qubits, gate.control_qubits)
+        else:
+            op = plan_matrix(m, gate.qubits, ())
+        if base and not any(isinstance(x, (np.ndarray, torch.Tensor)) for x in gate._parameters):
+            gate.__dict__["_qplan"] = fp + (op,)
+        return op
 
     def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:
         """The map of a channel on one or two qubits T as a 4^k x 4^k matrix
@@ -564,8 +613,20 @@ class MI355XBackend(GatePlanner):
                 raise_error(NotImplementedError, "controlled blocks on more than 3 targets")
             if len(op.targets) > 5:
                 raise_error(NotImplementedError, "dense blocks on more than 5 qubits are not supported")
-            m = torch.from_numpy(np.ascontiguousarray(op.data, dtype=npdt)).to(state.device,
-                                                                                non_blocking=False)
+            # the block's matrix goes to the device once per op (a pinned,
+            # stream-ordered copy) and is reused when the op runs again
+            cached = op.dev
+            cur = torch.cuda.current_stream(state.device)
+            if cached is not None and cached[0] == state.device and cached[1] == npdt:
+                m = cached[2]
+                if cached[4] != cur.cuda_stream:
+                    cur.wait_event(cached[3])  # the upload ran on another stream
+            else:
+                host = torch.from_numpy(np.ascontiguousarray(op.data, dtype=npdt)).pin_memory()
+                m = host.to(state.device, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(cur)
+                op.dev = (state.device, npdt, m, ev, cur.cuda_stream)
             rc = self.lib.qsim_apply_fused(state.data_ptr(), nqubits, code, tg, len(op.targets),
                                            m.data_ptr(), stream)
             _keep_alive(m, state)

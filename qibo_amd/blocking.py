@@ -172,6 +172,9 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
     ``native`` runs the pass selection in the library (qsim_plan_pass);
     False runs the same rule in Python (``_scan``).
     """
+    if native:
+        yield from _partition_native(ops, nqubits, low_bits, ordered, window, singles)
+        return
     source = iter(ops)
     items = []
     counter = 0
@@ -218,6 +221,78 @@ def partition(ops: Iterable, nqubits: int, low_bits: int, ordered: bool = False,
         extra = [b for b in range(nqubits) if b not in tile]
         bits = sorted(bits + extra[: TILE_BITS - len(bits)])
         yield ("block", (bits, [c.op for c in chosen]))
+
+
+def _masks(op, nqubits):
+    """(non-diagonal target mask, mask of all bits, blockable) of a planned op
+    over amplitude bits, kept on the op per register size."""
+    cached = op.masks
+    if cached is None or cached[0] != nqubits:
+        al = 0
+        for q in op.targets:
+            al |= 1 << (nqubits - 1 - q)
+        nd = 0 if op.kind in ("diag", "innerd") else al
+        for q in op.controls:
+            al |= 1 << (nqubits - 1 - q)
+        cached = op.masks = (nqubits, nd, al, blockable(op))
+    return cached
+
+
+def _partition_native(ops, nqubits, low_bits, ordered, window, singles):
+    """partition() with the pass selection in the library (qsim_plan_pass):
+    the pending gates' bit masks are kept as plain integers beside them."""
+    from . import _lib
+
+    lib = _lib.load()
+    source = iter(ops)
+    pend, nd, al, ok = [], [], [], []
+
+    def fill():
+        while len(pend) < window:
+            op = next(source, None)
+            if op is None:
+                return
+            if op.kind == "noop":
+                continue
+            _, m_nd, m_al, m_ok = _masks(op, nqubits)
+            pend.append(op)
+            nd.append(m_nd)
+            al.append(m_al)
+            ok.append(m_ok)
+
+    fill()
+    tile = ctypes.c_uint64(0)
+    while pend:
+        if not ok[0] or nqubits < TILE_BITS:
+            yield ("single", pend[0])
+            del pend[0], nd[0], al[0], ok[0]
+            fill()
+            continue
+        nitems = len(pend)
+        nd_a = np.array(nd, dtype=np.uint64)
+        al_a = np.array(al, dtype=np.uint64)
+        ok_a = np.array(ok, dtype=np.uint8)
+        taken = np.empty(nitems, dtype=np.uint8)
+        rc = lib.qsim_plan_pass(nitems, nd_a.ctypes.data, al_a.ctypes.data, ok_a.ctypes.data, nqubits, low_bits,
+                                TILE_BITS, 1 if ordered else 0, taken.ctypes.data, ctypes.byref(tile))
+        if rc < 0:
+            _lib.check(rc, lib)
+        flags = taken.tolist()
+        chosen = [op for op, t in zip(pend, flags) if t]
+        keep = [i for i, t in enumerate(flags) if not t]
+        pend[:] = [pend[i] for i in keep]
+        nd[:] = [nd[i] for i in keep]
+        al[:] = [al[i] for i in keep]
+        ok[:] = [ok[i] for i in keep]
+        fill()
+        if len(chosen) == 1 and singles:
+            yield ("single", chosen[0])
+            continue
+        tm = tile.value
+        bits = [b for b in range(tm.bit_length()) if (tm >> b) & 1]
+        extra = [b for b in range(nqubits) if not (tm >> b) & 1]
+        bits = sorted(bits + extra[: TILE_BITS - len(bits)])
+        yield ("block", (bits, chosen))
 
 
 def record_bytes(nent: int, itemsize: int) -> int:
@@ -377,12 +452,69 @@ def _schedule(items, npred, succ, ready, runs_here):
                 ready.append(j)
 
 
+_KIND_CODE = {"noop": 0, "diag": 1, "swap": 2, "dense": 3}
+_DESC_INTS = 44  # kind, ntargets, ncontrols, -, targets[5], controls[35] (qsim_compile_pass)
+
+
+def _desc(op):
+    """(int32 row, complex128 entries or None) describing a planned op to the
+    native compiler; kept on the op (its fields never change after planning)."""
+    cached = op.native
+    if cached is None:
+        row = np.zeros(_DESC_INTS, dtype=np.int32)
+        kind = _KIND_CODE[op.kind]
+        nt, nc = len(op.targets), len(op.controls)
+        if nt > 5 or nc > 35:
+            raise ValueError(f"a blocked pass takes at most 5 targets and 35 controls per gate, got {nt} and {nc}")
+        row[0], row[1], row[2] = kind, nt, nc
+        row[4:4 + nt] = op.targets
+        row[9:9 + nc] = op.controls
+        data = None
+        if op.data is not None and op.kind in ("diag", "dense"):
+            data = np.ascontiguousarray(op.data, dtype=np.complex128).ravel()
+        cached = op.native = (row, data)
+    return cached
+
+
+def _compile_pass_native(tile_bits, ops, nqubits, cdtype, ordered):
+    from . import _lib
+
+    lib = _lib.load()
+    rows, datas, offs = [], [], []
+    off = 0
+    for op in ops:
+        row, data = _desc(op)
+        rows.append(row)
+        if data is None:
+            offs.append(-1)
+        else:
+            offs.append(off)
+            datas.append(data)
+            off += data.size
+    desc = np.concatenate(rows) if rows else np.zeros(_DESC_INTS, dtype=np.int32)
+    data = np.concatenate(datas) if datas else np.zeros(1, dtype=np.complex128)
+    offs = np.asarray(offs if offs else [-1], dtype=np.int64)
+    code = _lib.C64 if np.dtype(cdtype) == np.complex64 else _lib.C128
+    cap = lib.qsim_compile_pass_capacity(len(ops), code)
+    out = np.empty(cap, dtype=np.uint8)
+    nbytes, count, switches = ctypes.c_size_t(0), ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib.qsim_compile_pass(nqubits, _lib.int_array(tile_bits), len(ops), desc.ctypes.data, data.ctypes.data,
+                               offs.ctypes.data, code, 1 if ordered else 0, out.ctypes.data, cap,
+                               ctypes.byref(nbytes), ctypes.byref(count), ctypes.byref(switches))
+    _lib.check(rc, lib)
+    return out[: nbytes.value].tobytes(), count.value, switches.value
+
+
 def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
-                 ordered: bool = False) -> Tuple[bytes, int, int]:
+                 ordered: bool = False, native: bool = True) -> Tuple[bytes, int, int]:
     """Instruction records of one blocked pass; returns (bytes, count, map switches).
 
     With ``ordered`` the instructions keep the order of ``ops`` and use only
-    the exact per-gate forms (DIAG, DENSE1, DENSE_LDS, MAP)."""
+    the exact per-gate forms (DIAG, DENSE1, DENSE_LDS, MAP). ``native`` runs
+    the compiler in the library (qsim_compile_pass, the same records);
+    False runs this Python restatement of it."""
+    if native:
+        return _compile_pass_native(tile_bits, ops, nqubits, cdtype, ordered)
     local = {b: i for i, b in enumerate(tile_bits)}
     items = [_Item(op, nqubits, i) for i, op in enumerate(ops)]
     npred, succ = _dependencies(items, ordered)
@@ -537,9 +669,10 @@ class BatchOp:
     sweep's Re <lambda| M |psi> for a 2x2 / diagonal M, accumulated into
     gradient slot ``slot``)."""
 
-    __slots__ = ("kind", "targets", "controls", "data", "slot")
+    __slots__ = ("kind", "targets", "controls", "data", "slot", "masks")
 
     def __init__(self, kind, targets, controls, data, slot=-1):
+        self.masks = None  # bit masks for the pass selection (_masks)
         self.kind = kind
         self.targets = tuple(targets)
         self.controls = tuple(controls)

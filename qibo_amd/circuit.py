@@ -304,7 +304,48 @@ class Circuit:
 
     # -- transformation ----------------------------------------------------
     def fuse(self, max_qubits: int = 2):
-        """Greedy neighbour fusion into ``FusedGate`` groups of <= ``max_qubits``."""
+        """Greedy neighbour fusion into ``FusedGate`` groups of <= ``max_qubits``
+        (circuit.py:954-1003): the grouping is planned in the library
+        (qsim_fuse_plan, host code); ``_fuse_python`` is the same planner in
+        Python, kept as its restatement."""
+        from . import _lib
+        from .kinds import is_measurement, is_special
+
+        lib = _lib.load()
+        n = self.nqubits
+        gates_in = list(self.queue)
+        ng = len(gates_in)
+        masks = np.zeros(max(ng, 1), dtype=np.uint64)
+        kind = np.zeros(max(ng, 1), dtype=np.uint8)
+        for i, g in enumerate(gates_in):
+            m = 0
+            for q in g.qubits:
+                m |= 1 << int(q)
+            masks[i] = m
+            kind[i] = 1 if is_measurement(g) else (2 if (is_special(g) or isinstance(g, gates.SpecialGate)) else 0)
+        status = np.zeros(max(ng, 1), dtype=np.int32)
+        members = np.zeros(max(ng, 1), dtype=np.int32)
+        offsets = np.zeros(ng + 1, dtype=np.int32)
+        _lib.check(lib.qsim_fuse_plan(n, ng, masks.ctypes.data, kind.ctypes.data, int(max_qubits),
+                                      status.ctypes.data, members.ctypes.data, offsets.ctypes.data), lib)
+        queue = _Queue(n)
+        st, mem, off = status.tolist(), members.tolist(), offsets.tolist()
+        for i, g in enumerate(gates_in):
+            if st[i] == 1:
+                queue.append(g)
+            elif st[i] == 2:
+                group = [gates_in[j] for j in mem[off[i]: off[i + 1]]]
+                qubits = sorted({q for h in group for q in h.qubits})
+                fgate = gates.FusedGate(*qubits)
+                for h in group:
+                    fgate.append(h)
+                queue.append(fgate)
+        circuit = self._shallow_copy()
+        circuit.queue = queue
+        return circuit
+
+    def _fuse_python(self, max_qubits: int = 2):
+        """Circuit.fuse's planner in Python, step for step as the reference."""
         queue = self.queue.to_fused()
         for gate in queue:
             if not gate.marked:
