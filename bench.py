@@ -8,24 +8,27 @@ and the sampled bitstrings, through the drop-in path ``circuit(nshots=...)``
 -> ``CircuitResult.samples()``. Inputs are synthetic (seeded angles); the
 state stays resident in HBM for the whole step.
 
-Reported: gate applications per second (``value``), the per-gate
-algorithmic HBM bandwidth of the whole step (the bytes a gate-by-gate
-execution must move, divided by the step time), the roofline of the dominant
-kernel (``k_blocked``: one blocked pass of many gates, 2 * 2^n * 16 bytes per
-launch) timed with HIP events on its stream inside the timed region, and a
-CPU baseline: the numpy oracle of qibo's NumpyBackend timed on a bounded
-sample of the same circuit's gates on this host.
+Reported: gate applications per second (``value``); the HBM bytes the
+step's kernels must move (one read and one write of the state per blocked
+pass, the first pass only writing; the probabilities and the CDF) over the
+step time; the roofline of the dominant kernel (``k_blocked``: one blocked
+pass of many gates, 2 * 2^n * 16 bytes per launch) timed with HIP events on
+its stream inside the timed region; and a CPU baseline: the numpy oracle of
+qibo's NumpyBackend timed on a bounded sample of the same circuit's gates on
+this host. The timed steps execute the same circuit again, so they replay the
+blocked passes compiled at the first execution (the backend's program cache);
+the same steps with the cache off -- planning and compiling every step -- are
+reported beside them.
 
 Beside the headline line the same JSON object carries the other parts of
 the metric, each on its own bounded workload:
 * "layered28": the same step at 28 qubits (20 layers, 100 000 shots);
 * "qft": qibo's QFT circuit at 30 qubits in complex128 and complex64;
 * "qft15": qibo's QFT at 15 qubits, complex128, microseconds per circuit;
-* "fused_c64": the layered circuit at 32 qubits in complex64 and complex128
-  (64 GiB state) through blocked passes; 24 dense 5-qubit blocks at 32 qubits
-  (complex64 on MFMA, complex128 in FP64) with ms per block, HBM GB/s and
-  TFLOP/s; infidelity and max-abs amplitude error of the fused complex64 state
-  against complex128 at 30 qubits;
+* "precision": complex64 and its ``fuse(max_qubits=5)`` form against the
+  complex128 path at 32 qubits (layered circuit, QFT, dense 5-qubit blocks on
+  MFMA), infidelity and max-abs amplitude error at 30 qubits, and the target
+  check (>= 2x at infidelity <= 1e-6) per workload, passed or failed;
 * "serving": a Poisson stream of layered circuits (n ~ U[10, 30], depth ~
   U[10, 50], half complex64, half <sum Z> / shots) through the continuous-
   batching server: circuits/s, p50 and p99 latency;
@@ -52,16 +55,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def gate_bytes(kind, ntargets, ncontrols, nqubits, itemsize):
-    """Algorithmic bytes of one planned gate: each changed amplitude read + written once."""
-    s = (2**nqubits) * itemsize
-    if kind == "noop":
-        return 0
-    if kind == "swap":
-        return 2 * s / 2 / (2**ncontrols)
-    return 2 * s / (2**ncontrols)
-
-
 def cpu_baseline(nqubits, nlayers, seed, budget_s):
     """Oracle (numpy NumpyBackend restatement) on a stratified gate sample, one process."""
     import numpy as np
@@ -69,10 +62,14 @@ def cpu_baseline(nqubits, nlayers, seed, budget_s):
     from oracle import OracleBackend
     from qibo_amd import models
 
+    blas_name = "unknown"
     try:
         from threadpoolctl import threadpool_info
 
-        blas_threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+        info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        blas_threads = max((i.get("num_threads", 1) for i in info), default=1)
+        if info:
+            blas_name = f"{info[0].get('internal_api')} {info[0].get('version')}"
     except Exception:  # noqa: BLE001
         blas_threads = 1
     c = models.random_layered_circuit(nqubits, nlayers, seed=seed, measure=False)
@@ -99,6 +96,8 @@ def cpu_baseline(nqubits, nlayers, seed, budget_s):
         "unit": "gate-apps/s",
         "cores": int(blas_threads),
         "kind": "port",
+        "host": {"os_cpu_count": os.cpu_count(), "blas": blas_name,
+                 "OPENBLAS_NUM_THREADS": os.environ.get("OPENBLAS_NUM_THREADS")},
         "sample": f"{done} gates of layer 0 at n={nqubits} complex128 ({', '.join(names)}); "
                   f"{elapsed:.1f} s; numpy {np.__version__} matmul with {blas_threads} BLAS threads, "
                   "transposes single-threaded",
@@ -140,10 +139,59 @@ def _block_circuit(n, nblocks, seed):
     return c
 
 
-def bench_fused(seed, n_time=32, n_fid=30, nlayers=20, nblocks=24):
-    """complex64 vs complex128: blocked passes on the layered circuit, and
-    dense 5-qubit blocks (complex64 on MFMA, complex128 FP64) with their
-    roofline; accuracy of fused complex64 against complex128."""
+def _time_forms(forms):
+    """Seconds of one execution of each (key, backend, circuit): one warm-up
+    run (allocator, program cache), then a timed run that reuses the state
+    block the warm-up left in torch's cache."""
+    import torch
+
+    out = {}
+    for key, backend, circ in forms:
+        circ._final_state = None
+        backend.execute_circuit(circ)
+        circ._final_state = None
+        r, dt = _timed(lambda: backend.execute_circuit(circ))
+        del r
+        circ._final_state = None
+        torch.cuda.empty_cache()
+        out[key] = dt
+    return out
+
+
+def _accuracy(b64, b128, c64_circ, c128_circ):
+    """qibo's infidelity 1 - |<a|b>|^2, the same for the normalised states and
+    the max-abs amplitude error of the complex64 execution of ``c64_circ``
+    against the complex128 execution of ``c128_circ`` (the same circuit,
+    possibly in another fused form), computed on the GPU."""
+    import torch
+
+    from qibo_amd.accuracy import compare_states
+
+    a = b64.execute_circuit(c64_circ).state()
+    c64_circ._final_state = None
+    b = b128.execute_circuit(c128_circ).state()
+    c128_circ._final_state = None
+    rep = compare_states(b64, a, b, reference="complex128 GPU execution (NumpyBackend to 1e-12)")
+    del a, b
+    torch.cuda.empty_cache()
+    return rep
+
+
+def bench_precision(seed, n_time=32, n_acc=30, nlayers=20, nblocks=24):
+    """complex64 and its fused form against the plain complex128 GPU path.
+
+    Three workloads, each timed at ``n_time`` qubits in every form and checked
+    for accuracy at ``n_acc`` qubits (the complex128 state of 2^32 amplitudes
+    is 64 GiB, the accuracy pair at 30 qubits 24 GiB):
+    * "layered": the layered RY/RZ/CZ generator, 20 layers -- unfused
+      complex128, unfused complex64, and ``circuit.fuse(max_qubits=5)`` in
+      complex64 (the configuration as written);
+    * "qft": qibo's QFT -- unfused complex128 against ``QFT.fuse(max_qubits=5)``
+      in complex64 (its seven 5-qubit groups run on MFMA);
+    * "blocks5": 24 Haar-random dense 5-qubit blocks -- complex64 on FP32 MFMA
+      against complex128 on FP64 MFMA, with ms per block, GB/s and TFLOP/s.
+    The check the target names -- the complex64 form at least 2x the complex128
+    path at infidelity <= 1e-6 -- is stated per workload, passed or failed."""
     import torch
 
     from qibo_amd import models
@@ -151,50 +199,69 @@ def bench_fused(seed, n_time=32, n_fid=30, nlayers=20, nblocks=24):
 
     b64 = MI355XBackend(dtype="complex64")
     b128 = MI355XBackend(dtype="complex128")
-    res = {"nqubits": n_time, "layers": nlayers}
-    circ = models.random_layered_circuit(n_time, nlayers, seed=seed, measure=False)
-    res["gates"] = circ.ngates
-    for key, backend in (("layered_c64_s", b64), ("layered_c128_s", b128)):
-        circ._final_state = None
-        backend.execute_circuit(circ)  # warm-up (allocator, first launch)
-        circ._final_state = None  # its state block stays in torch's cache for the timed run
-        r, dt = _timed(lambda: backend.execute_circuit(circ))
-        res[key] = dt
-        del r
-        circ._final_state = None
-        torch.cuda.empty_cache()
-    res["speedup_layered_c64_vs_c128"] = res["layered_c128_s"] / res["layered_c64_s"]
+    res = {"nqubits_timed": n_time, "nqubits_accuracy": n_acc}
+
+    # layered circuit, as configured
+    lay = models.random_layered_circuit(n_time, nlayers, seed=seed, measure=False)
+    lay_f = lay.fuse(max_qubits=5)
+    sizes = {}
+    for g in lay_f.queue:
+        sizes[len(g.qubits)] = sizes.get(len(g.qubits), 0) + 1
+    t = _time_forms([("c128_s", b128, lay), ("c64_s", b64, lay), ("c64_fused5_s", b64, lay_f)])
+    acc_c = models.random_layered_circuit(n_acc, nlayers, seed=seed, measure=False)
+    res["layered"] = {
+        "layers": nlayers, "gates": lay.ngates, "fused5_group_sizes": {str(k): v for k, v in sorted(sizes.items())},
+        **t,
+        "speedup_c64_vs_c128": t["c128_s"] / t["c64_s"],
+        "speedup_c64_fused5_vs_c128": t["c128_s"] / t["c64_fused5_s"],
+        "speedup_c64_fused5_vs_c64": t["c64_s"] / t["c64_fused5_s"],
+        "accuracy_c64": _accuracy(b64, b128, acc_c, acc_c),
+        "accuracy_c64_fused5": _accuracy(b64, b128, acc_c.fuse(max_qubits=5), acc_c),
+    }
+    del lay, lay_f
+
+    # QFT and its fused form
+    q = models.QFT(n_time)
+    qf = q.fuse(max_qubits=5)
+    sizes = {}
+    for g in qf.queue:
+        sizes[len(g.qubits)] = sizes.get(len(g.qubits), 0) + 1
+    t = _time_forms([("c128_s", b128, q), ("c64_s", b64, q), ("c64_fused5_s", b64, qf)])
+    qa = models.QFT(n_acc)
+    res["qft"] = {
+        "gates": q.ngates, "fused5_group_sizes": {str(k): v for k, v in sorted(sizes.items())}, **t,
+        "speedup_c64_vs_c128": t["c128_s"] / t["c64_s"],
+        "speedup_c64_fused5_vs_c128": t["c128_s"] / t["c64_fused5_s"],
+        "accuracy_c64": _accuracy(b64, b128, qa, qa),
+        "accuracy_c64_fused5": _accuracy(b64, b128, qa.fuse(max_qubits=5), qa),
+    }
+    del q, qf
+
     # dense 5-qubit blocks: one qsim_apply_fused launch each
     blocks = _block_circuit(n_time, nblocks, seed)
-    for key, backend, isz in (("blocks5_c64", b64, 8), ("blocks5_c128", b128, 16)):
-        blocks._final_state = None
-        backend.execute_circuit(blocks)
-        blocks._final_state = None  # (no empty_cache here: the timed run reuses the block)
-        r, dt = _timed(lambda: backend.execute_circuit(blocks))
-        del r
-        blocks._final_state = None
-        torch.cuda.empty_cache()
-        per = dt / nblocks
-        flops = 8 * 32 * 2**n_time  # 32 complex multiply-adds per amplitude
-        res[key] = {"s": dt, "blocks": nblocks, "ms_per_block": 1e3 * per,
-                    "hbm_GBs": 2 * 2**n_time * isz / per / 1e9, "TFLOPs": flops / per / 1e12}
-    res["blocks5_c64"]["mfma_peak_TFLOPs_fp32"] = 157.3
-    res["speedup_blocks5_c64_vs_c128"] = res["blocks5_c128"]["s"] / res["blocks5_c64"]["s"]
-    # accuracy at n_fid: fused complex64 against complex128
-    circ = models.random_layered_circuit(n_fid, nlayers, seed=seed, measure=False)
-    s64 = b64.execute_circuit(circ.fuse(max_qubits=5)).state()
-    circ._final_state = None
-    s128 = b128.execute_circuit(circ).state()
-    a = s64.to(torch.complex128)
-    overlap = complex(torch.vdot(a, s128).item())
-    na = float(torch.linalg.vector_norm(a).item()) ** 2
-    nb = float(torch.linalg.vector_norm(s128).item()) ** 2
-    res["nqubits_accuracy"] = n_fid
-    # fidelity of the normalised states; the complex64 norm drifts by ~1e-7
-    res["infidelity"] = float(1.0 - abs(overlap) ** 2 / (na * nb))
-    res["norm_c64_minus_1"] = na - 1.0
-    res["max_abs_err"] = float(torch.max(torch.abs(a - s128)).item())
-    del s64, s128, a
+    t = _time_forms([("c128_s", b128, blocks), ("c64_s", b64, blocks)])
+    blk = {"blocks": nblocks, **t, "speedup_c64_vs_c128": t["c128_s"] / t["c64_s"]}
+    flops = 8 * 32 * 2**n_time  # 32 complex multiply-adds per amplitude
+    for key, isz, peak in (("c64", 8, 157.3), ("c128", 16, 78.6)):
+        per = t[f"{key}_s"] / nblocks
+        blk[key] = {"ms_per_block": 1e3 * per, "hbm_GBs": 2 * 2**n_time * isz / per / 1e9,
+                    "TFLOPs": flops / per / 1e12, "mfma_peak_TFLOPs": peak}
+    ba = _block_circuit(n_acc, nblocks, seed)
+    blk["accuracy_c64"] = _accuracy(b64, b128, ba, ba)
+    res["blocks5"] = blk
+    del blocks, ba
+
+    checks = []
+    for name, key, acc_key in (("layered, fuse(max_qubits=5)", "layered", "accuracy_c64_fused5"),
+                               ("layered, unfused", "layered", "accuracy_c64"),
+                               ("QFT, fuse(max_qubits=5)", "qft", "accuracy_c64_fused5"),
+                               ("dense 5-qubit blocks", "blocks5", "accuracy_c64")):
+        w = res[key]
+        sp = w["speedup_c64_fused5_vs_c128"] if "fused5" in acc_key else w["speedup_c64_vs_c128"]
+        inf = w[acc_key]["infidelity"]
+        checks.append({"workload": name, "speedup_vs_c128": sp, "infidelity": inf,
+                       "passed": bool(sp >= 2.0 and abs(inf) <= 1e-6)})
+    res["target_c64_2x_at_1e-6"] = checks
     torch.cuda.empty_cache()
     return res
 
@@ -245,8 +312,21 @@ def bench_qft15(seed, reps=50):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     del r
+    # the same executions with the program cache off: every circuit is planned
+    # and compiled again on the host
+    backend.program_cache = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = backend.execute_circuit(c)
+        c._final_state = None
+    torch.cuda.synchronize()
+    dt_nc = (time.perf_counter() - t0) / reps
+    backend.program_cache = True
+    del r
     return {"nqubits": 15, "gates": c.ngates, "us_per_circuit": dt * 1e6, "gate_apps_per_s": c.ngates / dt,
-            "circuits_timed": reps}
+            "circuits_timed": reps, "program_cache": "replay of the compiled passes of the identical circuit",
+            "us_per_circuit_compiling_every_circuit": dt_nc * 1e6}
 
 
 def bench_layered28(seed, nshots=100000, steps=3):
@@ -447,16 +527,6 @@ def main():
     ngates = sum(1 for g in circuit.queue if g.__class__.__name__ != "M")
     itemsize = 16
 
-    # algorithmic bytes of one step
-    step_bytes = 0.0
-    for g in circuit.queue:
-        if g.__class__.__name__ == "M":
-            continue
-        op = backend.plan_gate(g)
-        step_bytes += gate_bytes(op.kind, len(op.targets), len(op.controls), n, itemsize)
-    probs_bytes = (2**n) * itemsize + (2**n) * 8
-    cumsum_bytes = 2 * (2**n) * 8
-    step_bytes += probs_bytes + cumsum_bytes
 
     def step():
         circuit._final_state = None
@@ -519,6 +589,11 @@ def main():
     backend.program_cache = True
 
     ms_per_step = 1000.0 * elapsed / args.steps
+    # HBM bytes the step's kernels move: each blocked pass reads and writes the
+    # state once (the first, from |0...0>, only writes it); the probabilities
+    # read the state and write 8 B per amplitude; the CDF reads and writes 8 B
+    npass = len(passes) / args.steps if passes else 0.0
+    step_bytes = (2 * npass - 1) * (2**n) * itemsize + (2**n) * (itemsize + 8) + (2**n) * 16
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
     # this same workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scales read off a
     # plain copy of one state buffer); only quoted for the workload it measured
@@ -561,11 +636,18 @@ def main():
             "nshots": args.nshots,
             "parallelism": "single GPU",
         },
-        "hbm_gbs_step": step_bytes / (elapsed / args.steps) / 1e9,
+        "hbm_GBs_step": {
+            "bytes_per_step": step_bytes,
+            "GBs": step_bytes / (elapsed / args.steps) / 1e9,
+            "counted": "per blocked pass one read and one write of the state (the first pass "
+                       "only writes), probabilities (16 B read + 8 B written per amplitude), "
+                       "CDF (8 B read + 8 B written per amplitude)",
+        },
         # the backend keeps the compiled passes of a repeated identical gate
         # queue (same gates, qubits and parameter values); every timed step
         # still runs all passes, probabilities and sampling on the GPU
         "host_program_cache": {
+            "timed_steps": "replay of the passes compiled at the first execution of the identical circuit",
             "hits_in_timed_steps": cache_hits,
             "ms_per_step_compiling_every_step": 1000.0 * elapsed_nocache / args.steps,
             "value_compiling_every_step": ngates * args.steps / elapsed_nocache,
@@ -601,7 +683,7 @@ def main():
     if not args.no_extras:
         chosen = None if args.extras is None else set(args.extras.split(","))
         for key, fn in (("layered28", bench_layered28), ("qft", bench_qft), ("qft15", bench_qft15),
-                        ("fused_c64", bench_fused), ("serving", bench_serving), ("adjoint", bench_adjoint),
+                        ("precision", bench_precision), ("serving", bench_serving), ("adjoint", bench_adjoint),
                         ("density", bench_density)):
             if chosen is not None and key not in chosen:
                 continue

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 8
+#define QSIM_ABI_VERSION 9
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -385,6 +385,19 @@ int qsim_fuse_plan(int nqubits, int ngates, const uint64_t* qubit_masks, const u
  * `out` receives the 2^rank x 2^rank complex128 product, row-major. */
 int qsim_matrix_fused(int rank, const int* qubits, int ngates, const int* gate_desc, const double* mats,
                       const int64_t* mat_offsets, double* out);
+
+/* ---- accuracy reports --------------------------------------------------
+ * Comparison of two states of the same size (either dtype each), accumulated
+ * in FP64: out[0] + i out[1] = <a|b> = sum conj(a_i) b_i, out[2] = <a|a>,
+ * out[3] = <b|b>, out[4] = max_i |a_i - b_i| (`out`: 5 device doubles).
+ * qibo's pure-state fidelity is |<a|b>|^2 and its infidelity 1 - fidelity
+ * (src/qibo/quantum_info/metrics.py:200-291); the package reports them with
+ * the max-abs amplitude error for a complex64 result against the complex128
+ * execution of the same circuit. A fixed reduction order: repeated calls are
+ * bitwise equal. `workspace` must hold qsim_compare_workspace(nqubits) bytes. */
+size_t qsim_compare_workspace(int nqubits);
+int qsim_compare_states(const void* a, int a_dtype, const void* b, int b_dtype, int nqubits, double* out,
+                        void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ SOURCES = {
     "planner.hip": [],
     "compile.hip": ["-ffp-contract=off"],
     "fuse.hip": ["-ffp-contract=off"],
+    "compare.hip": [],
 }
 
 COMMON = [

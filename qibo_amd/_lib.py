@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 # QIBO_AMD_LIB names another build of the library (an A/B comparison of two
 # builds on one device); by default the one next to this file
 LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -74,6 +74,8 @@ SIGNATURES = {
     "qsim_matrix_fused": (_i, [_i, _c_int_p, _i, _vp, _vp, _vp, _vp]),
     "qsim_compile_pass": (_i, [_i, _c_int_p, _i, _vp, _vp, _vp, _i, _i, _vp, _sz, _vp, _vp, _vp]),
     "qsim_apply_bitflips": (_i, [_vp, _i64, _i, _vp, _vp, _vp, _vp, _vp]),
+    "qsim_compare_workspace": (_sz, [_i]),
+    "qsim_compare_states": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _sz, _vp]),
 }
 
 _lock = threading.Lock()

@@ -399,6 +399,11 @@ class MI355XBackend(GatePlanner):
         self._programs = collections.OrderedDict()
         self._programs_lock = threading.Lock()
         self.program_cache_hits = 0
+        # attach an accuracy report (infidelity and max-abs amplitude error
+        # against the complex128 execution of the same circuit) to every
+        # state-vector result of a complex64 backend; off by default (the
+        # report runs the circuit a second time)
+        self.report_accuracy = False
         rec = self.lib.qsim_blocked_header_bytes()
         if rec != blocking.HEADER_BYTES:
             raise ImportError(f"qibo_amd: blocked-pass record header of {rec} bytes does not match the host layout")
@@ -736,6 +741,7 @@ class MI355XBackend(GatePlanner):
         self._validate_nqubits(nqubits, density_matrix=dm)
         if isinstance(initial_state, Circuit):
             return self.execute_circuit(initial_state + circuit, None, nshots)
+        source = (circuit, initial_state)  # for accuracy reports (re-execution at complex128)
         if initial_state is not None:
             initial_state = self.cast(initial_state, dtype=self.dtype, copy=True)
             valid = (2**nqubits, 2**nqubits) if dm else (2**nqubits,)
@@ -756,13 +762,30 @@ class MI355XBackend(GatePlanner):
                 )
             return self.execute_circuit_repeated(circuit, nshots, initial_state)
         try:
-            return self._execute_circuit(circuit, initial_state=initial_state, nshots=nshots)
+            result = self._execute_circuit(circuit, initial_state=initial_state, nshots=nshots)
         except self.oom_error:
             raise_error(
                 RuntimeError,
                 f"State does not fit in {self.device} memory.Please switch the execution device "
                 "to a different one using ``qibo.set_device``.",
             )
+        result._source = source
+        if self.report_accuracy and not circuit.density_matrix:
+            result.accuracy_report = result.accuracy()
+        return result
+
+    def reference_backend(self):
+        """The complex128 backend on the same device that accuracy reports of
+        this backend's results compare with (``self`` when it is complex128)."""
+        if self.dtype == "complex128":
+            return self
+        ref = self.__dict__.get("_reference")
+        if ref is None or ref.torch_device != self.torch_device:
+            ref = MI355XBackend(dtype="complex128", device=self.torch_device.index)
+            ref.blocking = self.blocking
+            ref.blocking_ordered = self.blocking_ordered
+            self._reference = ref
+        return ref
 
     def _execute_circuit(self, circuit, initial_state=None, nshots: int = 1000):
         from .result import CircuitResult, QuantumState
@@ -955,6 +978,13 @@ class MI355XBackend(GatePlanner):
         return None  # unknown parameter type: not cacheable
 
     def _gate_key(self, gate):
+        # fast path: the key kept on an ordinary gate (see below) while its
+        # parameter and qubit tuples are the same objects
+        cached = getattr(gate, "__dict__", {}).get("_qkey")
+        if cached is not None and (cached[0] is gate._parameters and cached[1] is gate._target_qubits
+                                   and cached[2] is gate._control_qubits
+                                   and cached[3] == gate.is_controlled_by):
+            return cached[4]
         if kinds.is_measurement(gate):
             return None if gate.collapse else ("M",)
         if getattr(gate, "symbolic_parameters", None):

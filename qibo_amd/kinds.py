@@ -36,11 +36,20 @@ def is_special(gate) -> bool:
     """A special gate that is not a fused group: it runs on the state through its
     own ``apply`` between the compiled passes (callbacks, and this package's
     other ``SpecialGate`` subclasses)."""
-    from .gates import SpecialGate
-
     if is_fused(gate):
         return False
-    return is_callback(gate) or isinstance(gate, SpecialGate)
+    return is_callback(gate) or isinstance(gate, _special_base())
+
+
+_SPECIAL = []
+
+
+def _special_base():
+    if not _SPECIAL:  # imported on first use: gates imports this module
+        from .gates import SpecialGate
+
+        _SPECIAL.append(SpecialGate)
+    return _SPECIAL[0]
 
 
 def runs_outside_passes(gate) -> bool:

@@ -42,6 +42,14 @@ class QuantumState:
             self._state, qubits, self.nqubits, density_matrix=self.density_matrix
         )
 
+    def accuracy(self) -> dict:
+        """Infidelity (qibo's 1 - |<state|reference>|^2) and max-abs amplitude
+        error of this state against the complex128 execution of the circuit
+        that produced it, computed on the GPU (``qibo_amd.accuracy``)."""
+        from .accuracy import result_accuracy
+
+        return result_accuracy(self)
+
     def symbolic(self, decimals: int = 5, cutoff: float = 1e-10, max_terms: int = 20):
         state = self.backend.to_numpy(self._state)
         terms = []
