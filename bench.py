@@ -177,6 +177,27 @@ def _accuracy(b64, b128, c64_circ, c128_circ):
     return rep
 
 
+def _blocks5_timing(b64, b128, seed, n, nblocks):
+    """Dense 5-qubit blocks at n qubits: one qsim_apply_fused launch each,
+    complex64 on FP32 MFMA and complex128 on FP64 MFMA."""
+    blocks = _block_circuit(n, nblocks, seed)
+    t = _time_forms([("c128_s", b128, blocks), ("c64_s", b64, blocks)])
+    blk = {"nqubits": n, "blocks": nblocks, **t, "speedup_c64_vs_c128": t["c128_s"] / t["c64_s"]}
+    flops = 8 * 32 * 2**n  # 32 complex multiply-adds per amplitude
+    for key, isz, peak in (("c64", 8, 157.3), ("c128", 16, 78.6)):
+        per = t[f"{key}_s"] / nblocks
+        blk[key] = {"ms_per_block": 1e3 * per, "hbm_GBs": 2 * 2**n * isz / per / 1e9,
+                    "TFLOPs": flops / per / 1e12, "mfma_peak_TFLOPs": peak}
+    return blk
+
+
+def bench_blocks5(seed, n=32, nblocks=24):
+    """The dense 5-qubit block workload alone (the MFMA kernels' profile runs)."""
+    from qibo_amd.backend import MI355XBackend
+
+    return _blocks5_timing(MI355XBackend(dtype="complex64"), MI355XBackend(dtype="complex128"), seed, n, nblocks)
+
+
 def bench_precision(seed, n_time=32, n_acc=30, nlayers=20, nblocks=24):
     """complex64 and its fused form against the plain complex128 GPU path.
 
@@ -237,15 +258,7 @@ def bench_precision(seed, n_time=32, n_acc=30, nlayers=20, nblocks=24):
     }
     del q, qf
 
-    # dense 5-qubit blocks: one qsim_apply_fused launch each
-    blocks = _block_circuit(n_time, nblocks, seed)
-    t = _time_forms([("c128_s", b128, blocks), ("c64_s", b64, blocks)])
-    blk = {"blocks": nblocks, **t, "speedup_c64_vs_c128": t["c128_s"] / t["c64_s"]}
-    flops = 8 * 32 * 2**n_time  # 32 complex multiply-adds per amplitude
-    for key, isz, peak in (("c64", 8, 157.3), ("c128", 16, 78.6)):
-        per = t[f"{key}_s"] / nblocks
-        blk[key] = {"ms_per_block": 1e3 * per, "hbm_GBs": 2 * 2**n_time * isz / per / 1e9,
-                    "TFLOPs": flops / per / 1e12, "mfma_peak_TFLOPs": peak}
+    blk = _blocks5_timing(b64, b128, seed, n_time, nblocks)
     ba = _block_circuit(n_acc, nblocks, seed)
     blk["accuracy_c64"] = _accuracy(b64, b128, ba, ba)
     res["blocks5"] = blk
@@ -683,10 +696,10 @@ def main():
     if not args.no_extras:
         chosen = None if args.extras is None else set(args.extras.split(","))
         for key, fn in (("layered28", bench_layered28), ("qft", bench_qft), ("qft15", bench_qft15),
-                        ("precision", bench_precision), ("serving", bench_serving), ("adjoint", bench_adjoint),
+                        ("precision", bench_precision), ("blocks5", bench_blocks5), ("serving", bench_serving), ("adjoint", bench_adjoint),
                         ("density", bench_density)):
-            if chosen is not None and key not in chosen:
-                continue
+            if (chosen is not None and key not in chosen) or (chosen is None and key == "blocks5"):
+                continue  # blocks5 is part of "precision"; alone only on request (profile runs)
             try:
                 out[key] = fn(args.seed)
             except Exception as exc:  # noqa: BLE001 - an extra must not hide the headline line
