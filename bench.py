@@ -376,7 +376,12 @@ def bench_layered28(seed, nshots=100000, steps=3):
             "gate_apps_per_s": ngates / dt}
 
 
-def bench_serving(seed, nrequests=48, qps=16.0):
+def bench_serving(seed, nrequests=48, qps_list=(8.0, 16.0, 32.0, 64.0), ncheck=8):
+    """The mixed stream through the continuous-batching server at increasing
+    offered load (Poisson arrivals): circuits/s, p50 and p99 latency per
+    offered QPS, up to saturation. A sample of each stream's requests is then
+    served again alone and its results compared (bitwise) with the stream's."""
+    import numpy as np
     import torch
 
     from qibo_amd.backend import MI355XBackend
@@ -385,13 +390,32 @@ def bench_serving(seed, nrequests=48, qps=16.0):
     backends = {"complex128": MI355XBackend(dtype="complex128"), "complex64": MI355XBackend(dtype="complex64")}
     ex = GPUExecutor(backends, nstreams=8)
     budget = int(0.8 * torch.cuda.mem_get_info()[0])
-    reqs = poisson_stream(nrequests, qps=qps, seed=seed)
-    stats = CircuitServer(ex, budget, max_inflight=8).run(reqs, realtime=True)
-    out = stats.summary()
-    out.update({"qps_offered": qps, "requests": nrequests, "nqubits_range": [10, 30],
-                "depth_range": [10, 50], "hbm_budget_bytes": budget})
-    torch.cuda.empty_cache()
-    return out
+    sweep = []
+    mismatches = checked = 0
+    for i, qps in enumerate(qps_list):
+        reqs = poisson_stream(nrequests, qps=qps, seed=seed + i)
+        stats = CircuitServer(ex, budget, max_inflight=8).run(reqs, realtime=True)
+        row = stats.summary()
+        row["qps_offered"] = qps
+        sweep.append(row)
+        rng = np.random.default_rng(seed + i)
+        for r in rng.choice(reqs, size=min(ncheck, len(reqs)), replace=False):
+            mixed = r.result
+            h = ex.start(r)
+            while not ex.done(h):
+                pass
+            alone = ex.result(h)
+            checked += 1
+            same = (mixed == alone) if r.kind == "expval" else np.array_equal(mixed, alone)
+            mismatches += 0 if same else 1
+        torch.cuda.empty_cache()
+    ex.close()
+    best = max(sweep, key=lambda r: r["circuits_per_s"])
+    return {"requests_per_level": nrequests, "nqubits_range": [10, 30], "depth_range": [10, 50],
+            "hbm_budget_bytes": budget, "sweep": sweep, "circuits_per_s_max": best["circuits_per_s"],
+            "p50_ms_at_max": best["p50_ms"], "p99_ms_at_max": best["p99_ms"],
+            "executor_paths": dict(ex.stats),
+            "alone_vs_stream_checked": checked, "alone_vs_stream_mismatches": mismatches}
 
 
 def _noisy_layered(n, nlayers, seed, lam=0.01):

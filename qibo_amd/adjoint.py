@@ -21,8 +21,13 @@ gate for the whole gradient.
 
 Derivative matrices: for exp(-i theta G / 2) rotations (RX, RY, RZ, CRX, CRY,
 CRZ, RXX, RYY, RZZ, RZX) dU/dtheta = -i/2 G U; for U1 / CU1 it is
-diag(0, i e^{i theta}) on the controlled block; other parametrized gates use a
-central difference of the host matrix (step 1e-7), documented as such.
+diag(0, i e^{i theta}) on the controlled block. Every other parametrized gate
+of the package has matrix entries that depend on each parameter x through
+one frequency k only (a constant plus a cos(k x) and sin(k x) or e^{+-i k x}
+term: k = 1/2 for half angles, 1 for whole angles; table _FREQUENCIES), so
+the shift rule on the matrix is its exact derivative:
+dU/dx = k (U(x + pi/(2k)) - U(x - pi/(2k))) / 2. A gate outside the table
+falls back to a central difference of the host matrix (step 1e-7).
 """
 
 from typing import List, Sequence, Tuple
@@ -43,6 +48,15 @@ _GENERATORS = {
     "RXX": np.kron(_X, _X), "RYY": np.kron(_Y, _Y), "RZZ": np.kron(_Z, _Z), "RZX": np.kron(_Z, _X),
 }
 _CONTROLLED_GEN = {"CRX": _X, "CRY": _Y, "CRZ": _Z}
+# the one frequency with which each parameter enters the gate's matrix
+# entries (qibo_amd.matrices, after npmatrices.py): 1/2 for half angles
+# (cos(theta / 2), e^{i (phi + lam) / 2}), 1 for whole angles
+_FREQUENCIES = {
+    "PRX": (0.5, 1.0), "GPI": (1.0,), "GPI2": (1.0,), "U2": (0.5, 0.5), "U3": (0.5, 0.5, 0.5),
+    "U1q": (0.5, 1.0), "CU2": (0.5, 0.5), "CU3": (0.5, 0.5, 0.5), "fSim": (1.0, 1.0),
+    "GeneralizedfSim": (None, 1.0), "RXXYY": (0.5,), "MS": (1.0, 1.0, 0.5), "GIVENS": (1.0,),
+    "RBS": (1.0,), "DEUTSCH": (1.0,), "Align": (1.0,),
+}
 
 
 def _gate_qubits(gate) -> Tuple[Tuple[int, ...], Tuple[int, ...]]:
@@ -66,9 +80,28 @@ def derivative_matrices(backend, gate) -> List[np.ndarray]:
         return [np.diag([0, 1j * u[1, 1]])]
     if name == "CU1":
         return [np.diag([0, 0, 0, 1j * u[3, 3]])]
-    # other parametrized gates: central difference of the host matrix
-    out = []
     params = list(gate.parameters)
+    freqs = _FREQUENCIES.get(name)
+    if freqs is not None and len(freqs) == len(params):
+        out = []
+        for k, f in enumerate(freqs):
+            if f is None:  # a matrix-valued parameter (GeneralizedfSim's unitary): no scalar derivative
+                out.append(np.zeros_like(u))
+                continue
+            shift = np.pi / (2 * f)
+            p_plus = list(params)
+            p_minus = list(params)
+            p_plus[k] = params[k] + shift
+            p_minus[k] = params[k] - shift
+            gate.parameters = p_plus
+            up = np.asarray(gate.matrix(backend), dtype=np.complex128)
+            gate.parameters = p_minus
+            um = np.asarray(gate.matrix(backend), dtype=np.complex128)
+            out.append(0.5 * f * (up - um))
+        gate.parameters = params
+        return out
+    # a gate outside the table: central difference of the host matrix
+    out = []
     eps = 1e-7
     for k in range(len(params)):
         p_plus = list(params)

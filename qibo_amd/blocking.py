@@ -34,7 +34,10 @@ lane):
   multiplies of DENSE1.
 * REG_PHASE: the product of consecutive diagonal gates that involve only
   register bits, folded on the host into one 16-entry table (one complex
-  multiply per amplitude for all of them).
+  multiply per amplitude for all of them). After a pass's first table, a
+  table's value at register 0 is taken out as a global factor (folded into
+  the first table): the rest is 1 wherever the bits the table depends on are
+  0, and the record multiplies only the other registers (arg = those bits).
 * LANE_PHASE: a diagonal gate on bits that are all fixed per lane: it scales
   all 16 amplitudes of a lane by the same factor, so the kernel multiplies it
   into a per-lane pending factor and applies that once, before the next
@@ -77,6 +80,7 @@ OP_DENSE1_REAL, OP_REG_PHASE, OP_LANE_PHASE, OP_NEG = 4, 5, 6, 7
 OP_ROT = 10
 ROT_RY, ROT_H, ROT_RX = 0, 1, 2  # ROT modes (arg = register bit | mode << 2)
 OP_LANE_REGPHASE, OP_QFLUSH = 11, 12
+OP_NOP = 0xFF  # header only, no effect: pads programs of a batched launch to one record count
 REG_FLAG = 0x80  # Diag target byte: register bit j of the current mapping is REG_FLAG | j
 _REG_GROUPS = {0: (8, 9, 10, 11), 1: (4, 5, 6, 7), 2: (0, 1, 2, 3)}
 _SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=complex)
@@ -405,6 +409,8 @@ def _reg_phase_pattern(tregs: Tuple[int, ...], cregs: Tuple[int, ...]):
     di.setflags(write=False)
     return ok, di
 _RFLIP = [np.arange(16) ^ (1 << j) for j in range(4)]  # register r -> r with bit j flipped
+# registers a REG_PHASE record with dependency mask D multiplies: r & D != 0
+_REG_DEP = [np.array([(r & d) != 0 for r in range(16)]) for d in range(16)]
 
 
 def _schedule(items, npred, succ, ready, runs_here):
@@ -524,6 +530,8 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
     switches = 0
     cur = 0
     pending = None  # REG_PHASE table being folded (complex128, 16 entries)
+    reg_first = None  # (entry offset, table) of the pass's first REG_PHASE record (all registers)
+    gfac = np.ones(1, dtype=np.complex128)  # global factor taken out of later tables
     sign = False  # net -1 left by ROT records of gates with c < 0
     absorb = None  # (byte offset, entries) of the first unconditional linear record
     qpend = None  # register bit with a pending LANE_REGPHASE factor
@@ -547,13 +555,35 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
 
     def flush_reg(j=None):
         """Emit the folded table; before a register gate on register bit j
-        only when the table depends on that bit (otherwise the two commute)."""
-        nonlocal pending
+        only when the table depends on that bit (otherwise the two commute).
+
+        The pass's first table multiplies all 16 registers. A later table T
+        with |T[0]| = 1 is split into the global factor T[0] (collected and
+        folded into the first table at the end: a scalar commutes with every
+        record) and T / T[0], whose entries are exactly 1 on the registers
+        where the bits T depends on are all 0: the record's arg is that bit
+        mask D and the kernel multiplies only the registers r with r & D != 0
+        (none at all when T is constant)."""
+        nonlocal pending, reg_first, gfac
         if pending is None:
             return
         if j is not None and np.array_equal(pending, pending[_RFLIP[j]]):
             return
-        emit(OP_REG_PHASE, 0, 0, 0, (), pending)
+        if reg_first is None:
+            reg_first = (len(buf) + HEADER_BYTES, pending.copy())
+            emit(OP_REG_PHASE, 0, 0, 0, (), pending)
+        else:
+            t0 = complex(pending[0])
+            mag = t0.real * t0.real + t0.imag * t0.imag
+            if abs(mag - 1.0) <= 1e-12:
+                dep = sum(1 << k for k in range(4) if not np.array_equal(pending, pending[_RFLIP[k]]))
+                gfac = gfac * np.array([t0])
+                if dep:
+                    inv = complex(t0.real / mag, -t0.imag / mag)
+                    table = np.where(_REG_DEP[dep], pending * inv, _ONES16)
+                    emit(OP_REG_PHASE, dep, 0, 0, (), table)
+            else:
+                emit(OP_REG_PHASE, 0, 0, 0, (), pending)
         pending = None
 
     def reg_bits(mapping):
@@ -645,6 +675,11 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             emit(OP_DENSE_LDS, k, cm, 0, lbits + [0] * (4 - k) + sorted(lbits), mat)
     flush_reg()
     qflush()
+    if reg_first is not None and gfac[0] != 1:
+        off, table = reg_first
+        view = np.frombuffer(buf, dtype=cdtype, count=16, offset=off)
+        view[:] = (table * gfac[0]).astype(cdtype)
+        del view
     if sign:
         # a global -1 commutes with every record: scale one unconditional
         # linear record's entries (all of them), else flip every sign at the end
@@ -817,6 +852,8 @@ def program_flops(program: bytes, count: int, itemsize: int) -> float:
         nent = int.from_bytes(program[off + 4: off + 8], "little")
         if op == OP_DENSE_LDS:
             total += 6.0 + 8.0 * (2**arg - 1)
+        elif op == OP_REG_PHASE and arg & 15:
+            total += 6.0 * int(_REG_DEP[arg & 15].sum()) / 16
         else:
             total += _FLOPS.get(op, 0.0)
         off += record_bytes(nent, itemsize)

@@ -24,7 +24,8 @@
 //   MAP (2)         change to mapping `arg` (transpose through LDS)
 //   DENSE_LDS (3)   2^k x 2^k (k = arg <= 3) on local bits, tile in LDS
 //   DENSE1_REAL (4) 2x2 with zero imaginary parts on register bit `arg`
-//   REG_PHASE (5)   v[r] *= m[r]: diagonal gates on register bits, folded by the host
+//   REG_PHASE (5)   v[r] *= m[r]: diagonal gates on register bits, folded by the host;
+//                   arg = D != 0: only the registers with r & D != 0 (m[r] = 1 elsewhere)
 //   LANE_PHASE (6)  diagonal on per-lane-fixed bits: multiplied into the lane's
 //                   pending factor P, applied to the 16 amplitudes before the
 //                   next MAP / DENSE_LDS / store (it commutes with everything
@@ -69,7 +70,8 @@ enum : uint32_t {
   kOpInnerDiag = 9,  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
   kOpRot = 10,       // unconditional 2x2 rotation / reflection on a register bit, as three shears
   kOpLaneRegPhase = 11,  // per-lane factor Q of the registers with one bit set (kept in LDS)
-  kOpQFlush = 12         // apply Q to the registers with bit arg set, reset it to 1
+  kOpQFlush = 12,        // apply Q to the registers with bit arg set, reset it to 1
+  kOpNop = 0xFF          // no effect (pads the programs of a batched launch to one record count)
 };
 
 // Record header (the host packs the same layout).
@@ -120,6 +122,42 @@ __device__ __forceinline__ Hdr load_hdr(const uint8_t* p) {
   h.b1 = w[5];
   h.tmask = uint64_t(w[6]) | (uint64_t(w[7]) << 32);
   return h;
+}
+
+// The first 64 bytes of a record's entries, read with its header one record
+// ahead: the forms with at most 64 bytes of uniform entries (ROT, DENSE1*,
+// DIAG / LANE_PHASE on at most one bit, LANE_REGPHASE) take them from these
+// scalar registers instead of loading them when they run.
+struct Ent {
+  uint32_t w[16];
+};
+
+__device__ __forceinline__ Ent load_ent(const uint8_t* p) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+  Ent e;
+#pragma clang loop unroll(full)
+  for (int i = 0; i < 16; ++i) e.w[i] = q[i];
+  return e;
+}
+
+// Entry i of the prefetched 64 bytes (i < 4 for complex128, i < 8 for complex64).
+template <typename R>
+__device__ __forceinline__ cplx<R> ent(const Ent& e, int i);
+
+template <>
+__device__ __forceinline__ cplx<double> ent<double>(const Ent& e, int i) {
+  cplx<double> c;
+  c.re = __hiloint2double(int(e.w[4 * i + 1]), int(e.w[4 * i]));
+  c.im = __hiloint2double(int(e.w[4 * i + 3]), int(e.w[4 * i + 2]));
+  return c;
+}
+
+template <>
+__device__ __forceinline__ cplx<float> ent<float>(const Ent& e, int i) {
+  cplx<float> c;
+  c.re = __uint_as_float(e.w[2 * i]);
+  c.im = __uint_as_float(e.w[2 * i + 1]);
+  return c;
 }
 
 __device__ __forceinline__ uint32_t hbit(const Hdr& h, uint32_t t) {
@@ -450,7 +488,7 @@ __device__ __forceinline__ void diag_regs(cplx<R>* v, const Hdr& h, const cplx<R
     // whose diagonal index is the lane part alone, f1 for those a register
     // bit sets; a lane outside the condition multiplies by exactly 1 (the
     // per-gate arithmetic, cmul(d, v), for the others).
-    const cplx<R> d0 = m[0], d1 = m[h.nt];
+    const cplx<R> d0 = m[0], d1 = h.nt ? m[1] : m[0];
     cplx<R> one;
     one.re = R(1);
     one.im = R(0);
@@ -470,6 +508,17 @@ __device__ __forceinline__ void diag_regs(cplx<R>* v, const Hdr& h, const cplx<R
     const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
                         ((r & 8) ? rm[3] : 0u);
     v[r] = csel(lane_ok, cmul(m[di], v[r]), v[r]);
+  }
+}
+
+// REG_PHASE: v[r] *= m[r] on the registers r with r & D != 0 (D = 0: all
+// 16); the host writes exactly 1 for the others, which are left alone.
+template <typename R, int D>
+__device__ __forceinline__ void reg_phase(cplx<R>* v, const cplx<R>* m) {
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    if (D != 0 && (r & D) == 0) continue;
+    cmul_inplace(v[r], m[r]);
   }
 }
 
@@ -640,23 +689,39 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   const uint64_t tile_base_s = tb_lo | (tb_hi << 32);  // uniform copy (scalar registers)
   uint32_t off = 0;
   Hdr h = {};
-  if (a.ninstr > 0) h = load_hdr(prog);
+  Ent e = {};
+  if (a.ninstr > 0) {
+    h = load_hdr(prog);
+    e = load_ent(prog + sizeof(BlkHdr));
+  }
   for (int k = 0; k < a.ninstr && off + uint32_t(sizeof(BlkHdr)) <= nbytes; ++k) {
+    // Scalar loads return out of order, so any wait for one waits for all:
+    // this record's header and first entries (read during the previous
+    // record) are awaited here, before the next record's are issued, so that
+    // the record's own arithmetic below never waits for the read-ahead.
+    asm volatile("" ::"s"(h.op), "s"(e.w[0]), "s"(e.w[15]) : "memory");
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
     const uint32_t nent = h.nent < kMaxEntries ? h.nent : kMaxEntries;  // no record is larger
     off += uint32_t(sizeof(BlkHdr)) + ((nent * uint32_t(sizeof(cplx<R>)) + 15u) & ~15u);
     const uint32_t next = off < nbytes ? off : nbytes;  // stays inside the padded buffer
-    const Hdr hn = load_hdr(prog + next);  // next header, in flight while this record runs
+    const Hdr hn = load_hdr(prog + next);  // next header and entries, in flight while this record runs
+    const Ent en = load_ent(prog + next + sizeof(BlkHdr));
     if ((tile_base_s & h.tmask) != h.tmask) {  // condition fails on the whole tile
       h = hn;
+      e = en;
       continue;
     }
+    // the prefetched entries of the forms with at most four (complex128) or
+    // eight (complex64) uniform entries
+    cplx<R> me[4];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) me[i] = ent<R>(e, i);
     // the lane condition is formed only by the forms that use it
     auto lane_ok = [&]() { return (lane_idx & h.cmask) == h.cmask; };
     switch (h.op) {
       case kOpLaneRegPhase: {
         const cplx<R> q = q_lds[threadIdx.x];
-        q_lds[threadIdx.x] = csel(lane_ok(), cmul(m[0], q), q);
+        q_lds[threadIdx.x] = csel(lane_ok(), cmul(me[0], q), q);
         break;
       }
       case kOpQFlush: {
@@ -672,22 +737,22 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
       }
       case kOpDense1Real:
         switch (h.arg & 3) {
-          case 0: dense1_regs<R, 0, true>(v, h, m, lane_idx); break;
-          case 1: dense1_regs<R, 1, true>(v, h, m, lane_idx); break;
-          case 2: dense1_regs<R, 2, true>(v, h, m, lane_idx); break;
-          default: dense1_regs<R, 3, true>(v, h, m, lane_idx); break;
+          case 0: dense1_regs<R, 0, true>(v, h, me, lane_idx); break;
+          case 1: dense1_regs<R, 1, true>(v, h, me, lane_idx); break;
+          case 2: dense1_regs<R, 2, true>(v, h, me, lane_idx); break;
+          default: dense1_regs<R, 3, true>(v, h, me, lane_idx); break;
         }
         break;
       case kOpDense1:
         switch (h.arg & 3) {
-          case 0: dense1_regs<R, 0, false>(v, h, m, lane_idx); break;
-          case 1: dense1_regs<R, 1, false>(v, h, m, lane_idx); break;
-          case 2: dense1_regs<R, 2, false>(v, h, m, lane_idx); break;
-          default: dense1_regs<R, 3, false>(v, h, m, lane_idx); break;
+          case 0: dense1_regs<R, 0, false>(v, h, me, lane_idx); break;
+          case 1: dense1_regs<R, 1, false>(v, h, me, lane_idx); break;
+          case 2: dense1_regs<R, 2, false>(v, h, me, lane_idx); break;
+          default: dense1_regs<R, 3, false>(v, h, me, lane_idx); break;
         }
         break;
       case kOpRot: {
-        const R t = m[0].re, sn = m[0].im;
+        const R t = me[0].re, sn = me[0].im;
         const uint32_t mode = h.arg >> 2;
         switch (h.arg & 3) {
           case 0: rot_regs<R, 0>(v, mode, t, sn); break;
@@ -698,14 +763,30 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
       }
       case kOpRegPhase:
-#pragma clang loop unroll(full)
-        for (int r = 0; r < kBReg; ++r) cmul_inplace(v[r], m[r]);
+        switch (h.arg & 15) {
+          case 0: reg_phase<R, 0>(v, m); break;
+          case 1: reg_phase<R, 1>(v, m); break;
+          case 2: reg_phase<R, 2>(v, m); break;
+          case 3: reg_phase<R, 3>(v, m); break;
+          case 4: reg_phase<R, 4>(v, m); break;
+          case 5: reg_phase<R, 5>(v, m); break;
+          case 6: reg_phase<R, 6>(v, m); break;
+          case 7: reg_phase<R, 7>(v, m); break;
+          case 8: reg_phase<R, 8>(v, m); break;
+          case 9: reg_phase<R, 9>(v, m); break;
+          case 10: reg_phase<R, 10>(v, m); break;
+          case 11: reg_phase<R, 11>(v, m); break;
+          case 12: reg_phase<R, 12>(v, m); break;
+          case 13: reg_phase<R, 13>(v, m); break;
+          case 14: reg_phase<R, 14>(v, m); break;
+          default: reg_phase<R, 15>(v, m); break;
+        }
         break;
       case kOpLanePhase: {
         uint32_t rm[4];
         const uint32_t di = diag_lane_part(h, lane_idx, rm);  // no register bits: rm stays 0
-        if (h.nt <= 1) {  // uniform entries: scalar loads and a per-lane select
-          const cplx<R> d0 = m[0], d1 = m[h.nt];
+        if (h.nt <= 1) {  // uniform entries (prefetched) and a per-lane select
+          const cplx<R> d0 = me[0], d1 = h.nt ? me[1] : me[0];
           P = csel(lane_ok(), cmul(di ? d1 : d0, P), P);
         } else {
           P = csel(lane_ok(), cmul(m[di], P), P);
@@ -726,7 +807,10 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
       }
       case kOpDiag:
-        diag_regs(v, h, m, lane_idx, lane_ok());
+        if (h.nt <= 1)
+          diag_regs(v, h, me, lane_idx, lane_ok());  // entries 0 and nt: prefetched
+        else
+          diag_regs(v, h, m, lane_idx, lane_ok());
         break;
       case kOpMap: {
         flush_lane_phase(v, P, pdirty);
@@ -751,6 +835,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
     }
     h = hn;
+    e = en;
   }
 
   flush_lane_phase(v, P, pdirty);
@@ -896,8 +981,13 @@ __device__ __forceinline__ void flush_lane_phase2(cplx<R>* v, cplx<R>* w, cplx<R
   pdirty = false;
 }
 
+#ifndef QSIM_ADJOINT_WAVES
+#define QSIM_ADJOINT_WAVES 3
+#endif
+
 template <typename R>
-__global__ __launch_bounds__(kBT) void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_ADJOINT_WAVES)))
+void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
                                                          cplx<R>* __restrict__ psi, cplx<R>* __restrict__ lam,
                                                          double* __restrict__ part) {
   __shared__ R lds[kBTile];

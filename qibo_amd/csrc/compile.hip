@@ -15,7 +15,9 @@
 //  3. emission: diagonal gates on lane bits become per-lane factors
 //     (LANE_PHASE), diagonal gates on register bits are folded into a 16-entry
 //     table (REG_PHASE) flushed before a register gate that does not commute
-//     with it, -1 phases become sign flips (NEG), controlled phases between one
+//     with it (after the pass's first table, the value at register 0 is taken
+//     out as a global factor folded into the first table, and the record
+//     multiplies only the registers whose bits the table depends on), -1 phases become sign flips (NEG), controlled phases between one
 //     register bit and lane bits become per-lane factors with a flush (QFLUSH),
 //     unconditional rotations / reflections become three-shear ROT records,
 //     other one-qubit gates DENSE1(_REAL), two- and three-qubit gates
@@ -76,6 +78,22 @@ struct Writer {
   bool overflow = false;
   long absorb_off = -1;  // byte offset of the entries of the first unconditional linear record
   int absorb_n = 0;
+
+  // Overwrite the n entries at byte offset `off` (complex128 values, stored in
+  // the record's precision).
+  void rewrite(long off, const C* data, int n) {
+    uint8_t* e = out + off;
+    for (int i = 0; i < n; ++i) {
+      if (itemsize == 16) {
+        std::memcpy(e + 16 * i, &data[i].re, 8);
+        std::memcpy(e + 16 * i + 8, &data[i].im, 8);
+      } else {
+        const float re = float(data[i].re), im = float(data[i].im);
+        std::memcpy(e + 8 * i, &re, 4);
+        std::memcpy(e + 8 * i + 4, &im, 4);
+      }
+    }
+  }
 
   void emit(uint8_t op, int arg, uint64_t cmask, int nt, const int* bits, int nbits, const C* data, int ndata,
             int rreq = 0) {
@@ -238,6 +256,11 @@ int compile(int n, const int* tile_bits, std::vector<Op>& items, bool ordered, W
   int cur = 0;
   bool have_pending = false;
   C pending[16];
+  // the pass's first REG_PHASE table (all registers) and the global factor
+  // taken out of the later ones, folded into it at the end
+  long reg_first_off = -1;
+  C reg_first[16];
+  C gfac = {1.0, 0.0};
   bool sign = false;
   int qpend = -1;
   *switches = 0;
@@ -257,12 +280,36 @@ int compile(int n, const int* tile_bits, std::vector<Op>& items, bool ordered, W
   };
   auto flush_reg = [&](int j) {
     if (!have_pending) return;
-    if (j >= 0) {
-      bool same = true;
-      for (int r = 0; r < 16 && same; ++r) same = ceq(pending[r], pending[r ^ (1 << j)]);
-      if (same) return;
+    auto depends_on = [&](int k) {
+      for (int r = 0; r < 16; ++r)
+        if (!ceq(pending[r], pending[r ^ (1 << k)])) return true;
+      return false;
+    };
+    if (j >= 0 && !depends_on(j)) return;
+    if (reg_first_off < 0) {
+      reg_first_off = long(w.len) + kHeader;
+      for (int r = 0; r < 16; ++r) reg_first[r] = pending[r];
+      w.emit(kRRegPhase, 0, 0, 0, nullptr, 0, pending, 16);
+    } else {
+      // T = T[0] (T / T[0]): the factor goes to the first table, the record
+      // multiplies only the registers where a bit T depends on is set
+      const C t0 = pending[0];
+      const double mag = t0.re * t0.re + t0.im * t0.im;
+      if (std::fabs(mag - 1.0) <= 1e-12) {
+        int dep = 0;
+        for (int k = 0; k < 4; ++k)
+          if (depends_on(k)) dep |= 1 << k;
+        gfac = cmulf(gfac, t0);
+        if (dep) {
+          const C inv = {t0.re / mag, -t0.im / mag};
+          C table[16];
+          for (int r = 0; r < 16; ++r) table[r] = (r & dep) ? cmulf(pending[r], inv) : C{1.0, 0.0};
+          w.emit(kRRegPhase, dep, 0, 0, nullptr, 0, table, 16);
+        }
+      } else {
+        w.emit(kRRegPhase, 0, 0, 0, nullptr, 0, pending, 16);
+      }
     }
-    w.emit(kRRegPhase, 0, 0, 0, nullptr, 0, pending, 16);
     have_pending = false;
   };
   auto split_cond = [&](const Op& o, int m, uint64_t* cm, int* rreq) {
@@ -416,6 +463,11 @@ int compile(int n, const int* tile_bits, std::vector<Op>& items, bool ordered, W
   }
   flush_reg(-1);
   qflush(-1);
+  if (reg_first_off >= 0 && !(gfac.re == 1.0 && gfac.im == 0.0) && !w.overflow) {
+    C table[16];
+    for (int r = 0; r < 16; ++r) table[r] = cmulf(reg_first[r], gfac);
+    w.rewrite(reg_first_off, table, 16);
+  }
   if (sign) {
     if (w.absorb_off >= 0)
       w.negate_absorbed();

@@ -304,9 +304,16 @@ struct Mfma64Args {
   uint8_t pos[kMaxFixed];  // target bit positions, ascending (for insert_zeros)
 };
 
+#ifndef QSIM_FUSED_WAVES
+#define QSIM_FUSED_WAVES 4
+#endif
+#ifndef QSIM_FUSED_PREFETCH
+#define QSIM_FUSED_PREFETCH 0
+#endif
+
 template <int K>
-__global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, const c128* __restrict__ mat,
-                                                        c128* __restrict__ psi) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QSIM_FUSED_WAVES)))
+void k_fused_mfma_c128(const Mfma64Args a, const c128* __restrict__ mat, c128* __restrict__ psi) {
   constexpr int D = 1 << K;          // complex rows
   constexpr int RPL = D / 4;         // rows per lane = K-steps per product
   constexpr int MB = D / 16;         // 16-row output blocks per product
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, con
   const uint64_t gend = gend0 < a.ngroups ? gend0 : a.ngroups;
   uint64_t g = uint64_t(xcd) * chunk + w;
   c128 x[RPL];
-  if (g < gend) {
+  if (QSIM_FUSED_PREFETCH && g < gend) {
     const uint64_t base = insert_zeros(16 * g + j, a.pos, a.nfixed) | qin;
 #pragma clang loop unroll(full)
     for (int s = 0; s < RPL; ++s) x[s] = ld_nt(psi, base | a.off[s]);
@@ -345,10 +352,22 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, con
   while (g < gend) {
     const uint64_t gn = g + wpx;
     c128 xn[RPL];
-    if (gn < gend) {  // the next group's loads overlap this group's products
-      const uint64_t bn = insert_zeros(16 * gn + j, a.pos, a.nfixed) | qin;
+    if (QSIM_FUSED_PREFETCH) {
+      if (gn < gend) {  // the next group's loads overlap this group's products
+        const uint64_t bn = insert_zeros(16 * gn + j, a.pos, a.nfixed) | qin;
 #pragma clang loop unroll(full)
-      for (int s = 0; s < RPL; ++s) xn[s] = ld_nt(psi, bn | a.off[s]);
+        for (int s = 0; s < RPL; ++s) xn[s] = ld_nt(psi, bn | a.off[s]);
+      }
+    } else {
+      // no register double buffer: the other resident waves cover the latency
+      // (addresses formed next to each load, not hoisted and held)
+      const c128* pin = psi + (insert_zeros(16 * g + j, a.pos, a.nfixed) | qin);
+#pragma clang loop unroll(full)
+      for (int s = 0; s < RPL; ++s) {
+        const c128* q = pin;
+        asm volatile("" : "+v"(q));
+        x[s] = ld_nt(q, a.off[s]);
+      }
     }
     doublex4 acc[3][MB];
 #pragma clang loop unroll(full)
@@ -368,7 +387,7 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, con
                                                           acc[2][mb], 0, 0, 0);
       }
     }
-    const uint64_t obase = insert_zeros(16 * g + j, a.pos, a.nfixed) | qout;
+    c128* pout = psi + (insert_zeros(16 * g + j, a.pos, a.nfixed) | qout);
 #pragma clang loop unroll(full)
     for (int mb = 0; mb < MB; ++mb)
 #pragma clang loop unroll(full)
@@ -376,12 +395,124 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c128(const Mfma64Args a, con
         c128 y;
         y.re = acc[0][mb][r] - acc[1][mb][r];
         y.im = (acc[2][mb][r] - acc[0][mb][r]) - acc[1][mb][r];
-        st_nt(psi, obase | a.off[16 * mb + 4 * r], y);
+        c128* q = pout;
+        asm volatile("" : "+v"(q));
+        st_nt(q, a.off[16 * mb + 4 * r], y);
       }
+    if (QSIM_FUSED_PREFETCH) {
 #pragma clang loop unroll(full)
-    for (int s = 0; s < RPL; ++s) x[s] = xn[s];
+      for (int s = 0; s < RPL; ++s) x[s] = xn[s];
+    }
     g = gn;
   }
+}
+
+// ---------------------------------------------------------------------------
+// complex64 blocks of 5 qubits straight from HBM (v_mfma_f32_32x32x2f32)
+// ---------------------------------------------------------------------------
+// Y = M X for 32 columns at a time (one wavefront) as the three real 32 x 32
+// products of the complex128 kernel above (Gauss), FP32 operands and FP32
+// accumulation: P1 = Re M Re X, P2 = Im M Im X, P3 = (Re M + Im M)(Re X +
+// Im X); Re Y = P1 - P2, Im Y = (P3 - P1) - P2. Each product is 16 K-steps of
+// the 32x32x2 MFMA.
+//
+// Operand layout of v_mfma_f32_32x32x2f32 (wave64):
+//   A (32 x 2):  lane l supplies A[l % 32][l / 32]
+//   B (2 x 32):  lane l supplies B[l / 32][l % 32]
+//   D (32 x 32): register j of lane l holds D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32]
+// Lane (c = l % 32, h = l / 32) therefore loads rows h, h + 2, ..., h + 30 of
+// column c (K-step s uses row 2 s + h) into registers and stores rows
+// 8 (j / 4) + 4 h + j % 4 of the same column. No LDS holds the state: a
+// column group belongs to one wavefront, whose loads complete before its
+// stores. The matrix fragments sit in LDS in lane order.
+template <int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QSIM_FUSED_WAVES)))
+void k_fused_mfma32_c64(const Mfma64Args a, const c64* __restrict__ mat, c64* __restrict__ psi) {
+  static_assert(K == 5, "32x32 blocks");
+  constexpr int D = 1 << K;
+  constexpr int KS = D / 2;  // K-steps per product
+  __shared__ float frag[3 * KS * 64];
+  // fragment (p, s), lane l: M_p[l % 32][2 s + l / 32], M_0 = Re M, M_1 = Im M, M_2 = Re M + Im M
+  for (int e = threadIdx.x; e < 3 * KS * 64; e += 256) {
+    const int l = e & 63, f = e >> 6;
+    const int st = f % KS, p = f / KS;
+    const c64 m = mat[(l & 31) * D + 2 * st + (l >> 5)];
+    frag[e] = p == 0 ? m.re : (p == 1 ? m.im : m.re + m.im);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 31, h = lane >> 5;
+  const uint64_t hin = a.off[h];       // row h
+  const uint64_t hout = a.off[4 * h];  // row 4 h
+  const uint32_t nxcd = gridDim.x >= 8 ? 8u : 1u;
+  const uint32_t xcd = blockIdx.x % nxcd;
+  const uint64_t wpx = uint64_t(gridDim.x / nxcd) * 4;
+  const uint64_t w = uint64_t(blockIdx.x / nxcd) * 4 + (threadIdx.x >> 6);
+  const uint64_t chunk = (a.ngroups + nxcd - 1) / nxcd;
+  const uint64_t gend0 = (uint64_t(xcd) + 1) * chunk;
+  const uint64_t gend = gend0 < a.ngroups ? gend0 : a.ngroups;
+  for (uint64_t g = uint64_t(xcd) * chunk + w; g < gend; g += wpx) {
+    const uint64_t col = insert_zeros(32 * g + c, a.pos, a.nfixed);
+    // lane pointers; the row offsets are uniform (index bits disjoint from the
+    // column's, so OR = ADD), each address formed next to its access
+    const c64* pin = psi + (col | hin);
+    c64 x[KS];
+#pragma clang loop unroll(full)
+    for (int st = 0; st < KS; ++st) {
+      const c64* q = pin;
+      asm volatile("" : "+v"(q));  // keep the 16 addresses from being hoisted and held
+      x[st] = ld_nt(q, a.off[2 * st]);
+    }
+    floatx16 acc0 = floatx16(0.0f), acc1 = floatx16(0.0f), acc2 = floatx16(0.0f);
+#pragma clang loop unroll(full)
+    for (int st = 0; st < KS; ++st) {
+      const float b0 = x[st].re, b1 = x[st].im, b2 = x[st].re + x[st].im;
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(0 * KS + st) * 64 + lane], b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(1 * KS + st) * 64 + lane], b1, acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(2 * KS + st) * 64 + lane], b2, acc2, 0, 0, 0);
+    }
+    c64* pout = psi + (col | hout);
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 16; ++j) {
+      c64 y;
+      y.re = acc0[j] - acc1[j];
+      y.im = (acc2[j] - acc0[j]) - acc1[j];
+      c64* q = pout;
+      asm volatile("" : "+v"(q));
+      st_nt(q, a.off[8 * (j / 4) + (j % 4)], y);
+    }
+  }
+}
+
+static int launch_fused_mfma32_c64(void* state, int n, const int* targets, const void* mat, hipStream_t s) {
+  constexpr int K = 5;
+  Mfma64Args a;
+  std::memset(&a, 0, sizeof(a));
+  a.nfixed = sorted_positions(n, targets, K, nullptr, 0, a.pos);
+  a.ngroups = (uint64_t(1) << (n - K)) / 32;
+  for (int r = 0; r < (1 << K); ++r) {
+    uint64_t off = 0;
+    for (int t = 0; t < K; ++t)
+      if ((r >> (K - 1 - t)) & 1) off |= uint64_t(1) << (n - 1 - targets[t]);
+    a.off[r] = off;
+  }
+  static const unsigned resident = [] {
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_mfma32_c64<K>, 256, 0) != hipSuccess || nb < 1)
+      nb = 2;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    return unsigned(nb) * unsigned(cus);
+  }();
+  const uint64_t want = (a.ngroups + 3) / 4;
+  unsigned grid = unsigned(want < resident ? want : resident);
+  if (grid >= 8) grid &= ~7u;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_fused_mfma32_c64<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c64*>(mat),
+                     reinterpret_cast<c64*>(state));
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
 }
 
 template <int K>
@@ -434,7 +565,7 @@ extern "C" int qsim_apply_fused(void* state, int nqubits, int dtype, const int* 
     if (nqubits >= 8 && ntargets == 4) return launch_fused_mfma_c128<4>(state, nqubits, targets, matrix_dev, s);
     return dispatch_fused<double>(state, nqubits, targets, ntargets, matrix_dev, s);
   }
-  if (nqubits >= kMfmaTileBits && ntargets == 5) return launch_fused_mfma<5>(state, nqubits, targets, matrix_dev, s);
+  if (nqubits >= 10 && ntargets == 5) return launch_fused_mfma32_c64(state, nqubits, targets, matrix_dev, s);
   if (nqubits >= kMfmaTileBits && ntargets == 4) return launch_fused_mfma<4>(state, nqubits, targets, matrix_dev, s);
   return dispatch_fused<float>(state, nqubits, targets, ntargets, matrix_dev, s);
 }

@@ -668,10 +668,17 @@ __device__ double chain_tile_chunks(const P* __restrict__ p, int64_t lo, int64_t
       }
       s = from_m(m2, cEb);
     } else {
-      for (int64_t i = cl; i < ch; ++i) {
-        s = s + load_p(p, i);
-        if (lane == 0) cdf[i] = s;
+      // element by element, in order: the chunk's values are loaded once
+      // (one per lane, coalesced) and passed along by shuffles, so the chain
+      // waits on no memory access; lane j keeps the sum after element j
+      const int64_t len = ch - cl;
+      const double pv = lane < len ? load_p(p, cl + lane) : 0.0;
+      double mine = 0.0;
+      for (int j = 0; j < len; ++j) {
+        s = s + __shfl(pv, j, 64);
+        if (lane == j) mine = s;
       }
+      if (lane < len) cdf[cl + lane] = mine;
     }
   }
   if (resolved) {
@@ -713,7 +720,13 @@ __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t 
   }
   const int64_t gend = g * 64 + 64 < ntiles ? g * 64 + 64 : ntiles;
   for (int64_t u = g * 64; u < gend; ++u) {
-    const TileInfo tu = info[u];
+    // tile u's info is held by lane u - 64 g (read once above)
+    const int src = int(u - g * 64);
+    TileInfo tu;
+    tu.c0 = __shfl(ti.c0, src, 64);
+    tu.c1 = __shfl(ti.c1, src, 64);
+    tu.Eb = __shfl(ti.Eb, src, 64);
+    tu.flag = __shfl(ti.flag, src, 64);
     if (lane == 0) tstart[u] = s;
     bool ok = tu.flag == 1 && in_binade(s, tu.Eb);
     int64_t m2 = 0;
@@ -768,7 +781,13 @@ __global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n
     }
     const int64_t kend = k + 64 < ngroups ? k + 64 : ngroups;
     for (int64_t g = k; g < kend; ++g) {
-      const TileInfo gu = ginfo[g];
+      // group g's info is held by lane g - k (read once above)
+      const int src = int(g - k);
+      TileInfo gu;
+      gu.c0 = __shfl(gi.c0, src, 64);
+      gu.c1 = __shfl(gi.c1, src, 64);
+      gu.Eb = __shfl(gi.Eb, src, 64);
+      gu.flag = __shfl(gi.flag, src, 64);
       bool ok = gu.flag == 1 && in_binade(s, gu.Eb);
       int64_t m2 = 0;
       if (ok) {

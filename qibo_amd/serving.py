@@ -148,58 +148,430 @@ class CircuitServer:
         return stats
 
 
-class GPUExecutor:
-    """Runs each admitted request on its own HIP stream of the MI355X backend."""
+def _zsum_tables(n):
+    """Pauli tables of sum_i Z_i on n qubits (qsim_expval_pauli layout)."""
+    xm = np.zeros(n, dtype=np.uint64)
+    zm = np.asarray([1 << (n - 1 - q) for q in range(n)], dtype=np.uint64)
+    ny = np.zeros(n, dtype=np.int32)
+    return xm, zm, ny, np.ones(n, dtype=np.float64)
 
-    def __init__(self, backends: dict, nstreams: int = 8):
+
+class _Tail:
+    """Device buffers of one request's measurement (``<sum Z_i>`` or shot
+    indices) and the launches that fill them: the same library calls as
+    ``calculate_probabilities`` -> ``exact_cdf`` -> ``search_cdf`` and
+    ``expectation_pauli``, into buffers owned here (so that a captured graph
+    replays into them), ending with the copy of the result to pinned host
+    memory."""
+
+    def __init__(self, backend, n, dtype_code, kind, nshots):
+        import torch
+
+        from . import _lib
+
+        self.lib = backend.lib
+        self.n, self.code, self.kind, self.nshots = n, dtype_code, kind, nshots
+        dev = backend.torch_device
+        if kind == "expval":
+            self.tables = _zsum_tables(n)
+            self.wsb = self.lib.qsim_expval_workspace(n, n)
+            self.ws = torch.empty(max(self.wsb, 256), dtype=torch.uint8, device=dev)
+            self.out = torch.empty(1, dtype=torch.float64, device=dev)
+            self.host = torch.empty(1, dtype=torch.float64, pin_memory=True)
+        else:
+            pdt = torch.float64 if dtype_code == _lib.C128 else torch.float32
+            self.pcode = _lib.F64 if dtype_code == _lib.C128 else _lib.F32
+            self.probs = torch.empty(2**n, dtype=pdt, device=dev)
+            self.pwsb = self.lib.qsim_probabilities_workspace(n, n)
+            self.pws = torch.empty(max(self.pwsb, 256), dtype=torch.uint8, device=dev)
+            self.cdf = torch.empty(2**n, dtype=torch.float64, device=dev)
+            self.cwsb = self.lib.qsim_cumsum_workspace(2**n)
+            self.cws = torch.empty(max(self.cwsb, 256), dtype=torch.uint8, device=dev)
+            self.u_host = torch.empty(max(nshots, 1), dtype=torch.float64, pin_memory=True)
+            self.u_dev = torch.empty(max(nshots, 1), dtype=torch.float64, device=dev)
+            self.out = torch.empty(max(nshots, 1), dtype=torch.int64, device=dev)
+            self.host = torch.empty(max(nshots, 1), dtype=torch.int64, pin_memory=True)
+        self.qubits = None
+
+    def set_uniforms(self, seed):
+        if self.kind == "shots":
+            self.u_host.numpy()[: self.nshots] = np.random.RandomState(seed).random_sample(self.nshots)
+
+    def launch(self, state, stream_ptr):
+        """Enqueue the measurement of ``state`` (2^n amplitudes) on the current stream."""
+        from . import _lib
+
+        lib, n = self.lib, self.n
+        if self.kind == "expval":
+            xm, zm, ny, c = self.tables
+            rc = lib.qsim_expval_pauli(state.data_ptr(), n, self.code, n, xm.ctypes.data, zm.ctypes.data,
+                                       ny.ctypes.data, c.ctypes.data, self.out.data_ptr(), self.ws.data_ptr(),
+                                       self.wsb, stream_ptr)
+            _lib.check(rc, lib)
+        else:
+            if self.qubits is None:
+                self.qubits = _lib.int_array(range(n))
+            rc = lib.qsim_probabilities(state.data_ptr(), n, self.code, self.qubits, n, self.probs.data_ptr(),
+                                        self.pws.data_ptr(), self.pwsb, stream_ptr)
+            _lib.check(rc, lib)
+            rc = lib.qsim_cumsum_exact(self.probs.data_ptr(), self.pcode, 2**n, self.cdf.data_ptr(),
+                                       self.cws.data_ptr(), self.cwsb, stream_ptr)
+            _lib.check(rc, lib)
+            self.u_dev.copy_(self.u_host, non_blocking=True)
+            rc = lib.qsim_search_sorted(self.cdf.data_ptr(), 2**n, self.u_dev.data_ptr(), self.nshots,
+                                        self.out.data_ptr(), stream_ptr)
+            _lib.check(rc, lib)
+        self.host.copy_(self.out, non_blocking=True)
+
+    def result(self):
+        if self.kind == "expval":
+            return float(self.host[0].item())
+        return self.host.numpy()[: self.nshots].copy()
+
+
+class _Graph:
+    """A request structure's captured launch sequence: the upload of the
+    compiled records from a pinned buffer, every blocked pass (the first from
+    |0...0>), and the measurement. A later request of the same structure
+    writes its own records and uniforms into the pinned inputs and replays it."""
+
+    def __init__(self, signature, nbytes, state, prog_host, prog_dev, tail, graph):
+        self.signature = signature
+        self.nbytes = nbytes
+        self.state, self.prog_host, self.prog_dev, self.tail, self.graph = state, prog_host, prog_dev, tail, graph
+        self.busy = None  # event of the replay in flight
+
+
+class GPUExecutor:
+    """Runs admitted requests on the MI355X backend.
+
+    * Host work -- gate planning and record compilation -- runs on worker
+      threads (the native planner and compiler release the GIL), so the
+      server's admission and completion loop never waits for it.
+    * Every request runs on a HIP stream of its own, so its passes overlap
+      with those of the other requests in flight.
+    * Requests on fewer than 12 qubits are batched: each is one blocked pass
+      on a 12-qubit buffer, and those ready at a flush run in one
+      ``qsim_apply_blocked_batched`` launch (grid z = request) over one
+      contiguous buffer, their programs padded to a common length with no-op
+      records; each then measures its own slice.
+    * Graphs: a request on 12..``graph_max_qubits`` qubits whose passes are
+      all blocked and whose structure (the gates' classes and qubits, register
+      size, dtype and measurement) ran before is captured as a HIP graph. A
+      later request of the same structure -- a new parameter set of the same
+      ansatz, or the same circuit again -- writes its compiled records and
+      uniforms into the graph's pinned inputs and replays it: one launch for
+      the program upload, every pass, the measurement and the result copy.
+      Graph nodes are kernels and copies only.
+    Results are bitwise the same on every path (the same kernels, records and
+    reductions)."""
+
+    def __init__(self, backends: dict, nstreams: int = 8, workers: int = 2, batch_small: bool = True,
+                 graphs: bool = True, graph_max_qubits: int = 22, graph_cache: int = 64):
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+
         import torch
 
         self.torch = torch
         self.backends = backends                      # dtype -> MI355XBackend
         dev = next(iter(backends.values())).torch_device
+        self.device = dev
         self.streams = [torch.cuda.Stream(device=dev) for _ in range(nstreams)]
         self.free_streams = list(range(nstreams))
+        self.pool = ThreadPoolExecutor(max_workers=workers, thread_name_prefix="qsim-serve")
+        self.batch_small = batch_small
+        self.graphs = graphs
+        self.graph_max_qubits = graph_max_qubits
+        self.graph_cache = graph_cache
+        self._graphs = collections.OrderedDict()      # structure key -> [_Graph]
+        self._seen = collections.OrderedDict()        # structures run once (captured when seen again)
+        self._lock = threading.Lock()
+        self._small = []                              # handles of small requests awaiting a batched launch
+        self.batch_stream = torch.cuda.Stream(device=dev)
+        self.stats = collections.Counter()            # path counts: graph_capture, graph_replay, direct, batched
 
+    # -- the executor interface ------------------------------------------------
     def start(self, req: Request):
-        torch = self.torch
-        backend = self.backends[req.dtype]
-        sid = self.free_streams.pop()
-        stream = self.streams[sid]
-        n = req.nqubits
-        with torch.cuda.stream(stream):
-            state = backend.zero_state(n)
-            state = backend.run_queue([g for g in req.circuit.queue], state, n)
-            if req.kind == "expval":
-                xm = np.zeros(n, dtype=np.uint64)
-                zm = np.asarray([1 << (n - 1 - q) for q in range(n)], dtype=np.uint64)
-                ny = np.zeros(n, dtype=np.int32)
-                out = backend.expectation_pauli(state, n, xm, zm, ny, np.ones(n))
-            else:
-                probs = backend.calculate_probabilities(state, list(range(n)), n)
-                cdf = backend.exact_cdf(probs)
-                u = np.random.RandomState(req.seed).random_sample(req.nshots)
-                out = backend.search_cdf(cdf, u)
-            host = torch.empty(out.shape, dtype=out.dtype, pin_memory=True)
-            host.copy_(out, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        return {"sid": sid, "event": ev, "host": host, "keep": (state, out)}
+        h = {"req": req, "sid": None, "event": None, "tail": None, "keep": None, "future": None,
+             "graph": None}
+        if self.batch_small and req.nqubits < 12:
+            h["future"] = self.pool.submit(self._compile_small, req)
+            with self._lock:
+                self._small.append(h)
+            return h
+        h["sid"] = self.free_streams.pop()
+        h["future"] = self.pool.submit(self._run, req, h)
+        return h
 
     def done(self, h) -> bool:
-        if h["event"].query():
-            if h["sid"] is not None:
-                self.free_streams.append(h["sid"])
-                h["sid"] = None
-                h["keep"] = None
+        if "value" in h:
             return True
-        return False
+        fut = h["future"]
+        if fut is not None and not fut.done():
+            return False
+        if fut is not None:
+            fut.result()  # re-raise a failure of the host work
+        if h["event"] is None and h["sid"] is None:
+            self.flush_small()  # a compiled small request still waiting for its batch
+        ev = h["event"]
+        if ev is None or not ev.query():
+            return False
+        # read the result before the graph (and its pinned output) is released
+        h["value"] = h["tail"].result()
+        h["keep"] = h["tail"] = None
+        if h["sid"] is not None:
+            self.free_streams.append(h["sid"])
+            h["sid"] = None
+        if h["graph"] is not None:
+            h["graph"].busy = None
+            h["graph"] = None
+        return True
 
     def result(self, h):
-        host = h["host"]
-        return float(host.item()) if host.dim() == 0 else host.numpy().copy()
+        return h["value"]
 
     def idle(self):
+        self.flush_small()
         time.sleep(50e-6)
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+    # -- requests on >= 12 qubits ---------------------------------------------
+    @staticmethod
+    def _structure(req):
+        parts = tuple((type(g).__name__, tuple(g.qubits)) for g in req.circuit.queue)
+        return (req.nqubits, req.dtype, req.kind, req.nshots, parts)
+
+    def _compile(self, backend, req):
+        from . import _lib, blocking
+
+        n = req.nqubits
+        ops = [backend.plan_gate(g) for g in req.circuit.queue]
+        code = _lib.C128 if req.dtype == "complex128" else _lib.C64
+        cdtype = np.complex128 if code == _lib.C128 else np.complex64
+        plan, program = blocking.compile_circuit(ops, n, cdtype, backend._low_bits(code),
+                                                 backend.blocking_ordered)
+        return code, plan, program
+
+    def _run(self, req, h):
+        torch = self.torch
+        from . import blocking
+
+        backend = self.backends[req.dtype]
+        stream = self.streams[h["sid"]]
+        n = req.nqubits
+        code, plan, program = self._compile(backend, req)
+        all_blocked = plan and all(p[0] == "block" for p in plan)
+        signature = tuple((p[1], p[2], p[3]) for p in plan if p[0] == "block")
+        key = self._structure(req) if (self.graphs and all_blocked and n <= self.graph_max_qubits) else None
+        with torch.cuda.stream(stream):
+            sptr = stream.cuda_stream
+            if key is not None:
+                g = self._graph_for(key, signature, len(program))
+                if g is not None:  # replay with this request's records and uniforms
+                    host = g.prog_host.numpy()
+                    host[: len(program)] = np.frombuffer(program, dtype=np.uint8)
+                    g.tail.set_uniforms(req.seed)
+                    g.graph.replay()
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                    g.busy = ev
+                    h.update(tail=g.tail, graph=g, event=ev)
+                    self.stats["graph_replay"] += 1
+                    return
+                g = self._capture(backend, req, key, code, plan, program, signature) if self._repeat(key) else None
+                if g is not None:
+                    g.tail.set_uniforms(req.seed)
+                    g.graph.replay()
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                    g.busy = ev
+                    h.update(tail=g.tail, graph=g, event=ev)
+                    self.stats["graph_capture"] += 1
+                    return
+            # direct launches
+            state = backend._alloc_state(n, req.dtype)
+            zero = [True]
+            if program:
+                host = torch.frombuffer(bytearray(program) + bytes(blocking.PAD_BYTES), dtype=torch.uint8)
+                prog = host.pin_memory().to(backend.torch_device, non_blocking=True)
+            else:
+                prog = None
+            cur = torch.cuda.current_stream(backend.torch_device)
+            for p in plan:
+                if p[0] == "single":
+                    if zero[0]:
+                        backend._fill_zero(state, n)
+                        zero[0] = False
+                    backend.apply_op(p[1], state, n)
+                else:
+                    _, tile_bits, off, count, ngates = p
+                    backend._launch_blocked(state, n, code, tile_bits, prog[off:], self._pass_bytes(plan, p, program),
+                                            count, ngates, sptr, cur, zero)
+            if zero[0]:
+                backend._fill_zero(state, n)
+            tail = _Tail(backend, n, code, req.kind, req.nshots)
+            tail.set_uniforms(req.seed)
+            tail.launch(state, sptr)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            h.update(tail=tail, event=ev, keep=(state, prog))
+            self.stats["direct"] += 1
+
+    @staticmethod
+    def _pass_bytes(plan, p, program):
+        """Bytes of pass p's records in the concatenated program."""
+        offs = [q[2] for q in plan if q[0] == "block"] + [len(program)]
+        i = offs.index(p[2])
+        return offs[i + 1] - p[2]
+
+    def _repeat(self, key) -> bool:
+        """True when the structure ran before: a graph is captured only for a
+        structure seen a second time (a one-off circuit is not worth one)."""
+        with self._lock:
+            if key in self._seen:
+                return True
+            self._seen[key] = True
+            while len(self._seen) > 4096:
+                self._seen.popitem(last=False)
+            return False
+
+    def _graph_for(self, key, signature, nbytes):
+        with self._lock:
+            for g in self._graphs.get(key, ()):
+                if g.busy is None and g.signature == signature and g.nbytes == nbytes:
+                    g.busy = True  # claimed until the replay's event is recorded
+                    self._graphs.move_to_end(key)
+                    return g
+        return None
+
+    def _capture(self, backend, req, key, code, plan, program, signature):
+        torch = self.torch
+        from . import _lib, blocking
+
+        n = req.nqubits
+        with self._lock:
+            total = sum(len(v) for v in self._graphs.values())
+            if total >= self.graph_cache:
+                return None
+        state = backend._alloc_state(n, req.dtype)
+        prog_host = torch.zeros(len(program) + blocking.PAD_BYTES, dtype=torch.uint8, pin_memory=True)
+        prog_host.numpy()[: len(program)] = np.frombuffer(program, dtype=np.uint8)
+        prog_dev = torch.empty_like(prog_host, device=backend.torch_device)
+        tail = _Tail(backend, n, code, req.kind, req.nshots)
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=backend.torch_device)
+        side.wait_stream(torch.cuda.current_stream(backend.torch_device))
+        # thread-local capture: the other workers and the server loop keep
+        # launching and polling their own streams meanwhile
+        with torch.cuda.graph(graph, stream=side, capture_error_mode="thread_local"):
+            sptr = side.cuda_stream
+            prog_dev.copy_(prog_host, non_blocking=True)
+            zero = [True]
+            for p in plan:
+                _, tile_bits, off, count, ngates = p
+                launch = backend.lib.qsim_apply_blocked_from_zero if zero[0] else backend.lib.qsim_apply_blocked
+                zero[0] = False
+                rc = launch(state.data_ptr(), n, code, _lib.int_array(list(tile_bits)), prog_dev.data_ptr() + off,
+                            self._pass_bytes(plan, p, program), count, sptr)
+                _lib.check(rc, backend.lib)
+            tail.launch(state, sptr)
+        g = _Graph(signature, len(program), state, prog_host, prog_dev, tail, graph)
+        g.busy = True
+        with self._lock:
+            self._graphs.setdefault(key, []).append(g)
+            self._graphs.move_to_end(key)
+        return g
+
+    # -- requests below 12 qubits: one batched launch ----------------------------
+    def _compile_small(self, req):
+        from . import _lib, blocking
+        from .backend import _Op
+
+        backend = self.backends[req.dtype]
+        n = req.nqubits
+        pad = blocking.TILE_BITS - n
+        ops = []
+        for g in req.circuit.queue:
+            op = backend.plan_gate(g)
+            ops.append(_Op(op.kind, tuple(q + pad for q in op.targets), tuple(q + pad for q in op.controls),
+                           op.data))
+        code = _lib.C128 if req.dtype == "complex128" else _lib.C64
+        cdtype = np.complex128 if code == _lib.C128 else np.complex64
+        plan, program = blocking.compile_circuit(ops, blocking.TILE_BITS, cdtype, backend._low_bits(code),
+                                                 backend.blocking_ordered)
+        if len(plan) > 1 or (plan and plan[0][0] != "block"):
+            return code, None  # not a single pass (e.g. a 4-qubit dense block): run it alone
+        count = plan[0][3] if plan else 0
+        return code, (program, count)
+
+    def flush_small(self):
+        """Launch every compiled small request in one batched pass per dtype."""
+        torch = self.torch
+        from . import _lib, blocking
+
+        with self._lock:
+            ready = [h for h in self._small if h["future"].done()]
+            if not ready:
+                return
+            self._small = [h for h in self._small if not h["future"].done()]
+        groups = collections.defaultdict(list)
+        for h in ready:
+            code, compiled = h["future"].result()
+            if compiled is None:  # not one pass: run on its own stream
+                req = h["req"]
+                backend = self.backends[req.dtype]
+                with torch.cuda.stream(self.batch_stream):
+                    state = backend.zero_state(req.nqubits)
+                    state = backend.run_queue(list(req.circuit.queue), state, req.nqubits)
+                    tail = _Tail(backend, req.nqubits, code, req.kind, req.nshots)
+                    tail.set_uniforms(req.seed)
+                    tail.launch(state, self.batch_stream.cuda_stream)
+                    ev = torch.cuda.Event()
+                    ev.record(self.batch_stream)
+                h.update(tail=tail, keep=(state,), event=ev, future=None)
+                self.stats["small_alone"] += 1
+                continue
+            groups[code].append((h, compiled))
+        for code, members in groups.items():
+            backend = self.backends[members[0][0]["req"].dtype]
+            nb = len(members)
+            maxcount = max(c for _, (_, c) in members)
+            maxbytes = max(len(p) + (maxcount - c) * blocking.HEADER_BYTES for _, (p, c) in members)
+            stride = (maxbytes + 255) // 256 * 256
+            progs = np.zeros(stride * nb + blocking.PAD_BYTES, dtype=np.uint8)
+            for i, (_, (p, c)) in enumerate(members):
+                progs[i * stride: i * stride + len(p)] = np.frombuffer(p, dtype=np.uint8)
+                # no-op records up to the common count (header only, no entries)
+                for k in range(maxcount - c):
+                    progs[i * stride + len(p) + k * blocking.HEADER_BYTES] = blocking.OP_NOP
+            tdt = torch.complex128 if code == _lib.C128 else torch.complex64
+            with torch.cuda.stream(self.batch_stream):
+                sptr = self.batch_stream.cuda_stream
+                states = torch.zeros(nb, 2**blocking.TILE_BITS, dtype=tdt, device=self.device)
+                states[:, 0] = 1
+                prog = torch.from_numpy(progs).pin_memory().to(self.device, non_blocking=True)
+                if maxcount:
+                    rc = backend.lib.qsim_apply_blocked_batched(
+                        states.data_ptr(), nb, blocking.TILE_BITS, code,
+                        _lib.int_array(range(blocking.TILE_BITS)), prog.data_ptr(), maxbytes, stride, maxcount, sptr)
+                    _lib.check(rc, backend.lib)
+                evs = []
+                for i, (h, _) in enumerate(members):
+                    req = h["req"]
+                    tail = _Tail(backend, req.nqubits, code, req.kind, req.nshots)
+                    tail.set_uniforms(req.seed)
+                    tail.launch(states[i, : 2**req.nqubits], sptr)
+                    h.update(tail=tail, keep=(states, prog), future=None)
+                    evs.append(h)
+                ev = torch.cuda.Event()
+                ev.record(self.batch_stream)
+            for h in evs:
+                h["event"] = ev
+            self.stats["batched"] += nb
+            self.stats["batched_launches"] += 1
 
 
 def poisson_stream(nrequests: int, qps: float, seed: int, nmin: int = 10, nmax: int = 30,

@@ -96,6 +96,9 @@ def run_pass(state, n, tile_bits, instrs):
             qbit = None
         elif op == blocking.OP_REG_PHASE:
             assert cmask == 0 and rreq == 0
+            dep = arg & 15  # the kernel skips the registers with r & dep == 0
+            if dep:
+                assert all(mat[r] == 1 for r in range(16) if not r & dep)
             r = np.zeros_like(gidx)
             for j in range(4):
                 r |= ((gidx >> rb[j]) & 1) << j
