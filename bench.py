@@ -393,10 +393,19 @@ def bench_serving(seed, nrequests=48, qps_list=(8.0, 16.0, 32.0, 64.0), ncheck=8
     sweep = []
     mismatches = checked = 0
     for i, qps in enumerate(qps_list):
-        reqs = poisson_stream(nrequests, qps=qps, seed=seed + i)
+        # the same requests at every level (the seed fixes the circuits; the
+        # offered rate only scales the arrival times)
+        reqs = poisson_stream(nrequests, qps=qps, seed=seed)
+        ex.timings.clear()
         stats = CircuitServer(ex, budget, max_inflight=8).run(reqs, realtime=True)
         row = stats.summary()
         row["qps_offered"] = qps
+        tm = np.asarray([(h, d) for _, h, d in ex.timings]) if ex.timings else np.zeros((1, 2))
+        # per request: host time from start to the last launch, then the wait
+        # for the device
+        row["host_ms_p50"] = float(np.percentile(tm[:, 0], 50) * 1e3)
+        row["host_ms_max"] = float(tm[:, 0].max() * 1e3)
+        row["device_wait_ms_p50"] = float(np.percentile(tm[:, 1], 50) * 1e3)
         sweep.append(row)
         rng = np.random.default_rng(seed + i)
         for r in rng.choice(reqs, size=min(ncheck, len(reqs)), replace=False):

@@ -95,8 +95,9 @@ int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const i
 /* Dense 2^k x 2^k block on k = ntargets (1..5) targets with the matrix in
  * DEVICE memory: the application of a FusedGate (abstract.py:2680-2717 builds
  * the matrix; apply_gate applies it). complex128 blocks of 4-5 qubits
- * (nqubits >= k + 4): v_mfma_f64_16x16x4f64 with FP64 accumulation, as three
- * real products (Re M Re X, Im M Im X, (Re M + Im M)(Re X + Im X)); complex64
+ * (nqubits >= 11, 2048-amplitude LDS tiles): v_mfma_f64_16x16x4f64 with FP64
+ * accumulation, as three real products (Re M Re X, Im M Im X,
+ * (Re M + Im M)(Re X + Im X)); complex64
  * blocks of 4-5 qubits (nqubits >= 12): v_mfma_f32_32x32x2f32 on the real
  * embedding [[Re, -Im], [Im, Re]] with FP32 accumulation; smaller blocks:
  * FMA per column in the state's precision. */

@@ -125,9 +125,9 @@ __device__ __forceinline__ Hdr load_hdr(const uint8_t* p) {
 }
 
 // The first 64 bytes of a record's entries, read with its header one record
-// ahead: the forms with at most 64 bytes of uniform entries (ROT, DENSE1*,
-// DIAG / LANE_PHASE on at most one bit, LANE_REGPHASE) take them from these
-// scalar registers instead of loading them when they run.
+// ahead by the adjoint sweep: the forms with at most 64 bytes of uniform
+// entries (DENSE1*, INNER1) take them from these scalar registers instead of
+// loading them when they run.
 struct Ent {
   uint32_t w[16];
 };
@@ -689,33 +689,21 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   const uint64_t tile_base_s = tb_lo | (tb_hi << 32);  // uniform copy (scalar registers)
   uint32_t off = 0;
   Hdr h = {};
-  Ent e = {};
-  if (a.ninstr > 0) {
-    h = load_hdr(prog);
-    e = load_ent(prog + sizeof(BlkHdr));
-  }
+  if (a.ninstr > 0) h = load_hdr(prog);
   for (int k = 0; k < a.ninstr && off + uint32_t(sizeof(BlkHdr)) <= nbytes; ++k) {
-    // Scalar loads return out of order, so any wait for one waits for all:
-    // this record's header and first entries (read during the previous
-    // record) are awaited here, before the next record's are issued, so that
-    // the record's own arithmetic below never waits for the read-ahead.
-    asm volatile("" ::"s"(h.op), "s"(e.w[0]), "s"(e.w[15]) : "memory");
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
     const uint32_t nent = h.nent < kMaxEntries ? h.nent : kMaxEntries;  // no record is larger
     off += uint32_t(sizeof(BlkHdr)) + ((nent * uint32_t(sizeof(cplx<R>)) + 15u) & ~15u);
     const uint32_t next = off < nbytes ? off : nbytes;  // stays inside the padded buffer
-    const Hdr hn = load_hdr(prog + next);  // next header and entries, in flight while this record runs
-    const Ent en = load_ent(prog + next + sizeof(BlkHdr));
+    const Hdr hn = load_hdr(prog + next);  // next header, in flight while this record runs
     if ((tile_base_s & h.tmask) != h.tmask) {  // condition fails on the whole tile
       h = hn;
-      e = en;
       continue;
     }
-    // the prefetched entries of the forms with at most four (complex128) or
-    // eight (complex64) uniform entries
-    cplx<R> me[4];
-#pragma clang loop unroll(full)
-    for (int i = 0; i < 4; ++i) me[i] = ent<R>(e, i);
+    // (entries read where used: reading them one record ahead as well, as the
+    // adjoint kernel does, measured 7 % slower here -- SGPR pressure and the
+    // copies of the read-ahead registers)
+    const cplx<R>* me = m;
     // the lane condition is formed only by the forms that use it
     auto lane_ok = [&]() { return (lane_idx & h.cmask) == h.cmask; };
     switch (h.op) {
@@ -785,7 +773,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
       case kOpLanePhase: {
         uint32_t rm[4];
         const uint32_t di = diag_lane_part(h, lane_idx, rm);  // no register bits: rm stays 0
-        if (h.nt <= 1) {  // uniform entries (prefetched) and a per-lane select
+        if (h.nt <= 1) {  // uniform entries: scalar loads and a per-lane select
           const cplx<R> d0 = me[0], d1 = h.nt ? me[1] : me[0];
           P = csel(lane_ok(), cmul(di ? d1 : d0, P), P);
         } else {
@@ -807,10 +795,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
       }
       case kOpDiag:
-        if (h.nt <= 1)
-          diag_regs(v, h, me, lane_idx, lane_ok());  // entries 0 and nt: prefetched
-        else
-          diag_regs(v, h, m, lane_idx, lane_ok());
+        diag_regs(v, h, m, lane_idx, lane_ok());
         break;
       case kOpMap: {
         flush_lane_phase(v, P, pdirty);
@@ -835,7 +820,6 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         break;
     }
     h = hn;
-    e = en;
   }
 
   flush_lane_phase(v, P, pdirty);
@@ -1024,13 +1008,24 @@ void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
   uint32_t off = 0;
   int slot = 0;
   Hdr h = {};
-  if (a.ninstr > 0) h = load_hdr(prog);
+  Ent e = {};
+  if (a.ninstr > 0) {
+    h = load_hdr(prog);
+    e = load_ent(prog + sizeof(BlkHdr));
+  }
   for (int k = 0; k < a.ninstr && off + uint32_t(sizeof(BlkHdr)) <= nbytes; ++k) {
+    // this record's header and first entries were read during the previous
+    // one; they are awaited before the next record's reads are issued
+    asm volatile("" ::"s"(h.op), "s"(e.w[0]), "s"(e.w[15]) : "memory");
     const cplx<R>* m = reinterpret_cast<const cplx<R>*>(prog + off + sizeof(BlkHdr));
     const uint32_t nent = h.nent < kMaxEntries ? h.nent : kMaxEntries;
     off += uint32_t(sizeof(BlkHdr)) + ((nent * uint32_t(sizeof(cplx<R>)) + 15u) & ~15u);
     const uint32_t next = off < nbytes ? off : nbytes;
     const Hdr hn = load_hdr(prog + next);
+    const Ent en = load_ent(prog + next + sizeof(BlkHdr));
+    cplx<R> me[4];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 4; ++i) me[i] = ent<R>(e, i);
     const bool tile_ok = (tile_base_s & h.tmask) == h.tmask;
     if (h.op == kOpInner1 || h.op == kOpInnerDiag) {
       // every INNER record writes its partial, zero where the condition fails
@@ -1040,10 +1035,10 @@ void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
           acc = inner_diag_regs(v, w, h, m, lane_idx);
         } else {
           switch (h.arg & 3) {
-            case 0: acc = inner1_regs<R, 0>(v, w, h, m); break;
-            case 1: acc = inner1_regs<R, 1>(v, w, h, m); break;
-            case 2: acc = inner1_regs<R, 2>(v, w, h, m); break;
-            default: acc = inner1_regs<R, 3>(v, w, h, m); break;
+            case 0: acc = inner1_regs<R, 0>(v, w, h, me); break;
+            case 1: acc = inner1_regs<R, 1>(v, w, h, me); break;
+            case 2: acc = inner1_regs<R, 2>(v, w, h, me); break;
+            default: acc = inner1_regs<R, 3>(v, w, h, me); break;
           }
         }
         // a pending lane factor P (a product of diagonal unitaries, applied
@@ -1055,10 +1050,12 @@ void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
         part[((bz * uint64_t(a.nslots) + uint64_t(slot)) * a.ntiles + tile) * 4 + wave] = acc;
       ++slot;
       h = hn;
+      e = en;
       continue;
     }
     if (!tile_ok) {
       h = hn;
+      e = en;
       continue;
     }
     const bool lane_ok = (lane_idx & h.cmask) == h.cmask;
@@ -1067,14 +1064,14 @@ void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
       case kOpDense1: {
         const bool real = h.op == kOpDense1Real;
         switch ((h.arg & 3) | (real ? 4 : 0)) {
-          case 0: dense1_regs<R, 0, false>(v, h, m, lane_idx); dense1_regs<R, 0, false>(w, h, m, lane_idx); break;
-          case 1: dense1_regs<R, 1, false>(v, h, m, lane_idx); dense1_regs<R, 1, false>(w, h, m, lane_idx); break;
-          case 2: dense1_regs<R, 2, false>(v, h, m, lane_idx); dense1_regs<R, 2, false>(w, h, m, lane_idx); break;
-          case 3: dense1_regs<R, 3, false>(v, h, m, lane_idx); dense1_regs<R, 3, false>(w, h, m, lane_idx); break;
-          case 4: dense1_regs<R, 0, true>(v, h, m, lane_idx); dense1_regs<R, 0, true>(w, h, m, lane_idx); break;
-          case 5: dense1_regs<R, 1, true>(v, h, m, lane_idx); dense1_regs<R, 1, true>(w, h, m, lane_idx); break;
-          case 6: dense1_regs<R, 2, true>(v, h, m, lane_idx); dense1_regs<R, 2, true>(w, h, m, lane_idx); break;
-          default: dense1_regs<R, 3, true>(v, h, m, lane_idx); dense1_regs<R, 3, true>(w, h, m, lane_idx); break;
+          case 0: dense1_regs<R, 0, false>(v, h, me, lane_idx); dense1_regs<R, 0, false>(w, h, me, lane_idx); break;
+          case 1: dense1_regs<R, 1, false>(v, h, me, lane_idx); dense1_regs<R, 1, false>(w, h, me, lane_idx); break;
+          case 2: dense1_regs<R, 2, false>(v, h, me, lane_idx); dense1_regs<R, 2, false>(w, h, me, lane_idx); break;
+          case 3: dense1_regs<R, 3, false>(v, h, me, lane_idx); dense1_regs<R, 3, false>(w, h, me, lane_idx); break;
+          case 4: dense1_regs<R, 0, true>(v, h, me, lane_idx); dense1_regs<R, 0, true>(w, h, me, lane_idx); break;
+          case 5: dense1_regs<R, 1, true>(v, h, me, lane_idx); dense1_regs<R, 1, true>(w, h, me, lane_idx); break;
+          case 6: dense1_regs<R, 2, true>(v, h, me, lane_idx); dense1_regs<R, 2, true>(w, h, me, lane_idx); break;
+          default: dense1_regs<R, 3, true>(v, h, me, lane_idx); dense1_regs<R, 3, true>(w, h, me, lane_idx); break;
         }
         break;
       }
@@ -1116,6 +1113,7 @@ void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
         break;
     }
     h = hn;
+    e = en;
   }
   flush_lane_phase2(v, w, P, pdirty);
   if (map != 0) {

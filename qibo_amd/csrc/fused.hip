@@ -1,7 +1,8 @@
 // Dense k-qubit blocks (k <= 5) with the matrix in device memory: the
-// application of a fused gate group. complex128 blocks and complex64 blocks
-// of fewer than 4 qubits use FMA arithmetic per column (k_fused_fp);
-// complex64 blocks of 4 or 5 qubits use the MFMA kernel further below.
+// application of a fused gate group. Blocks of fewer than 4 qubits use FMA
+// arithmetic per column (k_fused_fp); blocks of 4 or 5 qubits run on MFMA
+// through an LDS tile (complex64: FP32 MFMA, k_fused_mfma_c64; complex128:
+// FP64 MFMA, k_fused_tile_c128), further below.
 //
 // psi viewed as a (2^k) x (2^(n-k)) matrix after gathering the k target bits
 // is multiplied on the left by the 2^k x 2^k block (abstract.py:2350-2361
@@ -275,7 +276,7 @@ static int launch_fused_mfma(void* state, int n, const int* targets, const void*
 }
 
 // ---------------------------------------------------------------------------
-// complex128 blocks of k = 4 or 5 qubits on FP64 MFMA (v_mfma_f64_16x16x4f64)
+// complex128 blocks: FP64 MFMA (v_mfma_f64_16x16x4f64)
 // ---------------------------------------------------------------------------
 // Y = M X for 16 columns at a time (one wavefront), as three real products
 // (Gauss): P1 = Re M Re X, P2 = Im M Im X, P3 = (Re M + Im M)(Re X + Im X),
@@ -288,260 +289,195 @@ static int launch_fused_mfma(void* state, int n, const int* targets, const void*
 //   B (4 x 16):  lane l supplies B[l / 16][l % 16]
 //   D (16 x 16): register r of lane l holds D[4 r + l / 16][l % 16]
 // (the f64 form interleaves the rows over the four lane groups, unlike the
-// f32 16x16x4 form). Lane (j = l % 16, q = l / 16) therefore loads the D/4
+// f32 16x16x4 form). Lane (j = l % 16, q = l / 16) therefore reads the D/4
 // rows (D/4) q .. (D/4) q + D/4 - 1 of column j (K-step s uses row
-// (D/4) q + s) straight from HBM into registers, and stores rows
-// 16 mb + 4 r + q of the same column: no LDS staging of the state. The matrix fragments (A
-// operands) sit in LDS in lane order, read once per MFMA without bank
-// conflicts. A column group belongs to one wavefront, so the in-place update
-// needs no synchronisation: its loads complete before its stores.
+// (D/4) q + s) and writes rows 16 mb + 4 r + q of the same column. The
+// matrix fragments (A operands) sit in LDS in lane order, read once per MFMA
+// without bank conflicts.
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-struct Mfma64Args {
-  uint64_t ngroups;        // groups of 16 columns: 2^(n-K) / 16
-  uint64_t off[32];        // index bits of matrix row r (row bit K-1-t <-> target t)
-  int nfixed;
-  uint8_t pos[kMaxFixed];  // target bit positions, ascending (for insert_zeros)
+// k = 4 or 5 qubits, with the state staged through LDS as in the complex64
+// kernel: a workgroup owns a tile of 2048 amplitudes (the k target bits plus
+// the lowest 11 - k other bits), so every global load and store is a
+// coalesced slice of 256 consecutive tile elements whatever the targets are.
+// (Column loads straight from HBM into the B operands were measured at half
+// the bandwidth when the targets include the lowest index bits: 13.5 against
+// 6.9 ms per block at 30 qubits.)
+// Each wavefront multiplies groups of 16 columns read from the tile and
+// writes its results back to the same slots; the tile is then streamed out.
+// Persistent workgroups; the next tile's loads are in flight during the
+// products of the current one.
+constexpr int kT128Bits = 11;
+
+struct Tile128Args {
+  uint64_t ntiles;
+  uint8_t tbits[kT128Bits];   // global bit of local tile bit i, ascending
+  uint8_t rowbit[5];          // local tile bit of matrix row bit (K-1-t), t = target order
+  uint8_t colbit[kT128Bits];  // local tile bits of the column index, ascending (11 - K of them)
 };
 
-#ifndef QSIM_FUSED_WAVES
-#define QSIM_FUSED_WAVES 4
-#endif
-#ifndef QSIM_FUSED_PREFETCH
-#define QSIM_FUSED_PREFETCH 0
-#endif
+// LDS slot of local index u (16-byte slots): bits 0-3 ^= bits 4-7 ^ bits 8-10.
+// A bijection of [0, 2048). Sixteen lanes that read 16 columns whose index
+// bits are four consecutive local bits land on 16 distinct 16-byte bank
+// groups wherever those bits sit; the staging slices (consecutive elements)
+// stay conflict-free.
+__device__ __forceinline__ uint32_t swz128(uint32_t u) { return u ^ ((u >> 4) & 15u) ^ ((u >> 8) & 7u); }
 
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QSIM_FUSED_WAVES)))
-void k_fused_mfma_c128(const Mfma64Args a, const c128* __restrict__ mat, c128* __restrict__ psi) {
+__device__ __forceinline__ uint32_t tile128_col(const Tile128Args& a, uint32_t c) {
+  uint32_t u = 0;
+#pragma clang loop unroll(full)
+  for (int i = 0; i < kT128Bits - K; ++i)
+    if ((c >> i) & 1) u |= 1u << a.colbit[i];
+  return u;
+}
+
+// Global index of staging element k * 256 + thread of the tile whose local
+// bits 0-7 (the thread) are already in glow.
+__device__ __forceinline__ uint64_t tile128_addr(const Tile128Args& a, uint64_t glow, int k) {
+  uint64_t g = glow;
+#pragma clang loop unroll(full)
+  for (int i = 0; i < kT128Bits - 8; ++i)
+    if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
+  return g;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fused_tile_c128(const Tile128Args a, const c128* __restrict__ mat,
+                                                        c128* __restrict__ psi) {
   constexpr int D = 1 << K;          // complex rows
   constexpr int RPL = D / 4;         // rows per lane = K-steps per product
   constexpr int MB = D / 16;         // 16-row output blocks per product
   constexpr int NFRAG = 3 * MB * RPL;
+  constexpr int NCG = (1 << (kT128Bits - K)) / 16;  // 16-column groups per tile: 4 (K = 5) or 8 (K = 4)
+  constexpr int PER = (1 << kT128Bits) / 256;       // staging elements per thread
+  __shared__ c128 tile[1 << kT128Bits];
   __shared__ double frag[NFRAG * 64];
-  // fragment (p, mb, s), lane l: M_p[16 mb + l % 16][RPL (l / 16) + s] with
-  // M_0 = Re M, M_1 = Im M, M_2 = Re M + Im M
+  __shared__ uint32_t rowoff[D];
   for (int e = threadIdx.x; e < NFRAG * 64; e += 256) {
     const int l = e & 63, f = e >> 6;
     const int st = f % RPL, mb = (f / RPL) % MB, p = f / (RPL * MB);
     const c128 m = mat[(16 * mb + (l & 15)) * D + RPL * (l >> 4) + st];
     frag[e] = p == 0 ? m.re : (p == 1 ? m.im : m.re + m.im);
   }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < D) {
+    uint32_t u = 0;
+#pragma clang loop unroll(full)
+    for (int t = 0; t < K; ++t)
+      if ((threadIdx.x >> (K - 1 - t)) & 1) u |= 1u << a.rowbit[t];
+    rowoff[threadIdx.x] = u;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, q = lane >> 4;
-  // row offsets are OR-linear in the row bits: off[RPL q + s] = off[RPL q] | off[s]
-  const uint64_t qin = a.off[RPL * q];
-  const uint64_t qout = a.off[q];
-  // XCD-contiguous work split: workgroup b runs on XCD b % 8; each XCD walks one
-  // contiguous eighth of the column groups, its wavefronts interleaved
-  const uint32_t nxcd = gridDim.x >= 8 ? 8u : 1u;
-  const uint32_t xcd = blockIdx.x % nxcd;
-  const uint64_t wpx = uint64_t(gridDim.x / nxcd) * 4;  // wavefronts per XCD
-  const uint64_t w = uint64_t(blockIdx.x / nxcd) * 4 + (threadIdx.x >> 6);
-  const uint64_t chunk = (a.ngroups + nxcd - 1) / nxcd;
-  const uint64_t gend0 = (uint64_t(xcd) + 1) * chunk;
-  const uint64_t gend = gend0 < a.ngroups ? gend0 : a.ngroups;
-  uint64_t g = uint64_t(xcd) * chunk + w;
-  c128 x[RPL];
-  if (QSIM_FUSED_PREFETCH && g < gend) {
-    const uint64_t base = insert_zeros(16 * g + j, a.pos, a.nfixed) | qin;
+  uint64_t lanebits = 0;  // local bits 0-7 of this thread's staging elements, as global bits
 #pragma clang loop unroll(full)
-    for (int s = 0; s < RPL; ++s) x[s] = ld_nt(psi, base | a.off[s]);
+  for (int i = 0; i < 8; ++i)
+    if ((threadIdx.x >> i) & 1) lanebits |= uint64_t(1) << a.tbits[i];
+  uint64_t t = blockIdx.x;
+  c128 nxt[PER];
+  if (t < a.ntiles) {
+    const uint64_t glow = insert_zeros(t, a.tbits, kT128Bits) | lanebits;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < PER; ++k) nxt[k] = ld_nt(psi, tile128_addr(a, glow, k));
   }
-  while (g < gend) {
-    const uint64_t gn = g + wpx;
-    c128 xn[RPL];
-    if (QSIM_FUSED_PREFETCH) {
-      if (gn < gend) {  // the next group's loads overlap this group's products
-        const uint64_t bn = insert_zeros(16 * gn + j, a.pos, a.nfixed) | qin;
+  while (t < a.ntiles) {
+    const uint64_t glow = insert_zeros(t, a.tbits, kT128Bits) | lanebits;
+    // each thread writes (and later reads back) only its own staging slots
 #pragma clang loop unroll(full)
-        for (int s = 0; s < RPL; ++s) xn[s] = ld_nt(psi, bn | a.off[s]);
+    for (int k = 0; k < PER; ++k) tile[swz128(uint32_t(k) * 256u + threadIdx.x)] = nxt[k];
+    __syncthreads();  // (the first pass through also publishes frag and rowoff)
+    const uint64_t tn = t + gridDim.x;
+    if (tn < a.ntiles) {  // the next tile's loads overlap this tile's products
+      const uint64_t gnext = insert_zeros(tn, a.tbits, kT128Bits) | lanebits;
+#pragma clang loop unroll(full)
+      for (int k = 0; k < PER; ++k) nxt[k] = ld_nt(psi, tile128_addr(a, gnext, k));
+    }
+    // one 16-column group per wave at a time; the groups are disjoint, so each
+    // wave reads and writes only its own columns of the tile
+    for (int cg = wave; cg < NCG; cg += 4) {
+      const uint32_t colpart = tile128_col<K>(a, uint32_t(16 * cg + j));
+      doublex4 acc[3][MB];
+#pragma clang loop unroll(full)
+      for (int p = 0; p < 3; ++p)
+#pragma clang loop unroll(full)
+        for (int mb = 0; mb < MB; ++mb) acc[p][mb] = doublex4(0.0);
+#pragma clang loop unroll(full)
+      for (int st = 0; st < RPL; ++st) {
+        const c128 x = tile[swz128(rowoff[RPL * q + st] | colpart)];
+        const double b0 = x.re, b1 = x.im, b2 = x.re + x.im;
+#pragma clang loop unroll(full)
+        for (int mb = 0; mb < MB; ++mb) {
+          acc[0][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((0 * MB + mb) * RPL + st) * 64 + lane], b0,
+                                                            acc[0][mb], 0, 0, 0);
+          acc[1][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((1 * MB + mb) * RPL + st) * 64 + lane], b1,
+                                                            acc[1][mb], 0, 0, 0);
+          acc[2][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((2 * MB + mb) * RPL + st) * 64 + lane], b2,
+                                                            acc[2][mb], 0, 0, 0);
+        }
       }
-    } else {
-      // no register double buffer: the other resident waves cover the latency
-      // (addresses formed next to each load, not hoisted and held)
-      const c128* pin = psi + (insert_zeros(16 * g + j, a.pos, a.nfixed) | qin);
+      // rows 16 mb + 4 r + q of column j; this wave's reads of these columns
+      // fed the accumulators above
 #pragma clang loop unroll(full)
-      for (int s = 0; s < RPL; ++s) {
-        const c128* q = pin;
-        asm volatile("" : "+v"(q));
-        x[s] = ld_nt(q, a.off[s]);
-      }
+      for (int mb = 0; mb < MB; ++mb)
+#pragma clang loop unroll(full)
+        for (int r = 0; r < 4; ++r) {
+          c128 y;
+          y.re = acc[0][mb][r] - acc[1][mb][r];
+          y.im = (acc[2][mb][r] - acc[0][mb][r]) - acc[1][mb][r];
+          tile[swz128(rowoff[16 * mb + 4 * r + q] | colpart)] = y;
+        }
     }
-    doublex4 acc[3][MB];
+    __syncthreads();
 #pragma clang loop unroll(full)
-    for (int p = 0; p < 3; ++p)
-#pragma clang loop unroll(full)
-      for (int mb = 0; mb < MB; ++mb) acc[p][mb] = doublex4(0.0);
-#pragma clang loop unroll(full)
-    for (int s = 0; s < RPL; ++s) {
-      const double b0 = x[s].re, b1 = x[s].im, b2 = x[s].re + x[s].im;
-#pragma clang loop unroll(full)
-      for (int mb = 0; mb < MB; ++mb) {
-        acc[0][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((0 * MB + mb) * RPL + s) * 64 + lane], b0,
-                                                          acc[0][mb], 0, 0, 0);
-        acc[1][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((1 * MB + mb) * RPL + s) * 64 + lane], b1,
-                                                          acc[1][mb], 0, 0, 0);
-        acc[2][mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[((2 * MB + mb) * RPL + s) * 64 + lane], b2,
-                                                          acc[2][mb], 0, 0, 0);
-      }
-    }
-    c128* pout = psi + (insert_zeros(16 * g + j, a.pos, a.nfixed) | qout);
-#pragma clang loop unroll(full)
-    for (int mb = 0; mb < MB; ++mb)
-#pragma clang loop unroll(full)
-      for (int r = 0; r < 4; ++r) {
-        c128 y;
-        y.re = acc[0][mb][r] - acc[1][mb][r];
-        y.im = (acc[2][mb][r] - acc[0][mb][r]) - acc[1][mb][r];
-        c128* q = pout;
-        asm volatile("" : "+v"(q));
-        st_nt(q, a.off[16 * mb + 4 * r], y);
-      }
-    if (QSIM_FUSED_PREFETCH) {
-#pragma clang loop unroll(full)
-      for (int s = 0; s < RPL; ++s) x[s] = xn[s];
-    }
-    g = gn;
+    for (int k = 0; k < PER; ++k) st_nt(psi, tile128_addr(a, glow, k), tile[swz128(uint32_t(k) * 256u + threadIdx.x)]);
+    t = tn;
   }
-}
-
-// ---------------------------------------------------------------------------
-// complex64 blocks of 5 qubits straight from HBM (v_mfma_f32_32x32x2f32)
-// ---------------------------------------------------------------------------
-// Y = M X for 32 columns at a time (one wavefront) as the three real 32 x 32
-// products of the complex128 kernel above (Gauss), FP32 operands and FP32
-// accumulation: P1 = Re M Re X, P2 = Im M Im X, P3 = (Re M + Im M)(Re X +
-// Im X); Re Y = P1 - P2, Im Y = (P3 - P1) - P2. Each product is 16 K-steps of
-// the 32x32x2 MFMA.
-//
-// Operand layout of v_mfma_f32_32x32x2f32 (wave64):
-//   A (32 x 2):  lane l supplies A[l % 32][l / 32]
-//   B (2 x 32):  lane l supplies B[l / 32][l % 32]
-//   D (32 x 32): register j of lane l holds D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32]
-// Lane (c = l % 32, h = l / 32) therefore loads rows h, h + 2, ..., h + 30 of
-// column c (K-step s uses row 2 s + h) into registers and stores rows
-// 8 (j / 4) + 4 h + j % 4 of the same column. No LDS holds the state: a
-// column group belongs to one wavefront, whose loads complete before its
-// stores. The matrix fragments sit in LDS in lane order.
-template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QSIM_FUSED_WAVES)))
-void k_fused_mfma32_c64(const Mfma64Args a, const c64* __restrict__ mat, c64* __restrict__ psi) {
-  static_assert(K == 5, "32x32 blocks");
-  constexpr int D = 1 << K;
-  constexpr int KS = D / 2;  // K-steps per product
-  __shared__ float frag[3 * KS * 64];
-  // fragment (p, s), lane l: M_p[l % 32][2 s + l / 32], M_0 = Re M, M_1 = Im M, M_2 = Re M + Im M
-  for (int e = threadIdx.x; e < 3 * KS * 64; e += 256) {
-    const int l = e & 63, f = e >> 6;
-    const int st = f % KS, p = f / KS;
-    const c64 m = mat[(l & 31) * D + 2 * st + (l >> 5)];
-    frag[e] = p == 0 ? m.re : (p == 1 ? m.im : m.re + m.im);
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int c = lane & 31, h = lane >> 5;
-  const uint64_t hin = a.off[h];       // row h
-  const uint64_t hout = a.off[4 * h];  // row 4 h
-  const uint32_t nxcd = gridDim.x >= 8 ? 8u : 1u;
-  const uint32_t xcd = blockIdx.x % nxcd;
-  const uint64_t wpx = uint64_t(gridDim.x / nxcd) * 4;
-  const uint64_t w = uint64_t(blockIdx.x / nxcd) * 4 + (threadIdx.x >> 6);
-  const uint64_t chunk = (a.ngroups + nxcd - 1) / nxcd;
-  const uint64_t gend0 = (uint64_t(xcd) + 1) * chunk;
-  const uint64_t gend = gend0 < a.ngroups ? gend0 : a.ngroups;
-  for (uint64_t g = uint64_t(xcd) * chunk + w; g < gend; g += wpx) {
-    const uint64_t col = insert_zeros(32 * g + c, a.pos, a.nfixed);
-    // lane pointers; the row offsets are uniform (index bits disjoint from the
-    // column's, so OR = ADD), each address formed next to its access
-    const c64* pin = psi + (col | hin);
-    c64 x[KS];
-#pragma clang loop unroll(full)
-    for (int st = 0; st < KS; ++st) {
-      const c64* q = pin;
-      asm volatile("" : "+v"(q));  // keep the 16 addresses from being hoisted and held
-      x[st] = ld_nt(q, a.off[2 * st]);
-    }
-    floatx16 acc0 = floatx16(0.0f), acc1 = floatx16(0.0f), acc2 = floatx16(0.0f);
-#pragma clang loop unroll(full)
-    for (int st = 0; st < KS; ++st) {
-      const float b0 = x[st].re, b1 = x[st].im, b2 = x[st].re + x[st].im;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(0 * KS + st) * 64 + lane], b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(1 * KS + st) * 64 + lane], b1, acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(frag[(2 * KS + st) * 64 + lane], b2, acc2, 0, 0, 0);
-    }
-    c64* pout = psi + (col | hout);
-#pragma clang loop unroll(full)
-    for (int j = 0; j < 16; ++j) {
-      c64 y;
-      y.re = acc0[j] - acc1[j];
-      y.im = (acc2[j] - acc0[j]) - acc1[j];
-      c64* q = pout;
-      asm volatile("" : "+v"(q));
-      st_nt(q, a.off[8 * (j / 4) + (j % 4)], y);
-    }
-  }
-}
-
-static int launch_fused_mfma32_c64(void* state, int n, const int* targets, const void* mat, hipStream_t s) {
-  constexpr int K = 5;
-  Mfma64Args a;
-  std::memset(&a, 0, sizeof(a));
-  a.nfixed = sorted_positions(n, targets, K, nullptr, 0, a.pos);
-  a.ngroups = (uint64_t(1) << (n - K)) / 32;
-  for (int r = 0; r < (1 << K); ++r) {
-    uint64_t off = 0;
-    for (int t = 0; t < K; ++t)
-      if ((r >> (K - 1 - t)) & 1) off |= uint64_t(1) << (n - 1 - targets[t]);
-    a.off[r] = off;
-  }
-  static const unsigned resident = [] {
-    int nb = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_mfma32_c64<K>, 256, 0) != hipSuccess || nb < 1)
-      nb = 2;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 256;
-    return unsigned(nb) * unsigned(cus);
-  }();
-  const uint64_t want = (a.ngroups + 3) / 4;
-  unsigned grid = unsigned(want < resident ? want : resident);
-  if (grid >= 8) grid &= ~7u;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_fused_mfma32_c64<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c64*>(mat),
-                     reinterpret_cast<c64*>(state));
-  QSIM_CHECK_LAUNCH();
-  return QSIM_OK;
 }
 
 template <int K>
-static int launch_fused_mfma_c128(void* state, int n, const int* targets, const void* mat, hipStream_t s) {
-  Mfma64Args a;
+static int launch_fused_tile_c128(void* state, int n, const int* targets, const void* mat, hipStream_t s) {
+  Tile128Args a;
   std::memset(&a, 0, sizeof(a));
-  a.nfixed = sorted_positions(n, targets, K, nullptr, 0, a.pos);
-  a.ngroups = (uint64_t(1) << (n - K)) / 16;
-  for (int r = 0; r < (1 << K); ++r) {
-    uint64_t off = 0;
-    for (int t = 0; t < K; ++t)
-      if ((r >> (K - 1 - t)) & 1) off |= uint64_t(1) << (n - 1 - targets[t]);
-    a.off[r] = off;
+  a.ntiles = uint64_t(1) << (n - kT128Bits);
+  // tile bits: the K target bits plus the lowest non-target bits
+  bool is_target[QSIM_MAX_QUBITS] = {false};
+  for (int t = 0; t < K; ++t) is_target[n - 1 - targets[t]] = true;
+  uint8_t bits[kT128Bits];
+  int nb = 0, ncols = 0;
+  for (int b = 0; b < n && nb < kT128Bits; ++b) {
+    if (is_target[b]) {
+      bits[nb++] = uint8_t(b);
+    } else if (ncols < kT128Bits - K) {
+      bits[nb++] = uint8_t(b);
+      ++ncols;
+    }
   }
-  // persistent grid of the workgroups the device keeps resident (a multiple of 8)
+  if (nb != kT128Bits) {
+    set_error("qsim_apply_fused: tile bits do not fit the register");
+    return QSIM_EINVAL;
+  }
+  int nc = 0;
+  for (int i = 0; i < kT128Bits; ++i) {
+    a.tbits[i] = bits[i];
+    if (!is_target[bits[i]]) a.colbit[nc++] = uint8_t(i);
+  }
+  for (int t = 0; t < K; ++t) {
+    const int gb = n - 1 - targets[t];
+    for (int i = 0; i < kT128Bits; ++i)
+      if (bits[i] == gb) a.rowbit[t] = uint8_t(i);
+  }
+  // persistent grid: as many workgroups as the device keeps resident
   static const unsigned resident = [] {
-    int nb = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_mfma_c128<K>, 256, 0) != hipSuccess || nb < 1)
-      nb = 2;
+    int nbk = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, k_fused_tile_c128<K>, 256, 0) != hipSuccess || nbk < 1)
+      nbk = 2;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
       cus = 256;
-    return unsigned(nb) * unsigned(cus);
+    return unsigned(nbk) * unsigned(cus);
   }();
-  const uint64_t want = (a.ngroups + 3) / 4;  // one group per wavefront at least
-  unsigned grid = unsigned(want < resident ? want : resident);
-  if (grid >= 8) grid &= ~7u;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_fused_mfma_c128<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c128*>(mat),
+  const unsigned grid = unsigned(a.ntiles < resident ? a.ntiles : resident);
+  hipLaunchKernelGGL((k_fused_tile_c128<K>), dim3(grid), dim3(256), 0, s, a, reinterpret_cast<const c128*>(mat),
                      reinterpret_cast<c128*>(state));
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
@@ -561,11 +497,11 @@ extern "C" int qsim_apply_fused(void* state, int nqubits, int dtype, const int* 
   QSIM_CHECK_ARG(nqubits >= ntargets, "qsim_apply_fused: more targets than qubits");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128) {
-    if (nqubits >= 9 && ntargets == 5) return launch_fused_mfma_c128<5>(state, nqubits, targets, matrix_dev, s);
-    if (nqubits >= 8 && ntargets == 4) return launch_fused_mfma_c128<4>(state, nqubits, targets, matrix_dev, s);
+    if (nqubits >= kT128Bits && ntargets == 5) return launch_fused_tile_c128<5>(state, nqubits, targets, matrix_dev, s);
+    if (nqubits >= kT128Bits && ntargets == 4) return launch_fused_tile_c128<4>(state, nqubits, targets, matrix_dev, s);
     return dispatch_fused<double>(state, nqubits, targets, ntargets, matrix_dev, s);
   }
-  if (nqubits >= 10 && ntargets == 5) return launch_fused_mfma32_c64(state, nqubits, targets, matrix_dev, s);
+  if (nqubits >= kMfmaTileBits && ntargets == 5) return launch_fused_mfma<5>(state, nqubits, targets, matrix_dev, s);
   if (nqubits >= kMfmaTileBits && ntargets == 4) return launch_fused_mfma<4>(state, nqubits, targets, matrix_dev, s);
   return dispatch_fused<float>(state, nqubits, targets, ntargets, matrix_dev, s);
 }

@@ -290,11 +290,14 @@ class GPUExecutor:
         self._small = []                              # handles of small requests awaiting a batched launch
         self.batch_stream = torch.cuda.Stream(device=dev)
         self.stats = collections.Counter()            # path counts: graph_capture, graph_replay, direct, batched
+        # per completed request: (nqubits, host seconds from start to the last
+        # launch, seconds from then to completion seen by done())
+        self.timings = collections.deque(maxlen=1 << 16)
 
     # -- the executor interface ------------------------------------------------
     def start(self, req: Request):
         h = {"req": req, "sid": None, "event": None, "tail": None, "keep": None, "future": None,
-             "graph": None}
+             "graph": None, "t_start": time.perf_counter(), "t_host": None}
         if self.batch_small and req.nqubits < 12:
             h["future"] = self.pool.submit(self._compile_small, req)
             with self._lock:
@@ -319,6 +322,9 @@ class GPUExecutor:
             return False
         # read the result before the graph (and its pinned output) is released
         h["value"] = h["tail"].result()
+        t_done = time.perf_counter()
+        t_host = h["t_host"] if h["t_host"] is not None else t_done
+        self.timings.append((h["req"].nqubits, t_host - h["t_start"], t_done - t_host))
         h["keep"] = h["tail"] = None
         if h["sid"] is not None:
             self.free_streams.append(h["sid"])
@@ -362,6 +368,27 @@ class GPUExecutor:
         backend = self.backends[req.dtype]
         stream = self.streams[h["sid"]]
         n = req.nqubits
+        if n < blocking.TILE_BITS:
+            self._run_small_alone(req, h, backend, stream)
+            return
+        if not self.graphs or n > self.graph_max_qubits:
+            # large registers: the backend's own loop, which launches each pass
+            # as soon as it is compiled (planning overlaps the GPU) and replays
+            # the compiled passes of a repeated circuit
+            from . import _lib
+
+            code = _lib.C128 if req.dtype == "complex128" else _lib.C64
+            with torch.cuda.stream(stream):
+                state = backend._alloc_state(n, req.dtype)
+                state = backend.run_queue(list(req.circuit.queue), state, n, zero_pending=True)
+                tail = _Tail(backend, n, code, req.kind, req.nshots)
+                tail.set_uniforms(req.seed)
+                tail.launch(state, stream.cuda_stream)
+                ev = torch.cuda.Event()
+                ev.record(stream)
+            h.update(tail=tail, event=ev, keep=(state,), t_host=time.perf_counter())
+            self.stats["direct"] += 1
+            return
         code, plan, program = self._compile(backend, req)
         all_blocked = plan and all(p[0] == "block" for p in plan)
         signature = tuple((p[1], p[2], p[3]) for p in plan if p[0] == "block")
@@ -378,7 +405,7 @@ class GPUExecutor:
                     ev = torch.cuda.Event()
                     ev.record(stream)
                     g.busy = ev
-                    h.update(tail=g.tail, graph=g, event=ev)
+                    h.update(tail=g.tail, graph=g, event=ev, t_host=time.perf_counter())
                     self.stats["graph_replay"] += 1
                     return
                 g = self._capture(backend, req, key, code, plan, program, signature) if self._repeat(key) else None
@@ -388,7 +415,7 @@ class GPUExecutor:
                     ev = torch.cuda.Event()
                     ev.record(stream)
                     g.busy = ev
-                    h.update(tail=g.tail, graph=g, event=ev)
+                    h.update(tail=g.tail, graph=g, event=ev, t_host=time.perf_counter())
                     self.stats["graph_capture"] += 1
                     return
             # direct launches
@@ -417,7 +444,7 @@ class GPUExecutor:
             tail.launch(state, sptr)
             ev = torch.cuda.Event()
             ev.record(stream)
-            h.update(tail=tail, event=ev, keep=(state, prog))
+            h.update(tail=tail, event=ev, keep=(state, prog), t_host=time.perf_counter())
             self.stats["direct"] += 1
 
     @staticmethod
@@ -464,20 +491,23 @@ class GPUExecutor:
         graph = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream(device=backend.torch_device)
         side.wait_stream(torch.cuda.current_stream(backend.torch_device))
-        # thread-local capture: the other workers and the server loop keep
-        # launching and polling their own streams meanwhile
-        with torch.cuda.graph(graph, stream=side, capture_error_mode="thread_local"):
-            sptr = side.cuda_stream
-            prog_dev.copy_(prog_host, non_blocking=True)
-            zero = [True]
-            for p in plan:
-                _, tile_bits, off, count, ngates = p
-                launch = backend.lib.qsim_apply_blocked_from_zero if zero[0] else backend.lib.qsim_apply_blocked
-                zero[0] = False
-                rc = launch(state.data_ptr(), n, code, _lib.int_array(list(tile_bits)), prog_dev.data_ptr() + off,
-                            self._pass_bytes(plan, p, program), count, sptr)
-                _lib.check(rc, backend.lib)
-            tail.launch(state, sptr)
+        # Captured with capture_begin / capture_end on a side stream, not with
+        # the torch.cuda.graph context (whose entry synchronizes the device and
+        # empties the allocator cache: under load that stalled every request in
+        # flight). Thread-local capture: the other workers and the server loop
+        # keep launching and polling their own streams meanwhile.
+        with torch.cuda.stream(side):
+            graph.capture_begin(capture_error_mode="thread_local")
+            try:
+                self._launch_captured(backend, n, code, plan, program, state, prog_host, prog_dev, tail,
+                                      side.cuda_stream)
+            except BaseException:
+                try:
+                    graph.capture_end()  # leave the stream usable; the graph is dropped
+                except Exception:  # noqa: BLE001
+                    pass
+                raise
+            graph.capture_end()
         g = _Graph(signature, len(program), state, prog_host, prog_dev, tail, graph)
         g.busy = True
         with self._lock:
@@ -485,7 +515,59 @@ class GPUExecutor:
             self._graphs.move_to_end(key)
         return g
 
+    def _launch_captured(self, backend, n, code, plan, program, state, prog_host, prog_dev, tail, sptr):
+        """The launches a graph records: program upload, passes, measurement."""
+        from . import _lib
+
+        prog_dev.copy_(prog_host, non_blocking=True)
+        zero = True
+        for p in plan:
+            _, tile_bits, off, count, ngates = p
+            launch = backend.lib.qsim_apply_blocked_from_zero if zero else backend.lib.qsim_apply_blocked
+            zero = False
+            rc = launch(state.data_ptr(), n, code, _lib.int_array(list(tile_bits)), prog_dev.data_ptr() + off,
+                        self._pass_bytes(plan, p, program), count, sptr)
+            _lib.check(rc, backend.lib)
+        tail.launch(state, sptr)
+
     # -- requests below 12 qubits: one batched launch ----------------------------
+    def _run_small_alone(self, req, h, backend, stream):
+        """A request below 12 qubits on its own stream: the same padded
+        12-qubit pass as in a batched launch (so the results agree bitwise)."""
+        torch = self.torch
+        from . import _lib, blocking
+
+        code, compiled = self._compile_small(req)
+        n = req.nqubits
+        with torch.cuda.stream(stream):
+            sptr = stream.cuda_stream
+            if compiled is None:
+                state = backend.zero_state(n)
+                state = backend.run_queue(list(req.circuit.queue), state, n)
+                keep = (state,)
+            else:
+                program, count = compiled
+                tdt = torch.complex128 if code == _lib.C128 else torch.complex64
+                big = torch.zeros(2**blocking.TILE_BITS, dtype=tdt, device=self.device)
+                big[0] = 1
+                keep = (big,)
+                if count:
+                    host = torch.frombuffer(bytearray(program) + bytes(blocking.PAD_BYTES), dtype=torch.uint8)
+                    prog = host.pin_memory().to(self.device, non_blocking=True)
+                    rc = backend.lib.qsim_apply_blocked(big.data_ptr(), blocking.TILE_BITS, code,
+                                                        _lib.int_array(range(blocking.TILE_BITS)), prog.data_ptr(),
+                                                        len(program), count, sptr)
+                    _lib.check(rc, backend.lib)
+                    keep = (big, prog)
+                state = big[: 2**n]
+            tail = _Tail(backend, n, code, req.kind, req.nshots)
+            tail.set_uniforms(req.seed)
+            tail.launch(state, sptr)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        h.update(tail=tail, event=ev, keep=keep, t_host=time.perf_counter())
+        self.stats["small_alone"] += 1
+
     def _compile_small(self, req):
         from . import _lib, blocking
         from .backend import _Op
@@ -531,7 +613,7 @@ class GPUExecutor:
                     tail.launch(state, self.batch_stream.cuda_stream)
                     ev = torch.cuda.Event()
                     ev.record(self.batch_stream)
-                h.update(tail=tail, keep=(state,), event=ev, future=None)
+                h.update(tail=tail, keep=(state,), event=ev, future=None, t_host=time.perf_counter())
                 self.stats["small_alone"] += 1
                 continue
             groups[code].append((h, compiled))
@@ -556,7 +638,7 @@ class GPUExecutor:
                 if maxcount:
                     rc = backend.lib.qsim_apply_blocked_batched(
                         states.data_ptr(), nb, blocking.TILE_BITS, code,
-                        _lib.int_array(range(blocking.TILE_BITS)), prog.data_ptr(), maxbytes, stride, maxcount, sptr)
+                        _lib.int_array(range(blocking.TILE_BITS)), prog.data_ptr(), stride, maxbytes, maxcount, sptr)
                     _lib.check(rc, backend.lib)
                 evs = []
                 for i, (h, _) in enumerate(members):
@@ -564,12 +646,13 @@ class GPUExecutor:
                     tail = _Tail(backend, req.nqubits, code, req.kind, req.nshots)
                     tail.set_uniforms(req.seed)
                     tail.launch(states[i, : 2**req.nqubits], sptr)
-                    h.update(tail=tail, keep=(states, prog), future=None)
+                    h.update(tail=tail, keep=(states, prog), future=None, t_host=time.perf_counter())
                     evs.append(h)
                 ev = torch.cuda.Event()
                 ev.record(self.batch_stream)
             for h in evs:
                 h["event"] = ev
+                h["t_host"] = time.perf_counter()
             self.stats["batched"] += nb
             self.stats["batched_launches"] += 1
 
