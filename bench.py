@@ -262,7 +262,7 @@ def bench_precision(seed, n_time=32, n_acc=30, nlayers=20, nblocks=24):
     ba = _block_circuit(n_acc, nblocks, seed)
     blk["accuracy_c64"] = _accuracy(b64, b128, ba, ba)
     res["blocks5"] = blk
-    del blocks, ba
+    del ba
 
     checks = []
     for name, key, acc_key in (("layered, fuse(max_qubits=5)", "layered", "accuracy_c64_fused5"),

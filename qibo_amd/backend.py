@@ -172,6 +172,13 @@ def plan_matrix(matrix: np.ndarray, targets: Sequence[int], controls: Sequence[i
     return _Op("fused", targets, (), matrix)
 
 
+# plans of parameterless gates, keyed by (class, targets, controls,
+# controlled_by, dtype); an op is never modified after planning, only
+# annotated with caches that depend on nothing else (its record descriptor,
+# bit masks per register size)
+_SHARED_PLANS = {}
+
+
 class GatePlanner:
     """Host half of gate application: matrices and the kernel plan of a gate.
 
@@ -285,6 +292,16 @@ class GatePlanner:
             if (cached is not None and cached[0] is fp[0] and cached[1] is fp[1] and cached[2] is fp[2]
                     and cached[3] == fp[3] and cached[4] == fp[4]):
                 return cached[5]
+        # a gate without parameters has one plan per class, qubits and dtype,
+        # shared by every circuit (the analysis of a 4x4 matrix costs ~30 us)
+        skey = None
+        if base and not gate._parameters:
+            skey = (type(gate).__name__, gate._target_qubits, gate._control_qubits, gate.is_controlled_by,
+                    self.dtype)
+            op = _SHARED_PLANS.get(skey)
+            if op is not None:
+                gate.__dict__["_qplan"] = fp + (op,)
+                return op
         m = self.matrix(gate) if base else gate.matrix(self)
         if gate.is_controlled_by:
             op = plan_matrix(m, gate.target_qubits, gate.control_qubits)
@@ -292,6 +309,10 @@ class GatePlanner:
             op = plan_matrix(m, gate.qubits, ())
         if base and not any(isinstance(x, (np.ndarray, torch.Tensor)) for x in gate._parameters):
             gate.__dict__["_qplan"] = fp + (op,)
+        if skey is not None:
+            if len(_SHARED_PLANS) >= 65536:
+                _SHARED_PLANS.clear()
+            _SHARED_PLANS[skey] = op
         return op
 
     def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:

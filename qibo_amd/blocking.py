@@ -53,7 +53,8 @@ lane):
 * ROT: an unconditional one-qubit rotation [[c, -s], [s, c]] (RY), reflection
   [[c, s], [s, -c]] (H) or [[c, -is], [-is, c]] (RX) on a register bit as
   three shears x0 -= t x1; x1 += s x0; x0 -= t x1 with t = s / (1 + c)
-  (Paeth): three multiply-adds per real pair instead of four operations. The
+  (Paeth): three multiply-adds per real pair instead of four operations.
+  (t, s) travel in the record header (no entries: no scalar load of its own). The
   shears need c >= 0; a gate with c < 0 is applied as -(its negation), and
   the pass's net sign is folded into the data of one unconditional linear
   record of the pass (or, if it has none, a final sign flip).
@@ -331,6 +332,18 @@ def rotation_form(m: np.ndarray):
     if neg:
         c, s = -c, -s
     return mode, s / (1.0 + c), s, neg
+
+
+def _pack_rot(buf: bytearray, arg: int, t: float, s: float, cdtype):
+    """Append a ROT record: no entries; (t, s) ride in the header, in the
+    fields a ROT does not use (cmask and bits), so the kernel has them with
+    the header it reads one record ahead: complex128 t = bytes 8-15, s =
+    bytes 16-23 (float64); complex64 t = bytes 8-11, s = bytes 16-19 (float32)."""
+    if np.dtype(cdtype) == np.complex128:
+        tb, sb = struct.pack("<d", t), struct.pack("<d", s)
+    else:
+        tb, sb = struct.pack("<f", t) + bytes(4), struct.pack("<f", s) + bytes(4)
+    buf += bytes((OP_ROT, arg, 0, 0)) + bytes(4) + tb + sb + bytes(8)
 
 
 def _pack(buf: bytearray, op: int, arg: int, cmask: int, nt: int, bits: Sequence[int],
@@ -660,7 +673,8 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             rot = None if (ordered or cm or rreq) else rotation_form(m)
             if rot is not None:
                 mode, t, sn, neg = rot
-                emit(OP_ROT, j | (mode << 2), 0, 0, (), np.array([complex(t, sn)]))
+                _pack_rot(buf, j | (mode << 2), t, sn, cdtype)
+                count += 1
                 sign ^= neg
                 continue
             real = (not ordered) and not np.any(np.imag(m))

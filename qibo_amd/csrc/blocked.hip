@@ -68,7 +68,8 @@ enum : uint32_t {
   kOpNeg = 7,
   kOpInner1 = 8,     // adjoint sweep only: Re <w| M |v> for a 2x2 M on a register bit
   kOpInnerDiag = 9,  // adjoint sweep only: Re <w| diag |v> on up to 5 bits
-  kOpRot = 10,       // unconditional 2x2 rotation / reflection on a register bit, as three shears
+  kOpRot = 10,       // unconditional 2x2 rotation / reflection on a register bit, as three shears;
+                     // no entries: t, s in the header (cmask, bits fields), float64 or float32
   kOpLaneRegPhase = 11,  // per-lane factor Q of the registers with one bit set (kept in LDS)
   kOpQFlush = 12,        // apply Q to the registers with bit arg set, reset it to 1
   kOpNop = 0xFF          // no effect (pads the programs of a batched launch to one record count)
@@ -740,7 +741,15 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
         }
         break;
       case kOpRot: {
-        const R t = me[0].re, sn = me[0].im;
+        // (t, s) ride in the header (cmask and bits fields): no entry load
+        R t, sn;
+        if constexpr (sizeof(R) == 8) {
+          t = __longlong_as_double(static_cast<long long>(h.cmask));
+          sn = __hiloint2double(int(h.b1), int(h.b0));
+        } else {
+          t = __uint_as_float(uint32_t(h.cmask));
+          sn = __uint_as_float(h.b0);
+        }
         const uint32_t mode = h.arg >> 2;
         switch (h.arg & 3) {
           case 0: rot_regs<R, 0>(v, mode, t, sn); break;

@@ -20,7 +20,8 @@
 //     multiplies only the registers whose bits the table depends on), -1 phases become sign flips (NEG), controlled phases between one
 //     register bit and lane bits become per-lane factors with a flush (QFLUSH),
 //     unconditional rotations / reflections become three-shear ROT records,
-//     other one-qubit gates DENSE1(_REAL), two- and three-qubit gates
+//     (their t and s in the header), other one-qubit gates DENSE1(_REAL),
+//     two- and three-qubit gates
 //     DENSE_LDS; with `ordered` only the exact per-gate forms are used;
 //  4. a net -1 left by rotations with c < 0 is folded into the entries of the
 //     first unconditional linear record, else a final sign flip.
@@ -130,6 +131,29 @@ struct Writer {
       }
     }
     len += rec;
+    ++count;
+  }
+
+  // ROT record: no entries, (t, s) in the header's cmask and bits fields
+  // (float64 at bytes 8 and 16; complex64: float32 at bytes 8 and 16)
+  void emit_rot(int arg, double t, double s) {
+    if (len + kHeader > cap) {
+      overflow = true;
+      return;
+    }
+    uint8_t* h = out + len;
+    std::memset(h, 0, kHeader);
+    h[0] = kRRot;
+    h[1] = uint8_t(arg);
+    if (itemsize == 16) {
+      std::memcpy(h + 8, &t, 8);
+      std::memcpy(h + 16, &s, 8);
+    } else {
+      const float tf = float(t), sf = float(s);
+      std::memcpy(h + 8, &tf, 4);
+      std::memcpy(h + 16, &sf, 4);
+    }
+    len += kHeader;
     ++count;
   }
 
@@ -437,8 +461,7 @@ int compile(int n, const int* tile_bits, std::vector<Op>& items, bool ordered, W
       double t, sn;
       bool neg;
       if (!ordered && cm == 0 && rreq == 0 && rotation_form(o.data, &mode, &t, &sn, &neg)) {
-        const C e = {t, sn};
-        w.emit(kRRot, j | (mode << 2), 0, 0, nullptr, 0, &e, 1);
+        w.emit_rot(j | (mode << 2), t, sn);
         sign ^= neg;
         continue;
       }

@@ -339,7 +339,9 @@ class GPUExecutor:
 
     def idle(self):
         self.flush_small()
-        time.sleep(50e-6)
+        # the loop polls completions; a longer nap leaves the interpreter to
+        # the worker threads' planning (the polling thread competes for it)
+        time.sleep(500e-6)
 
     def close(self):
         self.pool.shutdown(wait=True)

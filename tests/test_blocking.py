@@ -6,6 +6,7 @@ reproduce the oracle's gate-by-gate state; the partition must keep every pair
 of non-commuting gates in circuit order.
 """
 
+import copy
 import numpy as np
 import pytest
 
@@ -30,6 +31,14 @@ def decode(program: bytes, first: int, count: int, itemsize: int):
         assert program[off + 24: off + 32] == bytes(8)
         size = blocking.record_bytes(nent, itemsize)
         mat = np.frombuffer(program[off + 32: off + 32 + nent * itemsize], dtype=cdt)
+        if op == blocking.OP_ROT:
+            # (t, s) in the header: float64 (complex128) or float32 at bytes 8 and 16
+            assert nent == 0
+            rdt = "<f8" if itemsize == 16 else "<f4"
+            t = float(np.frombuffer(program[off + 8: off + 16], dtype=rdt)[0])
+            s = float(np.frombuffer(program[off + 16: off + 24], dtype=rdt)[0])
+            mat = np.array([complex(t, s)], dtype=cdt)
+            cmask, bits = 0, [0] * 8
         out.append((op, arg, cmask, nt, bits, rreq, mat))
         off += size
     return out
@@ -232,10 +241,16 @@ def test_compiled_passes_reproduce_oracle(which, rng):
     assert any(e[0] == "block" for e in plan)
 
 
+def _distinct(ops):
+    """Copies of the planned ops, one object per gate (parameterless gates on
+    the same qubits share one plan object), so identities name positions."""
+    return [copy.copy(op) for op in ops]
+
+
 def test_partition_respects_order():
     n = 16
     c = _circuits(n)[2]
-    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    ops = _distinct([GatePlanner().plan_gate(g) for g in c.queue])
     index = {id(op): i for i, op in enumerate(ops)}
     order = []
     for kind, payload in blocking.partition(ops, n, 3):
@@ -263,7 +278,7 @@ def test_layered_30q_pass_count():
 def test_ordered_partition_keeps_circuit_order():
     n = 16
     c = _circuits(n)[2]
-    ops = [GatePlanner().plan_gate(g) for g in c.queue]
+    ops = _distinct([GatePlanner().plan_gate(g) for g in c.queue])
     order = []
     for kind, payload in blocking.partition(ops, n, 3, ordered=True):
         order.extend([payload] if kind == "single" else payload[1])

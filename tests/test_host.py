@@ -368,3 +368,21 @@ def test_standin_measurement_bookkeeping():
     assert np.array_equal(m2.result.samples(), samples[:, [2]])
     regs = out.samples(registers=True)
     assert np.array_equal(regs["a"], samples[:, [0, 1]]) and np.array_equal(regs["b"], samples[:, [2]])
+
+
+def test_shared_plans_of_parameterless_gates():
+    """Gates without parameters share one plan per class, qubits and dtype;
+    parametrized gates, other qubits, controls and dtypes get their own."""
+    from qibo_amd.backend import GatePlanner
+
+    g128, g64 = GatePlanner("complex128"), GatePlanner("complex64")
+    a, b = g128.plan_gate(gates.CZ(3, 4)), g128.plan_gate(gates.CZ(3, 4))
+    assert a is b
+    assert g128.plan_gate(gates.CZ(4, 5)) is not a
+    assert g64.plan_gate(gates.CZ(3, 4)) is not a
+    h = g128.plan_gate(gates.H(4))
+    hc = g128.plan_gate(gates.H(4).controlled_by(3))
+    assert hc is not h and hc.controls == (3,) and h.controls == ()
+    assert g128.plan_gate(gates.RY(3, 0.5)) is not g128.plan_gate(gates.RY(3, 0.5))
+    cn = g128.plan_gate(gates.CNOT(1, 2))
+    assert cn.kind == "dense" and cn.controls == (1,) and cn.targets == (2,)
