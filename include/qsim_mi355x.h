@@ -98,8 +98,10 @@ int qsim_apply_swap(void* state, int nqubits, int dtype, int q0, int q1, const i
  * (nqubits >= 11, 2048-amplitude LDS tiles): v_mfma_f64_16x16x4f64 with FP64
  * accumulation, as three real products (Re M Re X, Im M Im X,
  * (Re M + Im M)(Re X + Im X)); complex64
- * blocks of 4-5 qubits (nqubits >= 12): v_mfma_f32_32x32x2f32 on the real
- * embedding [[Re, -Im], [Im, Re]] with FP32 accumulation; smaller blocks:
+ * blocks of 4-5 qubits (nqubits >= 12, 4096-amplitude LDS tiles):
+ * v_mfma_f32_32x32x2f32 with FP32 accumulation, on the same three real
+ * products for 5 qubits and on the real embedding [[Re, -Im], [Im, Re]] for
+ * 4; smaller blocks:
  * FMA per column in the state's precision. */
 int qsim_apply_fused(void* state, int nqubits, int dtype, const int* targets, int ntargets,
                      const void* matrix_dev, void* stream);

@@ -85,10 +85,12 @@ static int dispatch_fused(void* state, int n, const int* targets, int k, const v
 // complex64 blocks of k = 4 or 5 qubits on MFMA (v_mfma_f32_32x32x2f32)
 // ---------------------------------------------------------------------------
 // The block acts on the 2^k-amplitude "columns" of the state (one column per
-// setting of the non-target bits). With the real embedding
+// setting of the non-target bits). For k = 4, with the real embedding
 //     [y_re; y_im] = [[M_re, -M_im], [M_im, M_re]] [x_re; x_im]
 // a batch of 32 columns is a (2D x 2D) x (2D x 32) real product, D = 2^k,
-// computed by 32x32x2 FP32 MFMA steps with FP32 accumulation. A workgroup
+// computed by 32x32x2 FP32 MFMA steps with FP32 accumulation; for k = 5 the
+// three 32 x 32 real products of the Gauss form (see the complex128 kernel)
+// take 48 MFMA steps per 32 columns instead of 64. A workgroup
 // stages a tile of 4096 amplitudes (k target bits + 12-k column bits, the
 // column bits being the lowest non-target bits so that loads are coalesced)
 // in LDS, each wavefront multiplies groups of 32 columns, and the tile is
@@ -150,15 +152,27 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
   constexpr int NCG = NCOL / 32;        // 32-column groups (4 for K = 5, 8 for K = 4)
   __shared__ c64 tile[1 << kMfmaTileBits];
   __shared__ uint32_t rowoff[D];  // local-index bits of matrix row r
-  // the real embedding, column-major (element (i, k) at k * 2D + i): the A
-  // operand of step (mb, ks) is two consecutive 32-float runs
-  __shared__ float amat[4 * D * D];
+  // K = 4: the real embedding, column-major (element (i, k) at k * 2D + i):
+  // the A operand of step (mb, ks) is two consecutive 32-float runs.
+  // K = 5: the three real 32 x 32 matrices of the Gauss form (below) in lane
+  // order: fragment (p, ks), lane l = M_p[l % 32][2 ks + l / 32] with M_0 =
+  // Re M, M_1 = Im M, M_2 = Re M + Im M.
+  constexpr bool GAUSS = K == 5;
+  constexpr int NA = GAUSS ? 3 * (D / 2) * 64 : 4 * D * D;
+  __shared__ float amat[NA];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (threadIdx.x < D) rowoff[threadIdx.x] = mfma_local<K>(a, threadIdx.x, 0);
-  for (int e = threadIdx.x; e < 4 * D * D; e += 256) {
-    const int k = e / (2 * D), i = e % (2 * D);
-    const c64 m = mat[(i % D) * D + (k % D)];
-    amat[e] = (i < D) ? ((k < D) ? m.re : -m.im) : ((k < D) ? m.im : m.re);
+  for (int e = threadIdx.x; e < NA; e += 256) {
+    if constexpr (GAUSS) {
+      const int l = e & 63, f = e >> 6;
+      const int ks = f % (D / 2), pr = f / (D / 2);
+      const c64 m = mat[(l & 31) * D + 2 * ks + (l >> 5)];
+      amat[e] = pr == 0 ? m.re : (pr == 1 ? m.im : m.re + m.im);
+    } else {
+      const int k = e / (2 * D), i = e % (2 * D);
+      const c64 m = mat[(i % D) * D + (k % D)];
+      amat[e] = (i < D) ? ((k < D) ? m.re : -m.im) : ((k < D) ? m.im : m.re);
+    }
   }
   // lane part (local bits 0-7) of this thread's slice element, as global bits
   uint64_t lanebits = 0;
@@ -192,6 +206,30 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
       // keeps the compiler from hoisting all 64 of them into registers)
       int alane = lane;
       asm volatile("" : "+v"(alane));
+      if constexpr (GAUSS) {
+        // Y = M X as three real products (Gauss): P1 = Re M Re X, P2 = Im M
+        // Im X, P3 = (Re M + Im M)(Re X + Im X); Re Y = P1 - P2, Im Y =
+        // (P3 - P1) - P2. 48 MFMA steps per 32 columns instead of the 64 of
+        // the real embedding.
+        floatx16 acc0 = floatx16(0.0f), acc1 = floatx16(0.0f), acc2 = floatx16(0.0f);
+#pragma clang loop unroll(full)
+        for (int ks = 0; ks < D / 2; ++ks) {
+          const c64 x = tile[mfma_slot(rowoff[2 * ks + (lane >> 5)] | colpart)];
+          const float b0 = x.re, b1 = x.im, b2 = x.re + x.im;
+          acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(amat[(0 * (D / 2) + ks) * 64 + alane], b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(amat[(1 * (D / 2) + ks) * 64 + alane], b1, acc1, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(amat[(2 * (D / 2) + ks) * 64 + alane], b2, acc2, 0, 0, 0);
+        }
+#pragma clang loop unroll(full)
+        for (int j = 0; j < 16; ++j) {
+          const int row = 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
+          c64 y;
+          y.re = acc0[j] - acc1[j];
+          y.im = (acc2[j] - acc0[j]) - acc1[j];
+          tile[mfma_slot(rowoff[row] | colpart)] = y;
+        }
+        continue;
+      }
       floatx16 acc[MB];
 #pragma clang loop unroll(full)
       for (int mb = 0; mb < MB; ++mb) acc[mb] = floatx16(0.0f);
