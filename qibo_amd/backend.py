@@ -172,6 +172,10 @@ def plan_matrix(matrix: np.ndarray, targets: Sequence[int], controls: Sequence[i
     return _Op("fused", targets, (), matrix)
 
 
+# a fused group of at most 3 qubits with at most this many gates runs as its
+# gates inside the blocked passes (GatePlanner.plan_ops)
+UNFUSE_MAX_GATES = 12
+
 # plans of parameterless gates, keyed by (class, targets, controls,
 # controlled_by, dtype); an op is never modified after planning, only
 # annotated with caches that depend on nothing else (its record descriptor,
@@ -314,6 +318,20 @@ class GatePlanner:
                 _SHARED_PLANS.clear()
             _SHARED_PLANS[skey] = op
         return op
+
+    def plan_ops(self, gate) -> List[_Op]:
+        """The ops that apply ``gate`` inside blocked passes. A fused group of
+        at most 3 qubits and at most ``UNFUSE_MAX_GATES`` gates whose gates are
+        all blockable runs as its gates (rotations, phase tables, sign flips
+        in registers) instead of its dense 4x4 / 8x8 product, which a pass
+        can only apply through an LDS gather (about 14 rotations' worth): the
+        same unitary, rounded differently. Larger groups keep their dense
+        matrix (4-5 qubits: one MFMA pass)."""
+        if kinds.is_fused(gate) and len(gate.target_qubits) <= 3 and len(gate.gates) <= UNFUSE_MAX_GATES:
+            parts = [self.plan_gate(g) for g in gate.gates]
+            if all(blocking.blockable(op) or op.kind == "noop" for op in parts):
+                return parts
+        return [self.plan_gate(gate)]
 
     def channel_transfer_matrix(self, channel) -> Optional[np.ndarray]:
         """The map of a channel on one or two qubits T as a 4^k x 4^k matrix
@@ -969,7 +987,7 @@ class MI355XBackend(GatePlanner):
                     return
                 if gate.symbolic_parameters:  # conditioned on an earlier collapse
                     gate.substitute_symbols()
-                yield self.plan_gate(gate)
+                yield from self.plan_ops(gate)
 
         while True:
             state = self._run_blocked(segment(), state, nqubits, zero, record)
@@ -1119,7 +1137,7 @@ class MI355XBackend(GatePlanner):
                 return state
             if gate.symbolic_parameters:
                 gate.substitute_symbols()
-            ops.append(self.plan_gate(gate))
+            ops.extend(self.plan_ops(gate))
         return self._run_ops(ops, state, nqubits)
 
     def _run_ops(self, ops, state: torch.Tensor, nqubits: int) -> torch.Tensor:

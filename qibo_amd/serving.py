@@ -356,7 +356,7 @@ class GPUExecutor:
         from . import _lib, blocking
 
         n = req.nqubits
-        ops = [backend.plan_gate(g) for g in req.circuit.queue]
+        ops = [op for g in req.circuit.queue for op in backend.plan_ops(g)]
         code = _lib.C128 if req.dtype == "complex128" else _lib.C64
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         plan, program = blocking.compile_circuit(ops, n, cdtype, backend._low_bits(code),
@@ -579,9 +579,9 @@ class GPUExecutor:
         pad = blocking.TILE_BITS - n
         ops = []
         for g in req.circuit.queue:
-            op = backend.plan_gate(g)
-            ops.append(_Op(op.kind, tuple(q + pad for q in op.targets), tuple(q + pad for q in op.controls),
-                           op.data))
+            for op in backend.plan_ops(g):
+                ops.append(_Op(op.kind, tuple(q + pad for q in op.targets), tuple(q + pad for q in op.controls),
+                               op.data))
         code = _lib.C128 if req.dtype == "complex128" else _lib.C64
         cdtype = np.complex128 if code == _lib.C128 else np.complex64
         plan, program = blocking.compile_circuit(ops, blocking.TILE_BITS, cdtype, backend._low_bits(code),

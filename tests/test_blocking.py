@@ -410,3 +410,23 @@ def test_rotation_sign_absorbed(rng):
     _, program = blocking.compile_circuit(ops, n, np.complex128, 3)
     kinds = [r[0] for r in decode(program, 0, plan[0][3], 16)]
     assert blocking.OP_NEG not in kinds and blocking.OP_REG_PHASE in kinds
+
+
+def test_small_fused_groups_run_as_their_gates(rng):
+    """A fused group of at most 3 qubits is planned as its gates (GatePlanner
+    .plan_ops); the passes built from them reproduce the oracle's state of the
+    unfused circuit, and groups of 4-5 qubits keep one dense op."""
+    n = 14
+    c = models.random_layered_circuit(n, 6, seed=4, measure=False)
+    fused = c.fuse(max_qubits=5)
+    gp = GatePlanner()
+    ops = [op for g in fused.queue for op in gp.plan_ops(g)]
+    assert len(ops) == c.ngates  # every 2-3-qubit group unfolded
+    psi = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi /= np.linalg.norm(psi)
+    got, _ = run_plan(ops, n, psi)
+    want = OracleBackend().execute(c, psi)
+    assert np.max(np.abs(got - want)) < 1e-12
+    big = models.QFT(12).fuse(max_qubits=5)
+    five = [g for g in big.queue if len(g.qubits) == 5]
+    assert five and all(len(gp.plan_ops(g)) == 1 for g in five)
