@@ -130,13 +130,27 @@ __device__ __forceinline__ uint32_t mfma_local(const MfmaArgs& a, uint32_t r, ui
 // stay conflict-free (a permutation inside each 32-slot row).
 __device__ __forceinline__ uint32_t mfma_slot(uint32_t u) { return u ^ ((u >> 5) & 31u); }
 
-// Global address of element k * 256 + thread of tile t (coalesced slices).
+// Staging moves pairs of neighbouring amplitudes (16 bytes per lane): the
+// tile's lowest local bit is global bit 0 (the lowest index bit is always a
+// target or a column bit), so elements 2p and 2p + 1 are adjacent in memory.
+// Global index of element 2 (k * 256 + thread) of the tile whose local bits
+// 1-8 (the thread) are already in glow (coalesced 4 KB slices).
 __device__ __forceinline__ uint64_t mfma_addr(const MfmaArgs& a, uint64_t glow, int k) {
   uint64_t g = glow;
 #pragma clang loop unroll(full)
-  for (int i = 0; i < 4; ++i)
-    if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[8 + i];
+  for (int i = 0; i < 3; ++i)
+    if ((k >> i) & 1) g |= uint64_t(1) << a.tbits[9 + i];
   return g;
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 ld_pair(const c64* psi, uint64_t i) {
+  return __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(psi + i));
+}
+
+__device__ __forceinline__ void st_pair(c64* psi, uint64_t i, floatx4 v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<floatx4*>(psi + i));
 }
 
 // Persistent workgroups, software-pipelined: while the MFMA product of tile t
@@ -174,29 +188,33 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
       amat[e] = (i < D) ? ((k < D) ? m.re : -m.im) : ((k < D) ? m.im : m.re);
     }
   }
-  // lane part (local bits 0-7) of this thread's slice element, as global bits
+  // lane part (local bits 1-8) of this thread's slice pair, as global bits
   uint64_t lanebits = 0;
 #pragma clang loop unroll(full)
   for (int i = 0; i < 8; ++i)
-    if ((threadIdx.x >> i) & 1) lanebits |= uint64_t(1) << a.tbits[i];
+    if ((threadIdx.x >> i) & 1) lanebits |= uint64_t(1) << a.tbits[i + 1];
   uint64_t t = blockIdx.x;
-  c64 nxt[16];
+  floatx4 nxt[8];  // pair k: elements 2 (k * 256 + thread) and 2 (k * 256 + thread) + 1
   if (t < a.ntiles) {
     const uint64_t glow = insert_zeros(t, a.tbits, kMfmaTileBits) | lanebits;
 #pragma clang loop unroll(full)
-    for (int k = 0; k < 16; ++k) nxt[k] = ld_nt(psi, mfma_addr(a, glow, k));
+    for (int k = 0; k < 8; ++k) nxt[k] = ld_pair(psi, mfma_addr(a, glow, k));
   }
   while (t < a.ntiles) {
     const uint64_t glow = insert_zeros(t, a.tbits, kMfmaTileBits) | lanebits;
     // each thread writes (and later reads back) only its own slots here
 #pragma clang loop unroll(full)
-    for (int k = 0; k < 16; ++k) tile[mfma_slot(k * 256 + threadIdx.x)] = nxt[k];
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t e = 2u * (uint32_t(k) * 256u + threadIdx.x);
+      tile[mfma_slot(e)] = c64{nxt[k].x, nxt[k].y};
+      tile[mfma_slot(e + 1)] = c64{nxt[k].z, nxt[k].w};
+    }
     __syncthreads();
     const uint64_t tn = t + gridDim.x;
     if (tn < a.ntiles) {  // next tile's loads overlap this tile's product
       const uint64_t gnext = insert_zeros(tn, a.tbits, kMfmaTileBits) | lanebits;
 #pragma clang loop unroll(full)
-      for (int k = 0; k < 16; ++k) nxt[k] = ld_nt(psi, mfma_addr(a, gnext, k));
+      for (int k = 0; k < 8; ++k) nxt[k] = ld_pair(psi, mfma_addr(a, gnext, k));
     }
     // one 32-column group per wave at a time; the groups are disjoint, so each
     // wave reads and writes only its own columns of the tile
@@ -262,7 +280,11 @@ __global__ __launch_bounds__(256) void k_fused_mfma_c64(const MfmaArgs a, const 
     }
     __syncthreads();
 #pragma clang loop unroll(full)
-    for (int k = 0; k < 16; ++k) st_nt(psi, mfma_addr(a, glow, k), tile[mfma_slot(k * 256 + threadIdx.x)]);
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t e = 2u * (uint32_t(k) * 256u + threadIdx.x);
+      const c64 y0 = tile[mfma_slot(e)], y1 = tile[mfma_slot(e + 1)];
+      st_pair(psi, mfma_addr(a, glow, k), floatx4{y0.re, y0.im, y1.re, y1.im});
+    }
     t = tn;
   }
 }

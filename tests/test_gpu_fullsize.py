@@ -10,8 +10,9 @@ Size-independent properties where the oracle cannot run the whole circuit:
 * the 32-qubit complex64 layered circuit grouped by ``fuse(max_qubits=5)``
   (32 GiB state) followed by the unfused inverse returns |0...0> to complex64
   accuracy. On this circuit qibo's fusion forms 2- and 3-qubit groups only,
-  which run as dense records inside blocked passes; the dense 5-qubit block
-  kernels at 32 qubits are covered by tests/test_gpu_configs.py.
+  which run as their gates inside blocked passes (GatePlanner.plan_ops); the
+  dense 5-qubit block kernels at 32 qubits are covered by
+  tests/test_gpu_configs.py.
 """
 
 import numpy as np
