@@ -294,6 +294,10 @@ class GPUExecutor:
         # launch, seconds from then to completion seen by done())
         self.timings = collections.deque(maxlen=1 << 16)
 
+    def _count(self, key, k=1):
+        with self._lock:  # worker threads count too
+            self.stats[key] += k
+
     # -- the executor interface ------------------------------------------------
     def start(self, req: Request):
         h = {"req": req, "sid": None, "event": None, "tail": None, "keep": None, "future": None,
@@ -389,7 +393,7 @@ class GPUExecutor:
                 ev = torch.cuda.Event()
                 ev.record(stream)
             h.update(tail=tail, event=ev, keep=(state,), t_host=time.perf_counter())
-            self.stats["direct"] += 1
+            self._count("direct", 1)
             return
         code, plan, program = self._compile(backend, req)
         all_blocked = plan and all(p[0] == "block" for p in plan)
@@ -408,7 +412,7 @@ class GPUExecutor:
                     ev.record(stream)
                     g.busy = ev
                     h.update(tail=g.tail, graph=g, event=ev, t_host=time.perf_counter())
-                    self.stats["graph_replay"] += 1
+                    self._count("graph_replay", 1)
                     return
                 g = self._capture(backend, req, key, code, plan, program, signature) if self._repeat(key) else None
                 if g is not None:
@@ -418,7 +422,7 @@ class GPUExecutor:
                     ev.record(stream)
                     g.busy = ev
                     h.update(tail=g.tail, graph=g, event=ev, t_host=time.perf_counter())
-                    self.stats["graph_capture"] += 1
+                    self._count("graph_capture", 1)
                     return
             # direct launches
             state = backend._alloc_state(n, req.dtype)
@@ -447,7 +451,7 @@ class GPUExecutor:
             ev = torch.cuda.Event()
             ev.record(stream)
             h.update(tail=tail, event=ev, keep=(state, prog), t_host=time.perf_counter())
-            self.stats["direct"] += 1
+            self._count("direct", 1)
 
     @staticmethod
     def _pass_bytes(plan, p, program):
@@ -568,7 +572,7 @@ class GPUExecutor:
             ev = torch.cuda.Event()
             ev.record(stream)
         h.update(tail=tail, event=ev, keep=keep, t_host=time.perf_counter())
-        self.stats["small_alone"] += 1
+        self._count("small_alone", 1)
 
     def _compile_small(self, req):
         from . import _lib, blocking
@@ -616,7 +620,7 @@ class GPUExecutor:
                     ev = torch.cuda.Event()
                     ev.record(self.batch_stream)
                 h.update(tail=tail, keep=(state,), event=ev, future=None, t_host=time.perf_counter())
-                self.stats["small_alone"] += 1
+                self._count("small_alone", 1)
                 continue
             groups[code].append((h, compiled))
         for code, members in groups.items():
@@ -655,8 +659,8 @@ class GPUExecutor:
             for h in evs:
                 h["event"] = ev
                 h["t_host"] = time.perf_counter()
-            self.stats["batched"] += nb
-            self.stats["batched_launches"] += 1
+            self._count("batched", nb)
+            self._count("batched_launches", 1)
 
 
 def poisson_stream(nrequests: int, qps: float, seed: int, nmin: int = 10, nmax: int = 30,
