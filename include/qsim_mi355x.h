@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 9
+#define QSIM_ABI_VERSION 10
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -229,6 +229,30 @@ int qsim_divide_by(const void* probs, int prob_dtype, int64_t n, const void* div
  * normalised cumulative sum, for nshots device uniforms u in [0, 1). */
 int qsim_search_sorted(const double* cdf, int64_t n, const double* uniforms, int64_t nshots,
                        int64_t* out, void* stream);
+
+/* Backend.sample_shots (abstract.py:2774-2781: random_choice -> numpy's
+ * np.random.choice(n, nshots, p=probs)) in one call, without writing the
+ * cumulative sum: out[k] equals qsim_search_sorted on qsim_cumsum_exact's cdf
+ * for the same uniforms (bitwise: the same exact sequential sums, read where
+ * the bisection compares them). stats (device, 3 doubles) receives the
+ * checks of qsim_prob_stats: sum, number of negative entries, number of
+ * NaNs. Probabilities with a negative entry, a NaN or an infinite sum are not
+ * sampled (out is filled with 0): the caller raises as np.random.choice does.
+ * `workspace` must hold qsim_sample_workspace(n) bytes (about 9 bytes per
+ * probability). */
+size_t qsim_sample_workspace(int64_t n);
+int qsim_sample_exact(const void* probs, int prob_dtype, int64_t n, const void* tile_stats, const double* uniforms,
+                      int64_t nshots, int64_t* out, double* stats, void* workspace, size_t workspace_bytes,
+                      void* stream);
+/* Probabilities of all qubits in order, as qsim_probabilities with
+ * qubits = 0..nqubits-1 (Backend.calculate_probabilities,
+ * abstract.py:2734-2758), and in the same pass the per-4096-entry sums and
+ * counts of negative/NaN entries that qsim_sample_exact computes first: pass
+ * `tile_stats` (qsim_tile_stats_bytes(2^nqubits) bytes) to qsim_sample_exact
+ * on these probabilities to skip that pass (null: computed there). */
+size_t qsim_tile_stats_bytes(int64_t n);
+int qsim_probabilities_tiles(const void* state, int nqubits, int dtype, void* probs, void* tile_stats,
+                             size_t tile_stats_bytes, void* stream);
 
 /* out[s, b] = (samples[s] >> (nbits-1-b)) & 1, as Backend.samples_to_binary
  * (abstract.py:2783-2786). out is int64 [nshots, nbits]. */

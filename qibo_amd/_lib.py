@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 # QIBO_AMD_LIB names another build of the library (an A/B comparison of two
 # builds on one device); by default the one next to this file
 LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -47,6 +47,10 @@ SIGNATURES = {
     "qsim_np_sum": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "qsim_divide_by": (_i, [_vp, _i, _i64, _vp, _vp, _vp]),
     "qsim_search_sorted": (_i, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    "qsim_sample_workspace": (_sz, [_i64]),
+    "qsim_sample_exact": (_i, [_vp, _i, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _sz, _vp]),
+    "qsim_tile_stats_bytes": (_sz, [_i64]),
+    "qsim_probabilities_tiles": (_i, [_vp, _i, _i, _vp, _vp, _sz, _vp]),
     "qsim_samples_to_binary": (_i, [_vp, _i64, _i, _vp, _vp]),
     "qsim_histogram": (_i, [_vp, _i64, _vp, _i64, _vp]),
     "qsim_expval_workspace": (_sz, [_i, _i]),

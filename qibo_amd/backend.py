@@ -1311,6 +1311,15 @@ class MI355XBackend(GatePlanner):
                                                 out.data_ptr(), self._stream(state))
             _lib.check(rc, self.lib)
             return out
+        if qubits == list(range(nqubits)) and nqubits <= 35:
+            # every qubit in order: the pass also writes the tile statistics
+            # that sample_shots would otherwise compute from p first
+            tiles = torch.empty(self.lib.qsim_tile_stats_bytes(out.numel()), dtype=torch.uint8, device=state.device)
+            rc = self.lib.qsim_probabilities_tiles(state.data_ptr(), nqubits, code, out.data_ptr(), tiles.data_ptr(),
+                                                   tiles.numel(), self._stream(state))
+            _lib.check(rc, self.lib)
+            out._qsim_tiles = (tiles, out._version)
+            return out
         wsb = self.lib.qsim_probabilities_workspace(nqubits, m)
         ws = self._workspace(wsb, "probs", state.device) if wsb else None
         rc = self.lib.qsim_probabilities(state.data_ptr(), nqubits, code, _lib.int_array(qubits), m,
@@ -1333,9 +1342,6 @@ class MI355XBackend(GatePlanner):
     def _validate_probabilities(self, p: torch.Tensor):
         """The checks np.random.choice applies to ``p`` before drawing (one
         pass over ``p``: sum, negative entries, NaNs)."""
-        atol = math.sqrt(np.finfo(np.float64).eps)
-        if p.dtype == torch.float32:
-            atol = max(atol, math.sqrt(np.finfo(np.float32).eps))
         p = p.contiguous()
         n = p.numel()
         out = torch.empty(3, dtype=torch.float64, device=p.device)
@@ -1344,13 +1350,7 @@ class MI355XBackend(GatePlanner):
         pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
         _lib.check(self.lib.qsim_prob_stats(p.data_ptr(), pcode, n, out.data_ptr(), ws.data_ptr(), wsb,
                                             self._stream(p)), self.lib)
-        total, nneg, nnan = out.tolist()
-        if nnan or math.isnan(total):
-            raise ValueError("probabilities contain NaN")
-        if nneg:
-            raise ValueError("probabilities are not non-negative")
-        if abs(total - 1.0) > atol:
-            raise ValueError("probabilities do not sum to 1")
+        self._check_prob_stats(*out.tolist(), p.dtype)
 
     def exact_cdf(self, p: torch.Tensor) -> torch.Tensor:
         """Sequential-order cumulative sum of ``p`` as float64 (numpy's cumsum)."""
@@ -1377,20 +1377,49 @@ class MI355XBackend(GatePlanner):
         _keep_alive(u, cdf)
         return out
 
+    @staticmethod
+    def _check_prob_stats(total, nneg, nnan, dtype):
+        """np.random.choice's checks on p (sum within sqrt(eps) of 1, no
+        negative entry, no NaN), from the device's statistics of p."""
+        atol = math.sqrt(np.finfo(np.float64).eps)
+        if dtype == torch.float32:
+            atol = max(atol, math.sqrt(np.finfo(np.float32).eps))
+        if nnan or math.isnan(total):
+            raise ValueError("probabilities contain NaN")
+        if nneg:
+            raise ValueError("probabilities are not non-negative")
+        if abs(total - 1.0) > atol:
+            raise ValueError("probabilities do not sum to 1")
+
     def sample_shots(self, probabilities, nshots: int):
+        """``nshots`` indices drawn as np.random.choice(n, nshots, p=p) draws
+        them from numpy's global generator (abstract.py:2774-2781), in one
+        library call (``qsim_sample_exact``: the exact sequential cumsum read
+        only where the bisections compare it). If p fails np.random.choice's
+        checks the generator is put back, so a failed call consumes nothing,
+        as the checks run before numpy's draws."""
         p = self._as_prob_tensor(probabilities)
-        # the uniforms are drawn while the device still computes p; if p fails
-        # the checks the generator is put back, so a failed call consumes
-        # nothing, as np.random.choice's checks run before its draws
+        n, k = p.numel(), int(nshots)
         rng_state = np.random.get_state()
-        u = np.random.random_sample(int(nshots))
+        u = np.random.random_sample(k)
+        out = torch.empty(k, dtype=torch.int64, device=p.device)
+        stats = torch.empty(3, dtype=torch.float64, device=p.device)
+        ud = torch.from_numpy(np.ascontiguousarray(u, dtype=np.float64)).pin_memory().to(p.device, non_blocking=True)
+        wsb = self.lib.qsim_sample_workspace(n)
+        ws = self._workspace(wsb, "sample", p.device)
+        pcode = _lib.F64 if p.dtype == torch.float64 else _lib.F32
+        tiles = getattr(p, "_qsim_tiles", None)
+        tiles = tiles[0].data_ptr() if tiles is not None and tiles[1] == p._version else None
+        rc = self.lib.qsim_sample_exact(p.data_ptr(), pcode, n, tiles, ud.data_ptr(), k, out.data_ptr(),
+                                        stats.data_ptr(), ws.data_ptr(), wsb, self._stream(p))
+        _lib.check(rc, self.lib)
+        _keep_alive(ud, p)
         try:
-            self._validate_probabilities(p)
+            self._check_prob_stats(*stats.tolist(), p.dtype)
         except ValueError:
             np.random.set_state(rng_state)
             raise
-        cdf = self.exact_cdf(p)
-        return self.search_cdf(cdf, u)
+        return out
 
     def np_sum(self, p: torch.Tensor) -> torch.Tensor:
         """numpy's ``np.sum`` of a power-of-two-long probability vector, bit for bit

@@ -465,6 +465,40 @@ __device__ __forceinline__ Map elem_map(double p, int Eb) {
   return {int64_t(k + (k & 1)), int64_t(k + ((k + 1) & 1))};
 }
 
+// The same element map and composition with the integers held in doubles
+// (exact below 2^53), for the bulk of k_tile_map: an FP64 pipe operation
+// each instead of 64-bit integer sequences. x = |p| / u = |p| 2^(1075 - Eb)
+// is exact whenever x >= 2^-1022, and a smaller x rounds to 0 either way;
+// c0 = x rounded to nearest, ties to even (m even: m + c even), c1 the same
+// with ties to odd (m odd). A p above the binade gives x >= 2^53, which the
+// caller's final check (< 2^53) rejects like the integer form's kInvalid;
+// sums of values >= 2^53 stay >= 2^53, and NaN fails the check.
+struct DMap {
+  double c0, c1;
+};
+
+__device__ __forceinline__ DMap elem_map_fp(double p, int Eb) {
+  const double x = ldexp(fabs(p), 1075 - Eb);
+  const double c0 = __builtin_rint(x);
+  const double fl = __builtin_floor(x);
+  const bool tie = (x - fl) == 0.5;
+  return {c0, tie ? (fl + fl + 1.0) - c0 : c0};
+}
+
+__device__ __forceinline__ bool odd_fp(double c) {
+  const double h = c * 0.5;
+  return h != __builtin_floor(h);
+}
+
+// g after f
+__device__ __forceinline__ DMap compose_fp(DMap f, DMap g) {
+  return {f.c0 + (odd_fp(f.c0) ? g.c1 : g.c0), f.c1 + (odd_fp(1.0 + f.c1) ? g.c1 : g.c0)};
+}
+
+__device__ __forceinline__ Map to_int_map(DMap f) {
+  return {f.c0 < 0x1p53 ? int64_t(f.c0) : kInvalid, f.c1 < 0x1p53 ? int64_t(f.c1) : kInvalid};
+}
+
 template <typename P>
 __device__ __forceinline__ double load_p(const P* p, int64_t i) {
   return double(p[i]);
@@ -477,10 +511,26 @@ __device__ __forceinline__ Map shfl_up_map(Map v, int delta) {
   return r;
 }
 
+// Lane `lane`'s value for the whole wavefront; `lane` must be the same on
+// every lane (a loop counter or a constant): one v_readlane per 32 bits,
+// no LDS round trip as in a general shuffle.
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+__device__ __forceinline__ int64_t rl(int64_t v, int lane) {
+  const uint64_t u = uint64_t(v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), lane));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), lane));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ double rl(double v, int lane) {
+  return __longlong_as_double((long long)rl(int64_t(__double_as_longlong(v)), lane));
+}
+
 __device__ __forceinline__ Map shfl_map(Map v, int lane) {
   Map r;
-  r.c0 = __shfl(v.c0, lane, 64);
-  r.c1 = __shfl(v.c1, lane, 64);
+  r.c0 = rl(v.c0, lane);
+  r.c1 = rl(v.c1, lane);
   return r;
 }
 
@@ -585,9 +635,10 @@ __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_
     return;
   }
   stage_tile(p, n, t * kTile, tile);
-  Map f = {0, 0};
+  DMap fd = {0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) f = compose(f, elem_map(tile[threadIdx.x * kPer + k], Eb));
+  for (int k = 0; k < kPer; ++k) fd = compose_fp(fd, elem_map_fp(tile[threadIdx.x * kPer + k], Eb));
+  const Map f = to_int_map(fd);
   const Map excl = block_scan_excl(f, wave_tot);
   xmap[t * kT + threadIdx.x] = excl;  // reused by k_tile_apply (same tile, same grid Eb)
   if (threadIdx.x == kT - 1) {
@@ -604,7 +655,7 @@ __global__ __launch_bounds__(64) void k_group_map(const TileInfo* __restrict__ i
   const int lane = threadIdx.x;
   const int64_t t = int64_t(blockIdx.x) * 64 + lane;
   TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
-  const int Eb0 = __shfl(ti.Eb, 0, 64);
+  const int Eb0 = rl(ti.Eb, 0);
   if (t >= ntiles) ti.Eb = Eb0;
   const bool uniform = __all(ti.flag == 1 && ti.Eb == Eb0);
   const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
@@ -615,93 +666,135 @@ __global__ __launch_bounds__(64) void k_group_map(const TileInfo* __restrict__ i
 }
 
 // Exact in-order sum of elements [lo, hi) (at most one tile) from the exact
-// start s, by the whole wavefront: lane c owns the 64-element chunk c. A
-// chunk whose partial sums stay in one binade advances the chain by its map
-// (and is finished by its lane from its exact start, in order); a chunk where
-// the sum crosses a power of two is added element by element. Writes
-// cdf[lo, hi) and returns the end value: bitwise the sequential chain.
+// start s, by the whole 256-thread block, in rounds. The tile is staged in
+// LDS; thread j owns the 16 consecutive elements from 16 j. A round takes the
+// binade of the current exact sum, composes every remaining element's map on
+// its grid (the floating-point form), scans the thread maps across the block
+// and lets each thread walk its elements from its exact start: every partial
+// sum that stays in the binade is final, and the first element whose sum
+// leaves it (m + c >= 2^53: the sum reaches the next power of two) is found
+// with an LDS minimum. That element is added with one IEEE addition and the
+// next round starts after it on its new binade. Without a crossing the tile
+// ends in one round; a round per binade crossed otherwise. The results are
+// stored coalesced to cdf[lo, hi); returns the end value: bitwise the
+// sequential chain.
+struct ChainLds {
+  double lt[kTile];
+  Map wave_tot[kT / 64];
+  double last;  // exact sum before the crossing element
+  int cross;
+};
+
 template <typename P>
-__device__ double chain_tile_chunks(const P* __restrict__ p, int64_t lo, int64_t hi, double s,
-                                    double* __restrict__ cdf) {
-  constexpr int C = 64;
-  const int lane = threadIdx.x & 63;
-  const int64_t c_lo = lo + int64_t(lane) * C;
-  const int64_t c_hi = c_lo + C < hi ? c_lo + C : hi;  // empty when c_lo >= hi
-  double asum = 0.0;
-  for (int64_t i = c_lo; i < c_hi; ++i) asum += load_p(p, i);
-  double incl = asum;  // approximate inclusive prefix of the chunk sums
+__device__ double chain_tile_rounds(const P* __restrict__ p, int64_t lo, int64_t hi, double s,
+                                    double* __restrict__ cdf, ChainLds& sh) {
+  const int tid = threadIdx.x;
+  const int len = int(hi - lo);  // 1 .. 4096
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += o;
+  for (int k = 0; k < kPer; ++k) {
+    const int e = k * kT + tid;
+    sh.lt[e] = e < len ? load_p(p, lo + e) : 0.0;
   }
-  const double A = s + (incl - asum), Z = s + incl;
-  const int Eb = binade_of(A);
-  const double margin = 1.0 + 0x1p-20;
-  const double blo = ldexp(1.0, Eb - 1023), bhi = ldexp(1.0, Eb - 1022);
-  bool safe = c_lo < c_hi && (Eb == 1 || A >= blo * margin) && (Z * margin < bhi);
-  Map f = {0, 0};
-  if (safe) {
-    for (int64_t i = c_lo; i < c_hi; ++i) f = compose(f, elem_map(load_p(p, i), Eb));
-    safe = f.c0 < kTwo53 && f.c1 < kTwo53;
-  }
-  double start = 0.0;
-  bool resolved = false;
-  for (int c = 0; c < 64; ++c) {
-    const int64_t cl = lo + int64_t(c) * C;
-    if (cl >= hi) break;
-    const int64_t ch = cl + C < hi ? cl + C : hi;
-    const bool sf = __shfl(safe ? 1 : 0, c, 64) != 0;
-    const int cEb = __shfl(Eb, c, 64);
-    const int64_t fc0 = __shfl(f.c0, c, 64), fc1 = __shfl(f.c1, c, 64);
-    bool ok = sf && in_binade(s, cEb);
-    int64_t m2 = 0;
-    if (ok) {
-      const int64_t m = m_of(s, cEb);
-      m2 = m + ((m & 1) ? fc1 : fc0);
-      ok = m2 < kTwo53;
+  if (tid == 0) sh.cross = len;
+  __syncthreads();
+  const int e0 = tid * kPer;
+  double vals[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) vals[k] = 0.0;
+  int pos = 0;  // first element not summed yet (the same on every thread)
+  while (pos < len) {
+    const int Eb = binade_of(s);
+    const int64_t m0 = m_of(s, Eb);
+    DMap fd = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int e = e0 + k;
+      if (e >= pos && e < len) fd = compose_fp(fd, elem_map_fp(sh.lt[e], Eb));
     }
-    if (ok) {
-      if (lane == c) {
-        start = s;
-        resolved = true;
+    const Map excl = block_scan_excl(to_int_map(fd), sh.wave_tot);
+    // the thread's exact start; >= 2^53 when the sum left the binade in an
+    // earlier thread (that thread reports the crossing)
+    int64_t m = m0 + ((m0 & 1) ? excl.c1 : excl.c0);
+    int cross = len;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int e = e0 + k;
+      if (e >= pos && e < len && cross == len && m < kTwo53) {
+        const DMap c = elem_map_fp(sh.lt[e], Eb);
+        const double cc = (m & 1) ? c.c1 : c.c0;
+        if (cc < 0x1p53 && m + int64_t(cc) < kTwo53) {
+          m += int64_t(cc);
+          vals[k] = from_m(m, Eb);
+        } else {
+          cross = e;
+        }
       }
-      s = from_m(m2, cEb);
+    }
+    if (cross < len) atomicMin(&sh.cross, cross);
+    __syncthreads();
+    const int cr = sh.cross;
+    if (cr == len) {
+      // no crossing: the last thread with elements holds the end value
+      if (tid == (len - 1) / kPer) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (k == (len - 1) % kPer) sh.last = vals[k];
+      }
+      __syncthreads();
+      if (pos < len) s = sh.last;
+      pos = len;
     } else {
-      // element by element, in order: the chunk's values are loaded once
-      // (one per lane, coalesced) and passed along by shuffles, so the chain
-      // waits on no memory access; lane j keeps the sum after element j
-      const int64_t len = ch - cl;
-      const double pv = lane < len ? load_p(p, cl + lane) : 0.0;
-      double mine = 0.0;
-      for (int j = 0; j < len; ++j) {
-        s = s + __shfl(pv, j, 64);
-        if (lane == j) mine = s;
+      // elements [pos, cr) are final; element cr is added in IEEE arithmetic
+      if (cr > pos) {
+        if (tid == (cr - 1) / kPer) {
+#pragma unroll
+          for (int k = 0; k < kPer; ++k)
+            if (k == (cr - 1) % kPer) sh.last = vals[k];
+        }
+        __syncthreads();
+        s = sh.last;
       }
-      if (lane < len) cdf[cl + lane] = mine;
+      s = s + sh.lt[cr];
+      if (tid == cr / kPer) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (k == cr % kPer) vals[k] = s;
+      }
+      pos = cr + 1;
     }
+    __syncthreads();
+    if (tid == 0) sh.cross = len;
+    __syncthreads();
   }
-  if (resolved) {
-    double x = start;
-    for (int64_t i = c_lo; i < c_hi; ++i) {
-      x = x + load_p(p, i);
-      cdf[i] = x;
-    }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (e0 + k < len) sh.lt[e0 + k] = vals[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = k * kT + tid;
+    if (e < len) cdf[lo + e] = sh.lt[e];
   }
+  __syncthreads();
   return s;
 }
 
-// Tile-level chain of one group (64 tiles) from the exact start s; every lane
-// computes the same scalar chain. Writes tile starts, tile flags (0 = finish
-// by maps, 1 = summed here) and the cdf of tiles summed here. Returns s.
+// Tile-level chain of one group (64 tiles) from the exact start s; every
+// lane of every wave computes the same scalar chain from the same data (the
+// waves only split the work of chain_tile_rounds), and wave 0 writes. Writes
+// tile starts, tile flags (0 = finish by maps, 1 = summed here) and the cdf
+// of tiles summed here. Returns s.
 template <typename P>
 __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t ntiles, int64_t g,
                                     TileInfo* __restrict__ info, double* __restrict__ tstart,
-                                    double* __restrict__ cdf, double s) {
-  const int lane = threadIdx.x;
+                                    double* __restrict__ cdf, double s, ChainLds& sh) {
+  const int lane = threadIdx.x & 63;
+  const bool w0 = threadIdx.x < 64;
   const int64_t t = g * 64 + lane;
   TileInfo ti = t < ntiles ? info[t] : TileInfo{0, 0, 0, 1};
-  const int Eb0 = __shfl(ti.Eb, 0, 64);
+  // every wave holds the group's tile records before wave 0 rewrites a flag
+  __syncthreads();
+  const int Eb0 = rl(ti.Eb, 0);
   if (t >= ntiles) ti.Eb = Eb0;
   if (__all(ti.flag == 1 && ti.Eb == Eb0) && in_binade(s, Eb0)) {
     const Map incl = wave_scan_incl(Map{ti.c0, ti.c1});
@@ -711,7 +804,7 @@ __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t 
     if (mend < kTwo53) {
       Map excl = shfl_up_map(incl, 1);
       if (lane == 0) excl = {0, 0};
-      if (t < ntiles) {
+      if (w0 && t < ntiles) {
         tstart[t] = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), Eb0);
         info[t].flag = 0;
       }
@@ -723,11 +816,11 @@ __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t 
     // tile u's info is held by lane u - 64 g (read once above)
     const int src = int(u - g * 64);
     TileInfo tu;
-    tu.c0 = __shfl(ti.c0, src, 64);
-    tu.c1 = __shfl(ti.c1, src, 64);
-    tu.Eb = __shfl(ti.Eb, src, 64);
-    tu.flag = __shfl(ti.flag, src, 64);
-    if (lane == 0) tstart[u] = s;
+    tu.c0 = rl(ti.c0, src);
+    tu.c1 = rl(ti.c1, src);
+    tu.Eb = rl(ti.Eb, src);
+    tu.flag = rl(ti.flag, src);
+    if (threadIdx.x == 0) tstart[u] = s;
     bool ok = tu.flag == 1 && in_binade(s, tu.Eb);
     int64_t m2 = 0;
     if (ok) {
@@ -737,31 +830,41 @@ __device__ double chain_group_tiles(const P* __restrict__ p, int64_t n, int64_t 
     }
     if (ok) {
       s = from_m(m2, tu.Eb);
-      if (lane == 0) info[u].flag = 0;
+      if (threadIdx.x == 0) info[u].flag = 0;
     } else {
       const int64_t lo = u * kTile;
       const int64_t hi = lo + kTile < n ? lo + kTile : n;
-      s = chain_tile_chunks(p, lo, hi, s, cdf);
-      if (lane == 0) info[u].flag = 1;
+      s = chain_tile_rounds(p, lo, hi, s, cdf, sh);
+      if (threadIdx.x == 0) info[u].flag = 1;
     }
   }
   return s;
 }
 
-// (D2) exact chain over groups: one wavefront, 64 groups (262144 elements)
-// per step when they are all resolvable by their maps; group flags become
-// 0 (start known, tiles finished by k_tile_starts) or 1 (chained tile by tile).
+// (D2) exact chain over groups: 64 groups (2^24 elements) per step when they
+// are all resolvable by their maps; group flags become 0 (start known, tiles
+// finished by k_tile_starts) or 1 (chained tile by tile). One block of four
+// waves: each wave runs the same scalar chain (wave 0 writes), and all four
+// sum the tiles that cross a power of two (chain_tile_rounds).
 template <typename P>
-__global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n, int64_t ntiles, int64_t ngroups,
+__global__ __launch_bounds__(kT) void k_carry(const P* __restrict__ p, int64_t n, int64_t ntiles, int64_t ngroups,
                                               TileInfo* __restrict__ info, TileInfo* __restrict__ ginfo,
                                               double* __restrict__ gstart, double* __restrict__ tstart,
-                                              double* __restrict__ cdf) {
-  const int lane = threadIdx.x;
+                                              double* __restrict__ cdf, const int* __restrict__ valid) {
+  __shared__ ChainLds sh;
+  const int lane = threadIdx.x & 63;
+  const bool w0 = threadIdx.x < 64;
+  // probabilities with a NaN, a negative entry or an infinite sum are not
+  // chained (np.random.choice rejects them; the chain would cross binades on
+  // every element)
+  if (valid != nullptr && *valid == 0) return;
   double s = 0.0;
   for (int64_t k = 0; k < ngroups; k += 64) {
     const int64_t gg = k + lane;
     TileInfo gi = gg < ngroups ? ginfo[gg] : TileInfo{0, 0, 0, 1};
-    const int Eb0 = __shfl(gi.Eb, 0, 64);
+    // every wave holds the step's group records before wave 0 rewrites a flag
+    __syncthreads();
+    const int Eb0 = rl(gi.Eb, 0);
     if (gg >= ngroups) gi.Eb = Eb0;
     if (__all(gi.flag == 1 && gi.Eb == Eb0) && in_binade(s, Eb0)) {
       const Map incl = wave_scan_incl(Map{gi.c0, gi.c1});
@@ -771,7 +874,7 @@ __global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n
       if (mend < kTwo53) {
         Map excl = shfl_up_map(incl, 1);
         if (lane == 0) excl = {0, 0};
-        if (gg < ngroups) {
+        if (w0 && gg < ngroups) {
           gstart[gg] = from_m(m0 + ((m0 & 1) ? excl.c1 : excl.c0), Eb0);
           ginfo[gg].flag = 0;
         }
@@ -784,10 +887,10 @@ __global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n
       // group g's info is held by lane g - k (read once above)
       const int src = int(g - k);
       TileInfo gu;
-      gu.c0 = __shfl(gi.c0, src, 64);
-      gu.c1 = __shfl(gi.c1, src, 64);
-      gu.Eb = __shfl(gi.Eb, src, 64);
-      gu.flag = __shfl(gi.flag, src, 64);
+      gu.c0 = rl(gi.c0, src);
+      gu.c1 = rl(gi.c1, src);
+      gu.Eb = rl(gi.Eb, src);
+      gu.flag = rl(gi.flag, src);
       bool ok = gu.flag == 1 && in_binade(s, gu.Eb);
       int64_t m2 = 0;
       if (ok) {
@@ -796,14 +899,14 @@ __global__ __launch_bounds__(64) void k_carry(const P* __restrict__ p, int64_t n
         ok = m2 < kTwo53;
       }
       if (ok) {
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
           gstart[g] = s;
           ginfo[g].flag = 0;
         }
         s = from_m(m2, gu.Eb);
       } else {
-        if (lane == 0) ginfo[g].flag = 1;
-        s = chain_group_tiles(p, n, ntiles, g, info, tstart, cdf, s);
+        if (threadIdx.x == 0) ginfo[g].flag = 1;
+        s = chain_group_tiles(p, n, ntiles, g, info, tstart, cdf, s, sh);
       }
     }
   }
@@ -910,12 +1013,375 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(64), 0, s, p, n, T, G, c.info, c.ginfo, c.gstart, c.tstart, cdf);
+  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(kT), 0, s, p, n, T, G, c.info, c.ginfo, c.gstart, c.tstart, cdf,
+                     static_cast<const int*>(nullptr));
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_tile_starts, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo, c.gstart, c.tstart);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_tile_apply<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.info, c.tstart, c.xmap, cdf);
   QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sampling without the full cumulative sum
+// ---------------------------------------------------------------------------
+// np.random.choice(n, size, p=p) returns, for a uniform u, the number of i
+// with fl(cdf[i] / cdf[n-1]) <= u (searchsorted side="right" on the
+// normalised cumsum). A draw compares only a few cdf values, and the chain
+// already fixes all of them without writing the cdf: k_carry yields every
+// tile's exact start, a tile it resolved by its map has every 16-element
+// chunk's exact start in its threads' exclusive maps (k_tile_map, on the
+// tile's grid), and a tile it summed element by element has its cdf written.
+// So each draw bisects the tile ends, then the chunk ends of its tile, then
+// adds at most 16 elements from the chunk's exact start; the full cdf (and
+// k_tile_apply's pass over every element) is never needed. The statistics
+// np.random.choice checks (sum, negative entries, NaNs) come from the same
+// pass as the tile sums; invalid probabilities are not chained.
+
+// (S0) per tile: the sum of k_tile_sum and the counts of negative and NaN
+// entries
+template <typename P>
+__global__ __launch_bounds__(kT) void k_tile_stats(const P* __restrict__ p, int64_t n, double* __restrict__ tsum,
+                                                   int2* __restrict__ tbad) {
+  __shared__ double lds[kT];
+  const int64_t base = int64_t(blockIdx.x) * kTile;
+  double acc = 0.0;
+  int neg = 0, nan = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    if (i < n) {
+      const double x = load_p(p, i);
+      acc += x;
+      neg += x < 0.0 ? 1 : 0;
+      nan += x != x ? 1 : 0;
+    }
+  }
+  const double tot = block_sum_fixed(acc, lds);
+  const double nneg = block_sum_fixed(double(neg), lds);
+  const double nnan = block_sum_fixed(double(nan), lds);
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = tot;
+    tbad[blockIdx.x] = make_int2(int(nneg), int(nnan));
+  }
+}
+
+// Probabilities of all qubits in order (k_probs_all) and, in the same pass,
+// the tile statistics of k_tile_stats over them (same order: bitwise the
+// same tile sums).
+template <typename R>
+__global__ __launch_bounds__(kT) void k_probs_tiles(const cplx<R>* __restrict__ psi, R* __restrict__ out, int64_t n,
+                                                    double* __restrict__ tsum, int2* __restrict__ tbad) {
+  __shared__ double lds[kT];
+  const int64_t base = int64_t(blockIdx.x) * kTile;
+  double acc = 0.0;
+  int neg = 0, nan = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    if (i < n) {
+      const R v = np_abs2(psi[i]);
+      out[i] = v;
+      const double x = double(v);
+      acc += x;
+      neg += x < 0.0 ? 1 : 0;
+      nan += x != x ? 1 : 0;
+    }
+  }
+  const double tot = block_sum_fixed(acc, lds);
+  const double nneg = block_sum_fixed(double(neg), lds);
+  const double nnan = block_sum_fixed(double(nan), lds);
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = tot;
+    tbad[blockIdx.x] = make_int2(int(nneg), int(nnan));
+  }
+}
+
+static size_t tile_stats_bytes(int64_t n) {
+  const size_t T = size_t((n + kTile - 1) / kTile);
+  return align256(T * sizeof(double)) + align256(T * sizeof(int2));
+}
+
+// Approximate exclusive prefix of the tile sums in three steps: 16 tiles
+// per thread, 4096 per block (S1), the block partials in one block (S2),
+// the tile starts per block (S3). Fixed order throughout (deterministic).
+constexpr int kScanPer = 16;
+constexpr int64_t kScanBlock = int64_t(kT) * kScanPer;
+
+// Inclusive scan of one double per thread across the block (Hillis-Steele).
+__device__ double block_scan_incl(double v, double* lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kT; off <<= 1) {
+    const double o = threadIdx.x >= off ? lds[threadIdx.x - off] : 0.0;
+    __syncthreads();
+    lds[threadIdx.x] += o;
+    __syncthreads();
+  }
+  const double r = lds[threadIdx.x];
+  __syncthreads();
+  return r;
+}
+
+// (S1)
+__global__ __launch_bounds__(kT) void k_scan_partials(const double* __restrict__ tsum, const int2* __restrict__ tbad,
+                                                      int64_t ntiles, double* __restrict__ psum,
+                                                      double2* __restrict__ pbad) {
+  __shared__ double lds[kT];
+  const int64_t t0 = int64_t(blockIdx.x) * kScanBlock + int64_t(threadIdx.x) * kScanPer;
+  double acc = 0.0, neg = 0.0, nan = 0.0;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t t = t0 + k;
+    if (t < ntiles) {
+      acc += tsum[t];
+      const int2 b = tbad[t];
+      neg += double(b.x);
+      nan += double(b.y);
+    }
+  }
+  acc = block_sum_fixed(acc, lds);
+  neg = block_sum_fixed(neg, lds);
+  nan = block_sum_fixed(nan, lds);
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = acc;
+    pbad[blockIdx.x] = make_double2(neg, nan);
+  }
+}
+
+// (S2) one block; nparts <= 2048 (n <= 2^35)
+__global__ __launch_bounds__(kT) void k_scan_top(const double* __restrict__ psum, const double2* __restrict__ pbad,
+                                                 int64_t nparts, double* __restrict__ pstart,
+                                                 double* __restrict__ stats, int* __restrict__ valid) {
+  __shared__ double lds[kT];
+  const int64_t per = (nparts + kT - 1) / kT;
+  const int64_t lo = int64_t(threadIdx.x) * per;
+  double acc = 0.0, neg = 0.0, nan = 0.0;
+  for (int64_t b = lo; b < lo + per && b < nparts; ++b) {
+    acc += psum[b];
+    neg += pbad[b].x;
+    nan += pbad[b].y;
+  }
+  const double incl = block_scan_incl(acc, lds);
+  lds[threadIdx.x] = incl;
+  __syncthreads();
+  const double total = lds[kT - 1];
+  __syncthreads();
+  double run = incl - acc;
+  for (int64_t b = lo; b < lo + per && b < nparts; ++b) {
+    pstart[b] = run;
+    run += psum[b];
+  }
+  neg = block_sum_fixed(neg, lds);
+  nan = block_sum_fixed(nan, lds);
+  if (threadIdx.x == 0) {
+    stats[0] = total;
+    stats[1] = neg;
+    stats[2] = nan;
+    *valid = (neg == 0.0 && nan == 0.0 && __builtin_isfinite(total)) ? 1 : 0;
+  }
+}
+
+// (S3)
+__global__ __launch_bounds__(kT) void k_scan_starts(const double* __restrict__ tsum, int64_t ntiles,
+                                                    const double* __restrict__ pstart, double* __restrict__ astart) {
+  __shared__ double lds[kT];
+  const int64_t t0 = int64_t(blockIdx.x) * kScanBlock + int64_t(threadIdx.x) * kScanPer;
+  double acc = 0.0;
+  for (int k = 0; k < kScanPer; ++k)
+    if (t0 + k < ntiles) acc += tsum[t0 + k];
+  const double incl = block_scan_incl(acc, lds);
+  double run = pstart[blockIdx.x] + (incl - acc);
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t t = t0 + k;
+    if (t < ntiles) {
+      astart[t] = run;
+      run += tsum[t];
+    }
+  }
+}
+
+// Exact cdf value of the start of 16-element chunk j of a tile the chain
+// resolved by its map (the value before the chunk's first element).
+__device__ __forceinline__ double chunk_start(const Map* __restrict__ xmap, int64_t t, int64_t j, int64_t m0, int Eb) {
+  const Map x = xmap[t * kT + j];
+  return from_m(m0 + ((m0 & 1) ? x.c1 : x.c0), Eb);
+}
+
+// (F) the exact total cdf[n - 1], the normaliser of the draws
+template <typename P>
+__global__ __launch_bounds__(64) void k_final_value(const P* __restrict__ p, int64_t n, int64_t ntiles,
+                                                    const TileInfo* __restrict__ info, const double* __restrict__ tstart,
+                                                    const Map* __restrict__ xmap, const double* __restrict__ cdf,
+                                                    const int* __restrict__ valid, double* __restrict__ fin) {
+  if (threadIdx.x != 0) return;
+  if (*valid == 0) {
+    *fin = 0.0;
+    return;
+  }
+  const int64_t t = ntiles - 1, base = t * kTile;
+  const TileInfo ti = info[t];
+  if (ti.flag != 0) {
+    *fin = cdf[n - 1];
+    return;
+  }
+  const int64_t j = (n - 1 - base) / kPer;
+  double s = chunk_start(xmap, t, j, m_of(tstart[t], ti.Eb), ti.Eb);
+  for (int64_t i = base + j * kPer; i < n; ++i) s = s + load_p(p, i);
+  *fin = s;
+}
+
+// (Q) one draw per thread
+template <typename P>
+__global__ __launch_bounds__(kT) void k_sample_search(const P* __restrict__ p, int64_t n, int64_t ntiles,
+                                                      const TileInfo* __restrict__ info,
+                                                      const double* __restrict__ tstart, const Map* __restrict__ xmap,
+                                                      const double* __restrict__ cdf, const int* __restrict__ valid,
+                                                      const double* __restrict__ fin, const double* __restrict__ u,
+                                                      int64_t nshots, int64_t* __restrict__ out) {
+  const int64_t k = int64_t(block_id()) * kT + threadIdx.x;
+  if (k >= nshots) return;
+  if (*valid == 0) {
+    out[k] = 0;
+    return;
+  }
+  const double total = *fin;
+  const double v = u[k];
+  // first tile whose end (the next tile's start; the total for the last
+  // tile, whose normalised end is 1 > v) exceeds v after normalisation
+  int64_t lo = 0, hi = ntiles - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (tstart[mid + 1] / total <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const int64_t t = lo, base = t * kTile;
+  const int64_t end = base + kTile < n ? base + kTile : n;
+  const TileInfo ti = info[t];
+  if (ti.flag != 0) {
+    // summed element by element: its cdf is written; cdf[end - 1] > v
+    int64_t a = base, b = end - 1;
+    while (a < b) {
+      const int64_t mid = a + ((b - a) >> 1);
+      if (cdf[mid] / total <= v)
+        a = mid + 1;
+      else
+        b = mid;
+    }
+    out[k] = a;
+    return;
+  }
+  // first chunk whose end (the next chunk's start; the tile end for the
+  // last chunk) exceeds v, then the first element of it
+  const int64_t m0 = m_of(tstart[t], ti.Eb);
+  int64_t a = 0, b = (end - base - 1) / kPer;
+  while (a < b) {
+    const int64_t mid = a + ((b - a) >> 1);
+    if (chunk_start(xmap, t, mid + 1, m0, ti.Eb) / total <= v)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  double s = chunk_start(xmap, t, a, m0, ti.Eb);
+  int64_t i = base + a * kPer;
+  const int64_t last = (i + kPer < end ? i + kPer : end) - 1;
+  for (; i < last; ++i) {
+    s = s + load_p(p, i);
+    if (!(s / total <= v)) break;
+  }
+  out[k] = i;
+}
+
+struct SampleWs {
+  CumsumWs c;
+  int2* tbad;
+  double* psum;
+  double2* pbad;
+  double* pstart;
+  double* fin;
+  int* valid;
+  double* cdf;  // written only for the tiles the chain sums element by element
+};
+
+static size_t sample_ws_bytes(int64_t n) {
+  const size_t T = size_t((n + kTile - 1) / kTile);
+  const size_t NP = size_t((int64_t(T) + kScanBlock - 1) / kScanBlock);
+  return cumsum_ws_bytes(n) + align256(T * sizeof(int2)) + align256(NP * sizeof(double)) +
+         align256(NP * sizeof(double2)) + align256(NP * sizeof(double)) + 2 * align256(sizeof(double)) +
+         align256(size_t(n) * sizeof(double));
+}
+
+template <typename P>
+static int sample_impl(const void* probs, int64_t n, const void* tiles, const double* u, int64_t nshots, int64_t* out,
+                       double* stats, void* ws, hipStream_t s) {
+  const P* p = reinterpret_cast<const P*>(probs);
+  const int64_t T = (n + kTile - 1) / kTile;
+  const int64_t G = (T + 63) / 64;
+  const int64_t NP = (T + kScanBlock - 1) / kScanBlock;
+  char* w = reinterpret_cast<char*>(ws);
+  SampleWs a;
+  a.c.tsum = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  a.c.astart = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  a.c.tstart = reinterpret_cast<double*>(w);
+  w += align256(T * sizeof(double));
+  a.c.info = reinterpret_cast<TileInfo*>(w);
+  w += align256(T * sizeof(TileInfo));
+  a.c.ginfo = reinterpret_cast<TileInfo*>(w);
+  w += align256(G * sizeof(TileInfo));
+  a.c.gstart = reinterpret_cast<double*>(w);
+  w += align256(G * sizeof(double));
+  a.c.xmap = reinterpret_cast<Map*>(w);
+  w += align256(T * kT * sizeof(Map));
+  a.tbad = reinterpret_cast<int2*>(w);
+  w += align256(T * sizeof(int2));
+  a.psum = reinterpret_cast<double*>(w);
+  w += align256(NP * sizeof(double));
+  a.pbad = reinterpret_cast<double2*>(w);
+  w += align256(NP * sizeof(double2));
+  a.pstart = reinterpret_cast<double*>(w);
+  w += align256(NP * sizeof(double));
+  a.fin = reinterpret_cast<double*>(w);
+  w += align256(sizeof(double));
+  a.valid = reinterpret_cast<int*>(w);
+  w += align256(sizeof(double));
+  a.cdf = reinterpret_cast<double*>(w);
+  if (tiles != nullptr) {
+    // computed with the probabilities (qsim_probabilities_tiles)
+    a.c.tsum = reinterpret_cast<double*>(const_cast<void*>(tiles));
+    a.tbad = reinterpret_cast<int2*>(reinterpret_cast<char*>(a.c.tsum) + align256(T * sizeof(double)));
+  } else {
+    hipLaunchKernelGGL(k_tile_stats<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, a.c.tsum, a.tbad);
+    QSIM_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_scan_partials, dim3(unsigned(NP)), dim3(kT), 0, s, a.c.tsum, a.tbad, T, a.psum, a.pbad);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kT), 0, s, a.psum, a.pbad, NP, a.pstart, stats, a.valid);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_starts, dim3(unsigned(NP)), dim3(kT), 0, s, a.c.tsum, T, a.pstart, a.c.astart);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, a.c.astart, a.c.tsum, a.c.info,
+                     a.c.xmap);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, a.c.info, T, a.c.ginfo);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_carry<P>, dim3(1), dim3(kT), 0, s, p, n, T, G, a.c.info, a.c.ginfo, a.c.gstart, a.c.tstart,
+                     a.cdf, static_cast<const int*>(a.valid));
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_tile_starts, dim3(unsigned(G)), dim3(64), 0, s, a.c.info, T, a.c.ginfo, a.c.gstart,
+                     a.c.tstart);
+  QSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_final_value<P>, dim3(1), dim3(64), 0, s, p, n, T, a.c.info, a.c.tstart, a.c.xmap, a.cdf,
+                     a.valid, a.fin);
+  QSIM_CHECK_LAUNCH();
+  if (nshots > 0) {
+    hipLaunchKernelGGL(k_sample_search<P>, launch_grid(ceil_div_u(uint64_t(nshots), kT)), dim3(kT), 0, s, p, n, T,
+                       a.c.info, a.c.tstart, a.c.xmap, a.cdf, a.valid, a.fin, u, nshots, out);
+    QSIM_CHECK_LAUNCH();
+  }
   return QSIM_OK;
 }
 
@@ -1161,6 +1627,53 @@ int qsim_cumsum_exact(const void* probs, int prob_dtype, int64_t n, double* cdf,
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (prob_dtype == QSIM_F64) return cumsum_impl<double>(probs, n, cdf, workspace, s);
   return cumsum_impl<float>(probs, n, cdf, workspace, s);
+}
+
+size_t qsim_sample_workspace(int64_t n) {
+  return (n < 1 || n > (int64_t(1) << 35)) ? 0 : sample_ws_bytes(n);
+}
+
+size_t qsim_tile_stats_bytes(int64_t n) { return (n < 1 || n > (int64_t(1) << 35)) ? 0 : tile_stats_bytes(n); }
+
+int qsim_probabilities_tiles(const void* state, int nqubits, int dtype, void* probs, void* tile_stats,
+                             size_t tile_stats_bytes_, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr && probs != nullptr && tile_stats != nullptr,
+                 "qsim_probabilities_tiles: null pointer");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_probabilities_tiles: bad dtype");
+  QSIM_CHECK_ARG(nqubits >= 0 && nqubits <= 35 && nqubits <= QSIM_MAX_QUBITS,
+                 "qsim_probabilities_tiles: bad number of qubits");
+  const int64_t n = int64_t(1) << nqubits;
+  QSIM_CHECK_ARG(tile_stats_bytes_ >= tile_stats_bytes(n), "qsim_probabilities_tiles: tile_stats too small");
+  const int64_t T = (n + kTile - 1) / kTile;
+  double* tsum = reinterpret_cast<double*>(tile_stats);
+  int2* tbad = reinterpret_cast<int2*>(reinterpret_cast<char*>(tile_stats) + align256(T * sizeof(double)));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    hipLaunchKernelGGL(k_probs_tiles<double>, dim3(unsigned(T)), dim3(kT), 0, s,
+                       reinterpret_cast<const cplx<double>*>(state), reinterpret_cast<double*>(probs), n, tsum, tbad);
+  else
+    hipLaunchKernelGGL(k_probs_tiles<float>, dim3(unsigned(T)), dim3(kT), 0, s,
+                       reinterpret_cast<const cplx<float>*>(state), reinterpret_cast<float*>(probs), n, tsum, tbad);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_sample_exact(const void* probs, int prob_dtype, int64_t n, const void* tile_stats, const double* uniforms,
+                      int64_t nshots, int64_t* out, double* stats, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(probs != nullptr && stats != nullptr, "qsim_sample_exact: null pointer");
+  QSIM_CHECK_ARG(nshots == 0 || (uniforms != nullptr && out != nullptr), "qsim_sample_exact: null pointer");
+  QSIM_CHECK_ARG(n >= 1 && n <= (int64_t(1) << 35), "qsim_sample_exact: length must be in [1, 2^35]");
+  QSIM_CHECK_ARG(nshots >= 0 && nshots <= (int64_t(1) << 40), "qsim_sample_exact: bad number of shots");
+  QSIM_CHECK_ARG(prob_dtype == QSIM_F64 || prob_dtype == QSIM_F32, "qsim_sample_exact: bad dtype");
+  QSIM_CHECK_ARG(workspace != nullptr && workspace_bytes >= sample_ws_bytes(n),
+                 "qsim_sample_exact: workspace too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (prob_dtype == QSIM_F64)
+    return sample_impl<double>(probs, n, tile_stats, uniforms, nshots, out, stats, workspace, s);
+  return sample_impl<float>(probs, n, tile_stats, uniforms, nshots, out, stats, workspace, s);
 }
 
 size_t qsim_np_sum_workspace(int64_t n) {

@@ -159,8 +159,8 @@ def _zsum_tables(n):
 class _Tail:
     """Device buffers of one request's measurement (``<sum Z_i>`` or shot
     indices) and the launches that fill them: the same library calls as
-    ``calculate_probabilities`` -> ``exact_cdf`` -> ``search_cdf`` and
-    ``expectation_pauli``, into buffers owned here (so that a captured graph
+    ``calculate_probabilities`` -> ``sample_shots`` and ``expectation_pauli``,
+    into buffers owned here (so that a captured graph
     replays into them), ending with the copy of the result to pinned host
     memory."""
 
@@ -184,9 +184,9 @@ class _Tail:
             self.probs = torch.empty(2**n, dtype=pdt, device=dev)
             self.pwsb = self.lib.qsim_probabilities_workspace(n, n)
             self.pws = torch.empty(max(self.pwsb, 256), dtype=torch.uint8, device=dev)
-            self.cdf = torch.empty(2**n, dtype=torch.float64, device=dev)
-            self.cwsb = self.lib.qsim_cumsum_workspace(2**n)
-            self.cws = torch.empty(max(self.cwsb, 256), dtype=torch.uint8, device=dev)
+            self.swsb = self.lib.qsim_sample_workspace(2**n)
+            self.sws = torch.empty(max(self.swsb, 256), dtype=torch.uint8, device=dev)
+            self.stats = torch.empty(3, dtype=torch.float64, device=dev)
             self.u_host = torch.empty(max(nshots, 1), dtype=torch.float64, pin_memory=True)
             self.u_dev = torch.empty(max(nshots, 1), dtype=torch.float64, device=dev)
             self.out = torch.empty(max(nshots, 1), dtype=torch.int64, device=dev)
@@ -214,12 +214,10 @@ class _Tail:
             rc = lib.qsim_probabilities(state.data_ptr(), n, self.code, self.qubits, n, self.probs.data_ptr(),
                                         self.pws.data_ptr(), self.pwsb, stream_ptr)
             _lib.check(rc, lib)
-            rc = lib.qsim_cumsum_exact(self.probs.data_ptr(), self.pcode, 2**n, self.cdf.data_ptr(),
-                                       self.cws.data_ptr(), self.cwsb, stream_ptr)
-            _lib.check(rc, lib)
             self.u_dev.copy_(self.u_host, non_blocking=True)
-            rc = lib.qsim_search_sorted(self.cdf.data_ptr(), 2**n, self.u_dev.data_ptr(), self.nshots,
-                                        self.out.data_ptr(), stream_ptr)
+            rc = lib.qsim_sample_exact(self.probs.data_ptr(), self.pcode, 2**n, None, self.u_dev.data_ptr(), self.nshots,
+                                       self.out.data_ptr(), self.stats.data_ptr(), self.sws.data_ptr(), self.swsb,
+                                       stream_ptr)
             _lib.check(rc, lib)
         self.host.copy_(self.out, non_blocking=True)
 
