@@ -637,9 +637,10 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     # HBM bytes the step's kernels move: each blocked pass reads and writes the
     # state once (the first, from |0...0>, only writes it); the probabilities
-    # read the state and write 8 B per amplitude; the CDF reads and writes 8 B
+    # read the state and write 8 B per amplitude; the sampling maps read the
+    # probabilities again and write 1 B per amplitude (16 B per 16 entries)
     npass = len(passes) / args.steps if passes else 0.0
-    step_bytes = (2 * npass - 1) * (2**n) * itemsize + (2**n) * (itemsize + 8) + (2**n) * 16
+    step_bytes = (2 * npass - 1) * (2**n) * itemsize + (2**n) * (itemsize + 8) + (2**n) * 9
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
     # this same workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scales read off a
     # plain copy of one state buffer); only quoted for the workload it measured
@@ -687,7 +688,7 @@ def main():
             "GBs": step_bytes / (elapsed / args.steps) / 1e9,
             "counted": "per blocked pass one read and one write of the state (the first pass "
                        "only writes), probabilities (16 B read + 8 B written per amplitude), "
-                       "CDF (8 B read + 8 B written per amplitude)",
+                       "sampling maps (8 B read + 1 B written per amplitude)",
         },
         # the backend keeps the compiled passes of a repeated identical gate
         # queue (same gates, qubits and parameter values); every timed step
