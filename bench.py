@@ -220,6 +220,7 @@ def bench_precision(seed, n_time=32, n_acc=30, nlayers=20, nblocks=24):
 
     b64 = MI355XBackend(dtype="complex64")
     b128 = MI355XBackend(dtype="complex128")
+    b64.specialize = b128.specialize = False  # interpreted passes: the forms compared are the kernels'
     res = {"nqubits_timed": n_time, "nqubits_accuracy": n_acc}
 
     # layered circuit, as configured
@@ -292,6 +293,7 @@ def bench_qft(seed, n=30):
     res["gates"] = c.ngates
     for dtype in ("complex128", "complex64"):
         backend = MI355XBackend(dtype=dtype)
+        backend.specialize = False  # one timed run: nothing is repeated
         c._final_state = None
         backend.execute_circuit(c)
         c._final_state = None  # the timed run reuses the cached state block
@@ -313,10 +315,12 @@ def bench_qft15(seed, reps=50):
     from qibo_amd.backend import MI355XBackend
 
     backend = MI355XBackend(dtype="complex128")
+    backend.specialize_after = 1  # compiled during the warm-up execution
     c = models.QFT(15)
     c._final_state = None
     backend.execute_circuit(c)
     c._final_state = None
+    backend.wait_specializations()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -352,6 +356,7 @@ def bench_layered28(seed, nshots=100000, steps=3):
     from qibo_amd.backend import MI355XBackend
 
     backend = MI355XBackend(dtype="complex128")
+    backend.specialize_after = 1  # compiled during the warm-up step
     c = models.random_layered_circuit(28, 20, seed=seed)
 
     def step():
@@ -362,6 +367,7 @@ def bench_layered28(seed, nshots=100000, steps=3):
         return res.samples()
 
     step()
+    backend.wait_specializations()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -388,6 +394,8 @@ def bench_serving(seed, nrequests=48, qps_list=(8.0, 16.0, 32.0, 64.0), ncheck=8
     from qibo_amd.serving import CircuitServer, GPUExecutor, poisson_stream
 
     backends = {"complex128": MI355XBackend(dtype="complex128"), "complex64": MI355XBackend(dtype="complex64")}
+    for b in backends.values():
+        b.specialize = False  # requests of many structures: no background compiles while serving
     ex = GPUExecutor(backends, nstreams=8)
     budget = int(0.8 * torch.cuda.mem_get_info()[0])
     sweep = []
@@ -459,6 +467,7 @@ def bench_density(seed, n=14, nlayers=6, n_cpu=9, reps=3):
     from qibo_amd.backend import MI355XBackend
 
     backend = MI355XBackend(dtype="complex128")
+    backend.specialize = False  # one timed run per size: nothing is repeated
     out = {"nqubits": n, "layers": nlayers}
     for size, key in ((n, "gpu"), (n_cpu, "gpu_small")):
         c = _noisy_layered(size, nlayers, seed)
@@ -517,6 +526,7 @@ def bench_adjoint(seed, n=20, nlayers=10, nsets=64):
     out = {"nqubits": n, "layers": nlayers, "parameters": 2 * n * nlayers, "parameter_sets": nsets}
     for dtype in ("complex128", "complex64"):
         backend = MI355XBackend(dtype=dtype)
+        backend.specialize = False  # the sweeps run batched passes (interpreted)
         c = models.hardware_efficient_ansatz(n, nlayers)
         ham = hamiltonians.TFIM(n, h=1.0, closed_boundary=False, backend=backend)
         batched_expectation_and_gradient(backend, c, ham, params[:1])  # warm-up
@@ -568,6 +578,9 @@ def main():
     from qibo_amd.backend import MI355XBackend
 
     backend = MI355XBackend(dtype="complex128", device=0)
+    # compile the passes' specialised kernels at their first launch (in the
+    # warm-up), not at the second
+    backend.specialize_after = 1
     n = args.nqubits
     circuit = models.random_layered_circuit(n, args.nlayers, seed=args.seed)
     ngates = sum(1 for g in circuit.queue if g.__class__.__name__ != "M")
@@ -605,6 +618,12 @@ def main():
         r, s = step()
         del r, s
     torch.cuda.synchronize()
+    # the kernels specialised to the step's record sequences were compiled in
+    # the background during the warm-up (hiprtc on the host); wait for them
+    t_spec = time.perf_counter()
+    nspec = backend.wait_specializations() if backend.specialize else 0
+    spec_wait_s = time.perf_counter() - t_spec
+    spec0 = backend.specialized_launches
 
     # events around every blocked-pass launch on the stream it runs on
     backend.block_stats.clear()
@@ -620,6 +639,7 @@ def main():
     kernel_ms = backend.disable_kernel_timer()
     passes = list(backend.block_stats)
     cache_hits = backend.program_cache_hits - hits0
+    spec_launches = backend.specialized_launches - spec0
 
     # the same steps with the compiled-program cache off: every step plans and
     # compiles its passes on the host again (overlapped with the GPU after the
@@ -633,6 +653,18 @@ def main():
     torch.cuda.synchronize()
     elapsed_nocache = time.perf_counter() - t0
     backend.program_cache = True
+
+    # the same steps with every pass interpreted by k_blocked
+    spec_on = backend.specialize
+    backend.specialize = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, s = step()
+        del r, s
+    torch.cuda.synchronize()
+    elapsed_interp = time.perf_counter() - t0
+    backend.specialize = spec_on
 
     ms_per_step = 1000.0 * elapsed / args.steps
     # HBM bytes the step's kernels move: each blocked pass reads and writes the
@@ -699,9 +731,21 @@ def main():
             "ms_per_step_compiling_every_step": 1000.0 * elapsed_nocache / args.steps,
             "value_compiling_every_step": ngates * args.steps / elapsed_nocache,
         },
+        # every pass runs the kernel specialised to its record sequence
+        # (qsim_blocked_spec, compiled by hiprtc during the warm-up): the same
+        # operations as k_blocked's interpreter, bitwise the same amplitudes
+        "specialized_passes": {
+            "kernels": nspec,
+            "compile_wait_after_warmup_s": spec_wait_s,
+            "specialized_launches_in_timed_steps": spec_launches,
+            "launches_in_timed_steps": len(passes),
+            "ms_per_step_interpreted": 1000.0 * elapsed_interp / args.steps,
+            "value_interpreted": ngates * args.steps / elapsed_interp,
+        },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_blocked<double> (blocked pass: state read + written once)",
+            "kernel": ("qsim_blocked_spec (blocked pass specialised to its records: state read + written once)"
+                       if spec_launches else "k_blocked<double> (blocked pass: state read + written once)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -145,6 +145,38 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
 int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                                  size_t program_bytes, int ninstr, void* stream);
 
+/* Blocked passes specialised to their record sequence (no counterpart in the
+ * reference: a faster form of qsim_apply_blocked, whose results it
+ * reproduces bitwise). The library emits one straight-line kernel per
+ * sequence of records -- opcode, argument, entry count and whether a tile
+ * condition is present -- with every other field (angles, masks, entries)
+ * read from the program at run time, compiles it with hiprtc for gfx950 and
+ * loads it into the current device (1-3 s per pass on the host, once per
+ * sequence and process). `program` is the host copy of the records.
+ * qsim_blocked_specialize compiles (or finds) the kernel and returns its
+ * handle; qsim_blocked_specialized_lookup only finds it (*handle = 0 when the
+ * sequence has no kernel yet) and gives a 64-bit hash of the sequence;
+ * qsim_apply_blocked_specialized launches it like qsim_apply_blocked (the
+ * same arguments and contract) on a device program with the same record
+ * sequence (not checked beyond its record count and length: a program of
+ * another sequence gives wrong amplitudes). The from-zero form (the input
+ * taken to be |0...0>) is a separate kernel (from_zero = 1). A sequence
+ * with a conditional mapping change is not specialised (QSIM_EINVAL).
+ * qsim_blocked_specialized_source returns the kernel's source (length + 1;
+ * written to out when cap > 0) and qsim_blocked_compile_check compiles it
+ * without a device (both for tests and inspection). */
+int qsim_blocked_specialize(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+                            uint64_t* handle);
+int qsim_blocked_specialized_lookup(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+                                    uint64_t* handle, uint64_t* key_hash);
+int qsim_apply_blocked_specialized(uint64_t handle, void* state, int nqubits, int dtype, const int* tile_bits,
+                                   const void* program_dev, size_t program_bytes, int ninstr, void* stream);
+size_t qsim_blocked_specialized_source(const void* program, size_t program_bytes, int ninstr, int dtype,
+                                       int from_zero, char* out, size_t cap);
+int qsim_blocked_compile_check(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+                               size_t* code_bytes);
+size_t qsim_blocked_specialized_count(void);
+
 /* The blocked pass for a batch of `nbatch` states stored one after the other
  * (2^nqubits amplitudes apart), state b running the program at
  * programs_dev + b * program_stride (all programs program_bytes long, same

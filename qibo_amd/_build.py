@@ -33,7 +33,27 @@ SOURCES = {
     "compile.hip": ["-ffp-contract=off"],
     "fuse.hip": ["-ffp-contract=off"],
     "compare.hip": [],
+    # run-time specialised blocked passes: the device headers' text is
+    # embedded (jit_sources.inc, written below) and compiled by hiprtc
+    "jit.hip": ["-I" + os.path.join(HERE, "_objs")],
 }
+
+# headers compiled again at run time by hiprtc (jit.hip), embedded as strings
+JIT_HEADERS = (("kJitQsimDeviceH", "qsim_device.h"), ("kJitBlockedDevH", "blocked_dev.h"))
+
+
+def _write_jit_sources():
+    parts = []
+    for name, fname in JIT_HEADERS:
+        text = open(os.path.join(CSRC, fname)).read()
+        if ")QSIMJIT" in text:
+            raise RuntimeError(f"{fname} contains the raw-string delimiter")
+        parts.append(f'static const char {name}[] = R"QSIMJIT({text})QSIMJIT";\n')
+    out = os.path.join(BUILD, "jit_sources.inc")
+    data = "".join(parts)
+    if not os.path.exists(out) or open(out).read() != data:
+        with open(out, "w") as f:
+            f.write(data)
 
 COMMON = [
     "-O3",
@@ -67,6 +87,7 @@ def _compile(src, extra):
 def build(verbose: bool = False) -> str:
     """Compile every source and link the shared library; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
+    _write_jit_sources()
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         results = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
     objs = [o for o, _ in results]
@@ -75,7 +96,9 @@ def build(verbose: bool = False) -> str:
             if err.strip():
                 print(f"[{src}] {err}", file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(_hipcc()))), "lib")
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp,
+           f"-L{rocm_lib}", "-lhiprtc", f"-Wl,-rpath,{rocm_lib}"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -63,6 +63,12 @@ SIGNATURES = {
     "qsim_apply_blocked": (_i, [_vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
     "qsim_apply_blocked_from_zero": (_i, [_vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
     "qsim_apply_blocked_batched": (_i, [_vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
+    "qsim_blocked_specialize": (_i, [_vp, _sz, _i, _i, _i, _vp]),
+    "qsim_blocked_specialized_lookup": (_i, [_vp, _sz, _i, _i, _i, _vp, _vp]),
+    "qsim_apply_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
+    "qsim_blocked_specialized_source": (_sz, [_vp, _sz, _i, _i, _i, ctypes.c_char_p, _sz]),
+    "qsim_blocked_compile_check": (_i, [_vp, _sz, _i, _i, _i, _vp]),
+    "qsim_blocked_specialized_count": (_sz, []),
     "qsim_adjoint_partials_count": (_sz, [_i, _i, _i]),
     "qsim_adjoint_blocked": (_i, [_vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _i, _vp, _sz, _vp]),
     "qsim_adjoint_reduce": (_i, [_vp, _sz, _i, _i, _i, _vp, _i, _vp, _vp]),

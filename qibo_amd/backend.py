@@ -28,6 +28,7 @@ the terms multiplied by an exact zero.
 """
 
 import collections
+import ctypes
 import math
 import os
 import threading
@@ -435,6 +436,18 @@ class MI355XBackend(GatePlanner):
         self.dm_superoperators = True
         self.program_cache = True
         self.program_cache_size = 16
+        # blocked passes specialised to their record sequence (jit.hip): a
+        # sequence launched `specialize_after` times is compiled in the
+        # background (hiprtc, 1-3 s per pass on the host) and later launches
+        # of any program with that sequence (other angles included) run the
+        # straight-line kernel, bitwise equal to the interpreted pass
+        self.specialize = os.environ.get("QIBO_AMD_SPECIALIZE", "1") != "0"
+        self.specialize_after = 2
+        self._spec_seen = {}
+        self._spec_jobs = {}
+        self._spec_pool = None
+        self._spec_lock = threading.Lock()
+        self.specialized_launches = 0
         self._programs = collections.OrderedDict()
         self._programs_lock = threading.Lock()
         self.program_cache_hits = 0
@@ -1110,8 +1123,8 @@ class MI355XBackend(GatePlanner):
                     zero[0] = False
                 self.apply_op(act[1], state, nqubits)
                 continue
-            _, tile_bits, prog, nbytes, count, ngates = act
-            self._launch_blocked(state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero)
+            _, tile_bits, prog, nbytes, count, ngates, host = act
+            self._launch_blocked(state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero, host)
         if zero[0]:
             self._fill_zero(state, nqubits)
             zero[0] = False
@@ -1266,22 +1279,75 @@ class MI355XBackend(GatePlanner):
             host = torch.frombuffer(bytearray(program) + bytes(blocking.PAD_BYTES), dtype=torch.uint8).pin_memory()
             prog = host.to(state.device, non_blocking=True)
             self._launch_blocked(state, nqubits, code, tuple(tile_bits), prog, len(program), count, len(pops),
-                                 stream, cur, zero)
+                                 stream, cur, zero, program)
             if record is not None:
-                record.append(("block", tuple(tile_bits), prog, len(program), count, len(pops)))
+                record.append(("block", tuple(tile_bits), prog, len(program), count, len(pops), bytes(program)))
         return state
 
-    def _launch_blocked(self, state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero):
+    # -- specialised blocked passes --------------------------------------------
+    def _specialized(self, host, count, code, from_zero, device):
+        """Handle of the specialised kernel of this program's record sequence,
+        or 0; counts the sighting and starts the compile in the background
+        once the sequence has been launched ``specialize_after`` times."""
+        if not self.specialize or host is None:
+            return 0
+        handle, key_hash = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        rc = self.lib.qsim_blocked_specialized_lookup(host, len(host), count, code, int(from_zero),
+                                                      ctypes.byref(handle), ctypes.byref(key_hash))
+        if rc != 0:
+            return 0  # not specialisable (a conditional mapping change)
+        if handle.value:
+            return handle.value
+        key = key_hash.value
+        with self._spec_lock:
+            seen = self._spec_seen.get(key, 0) + 1
+            self._spec_seen[key] = seen
+            if seen < self.specialize_after or key in self._spec_jobs:
+                return 0
+            if self._spec_pool is None:
+                from concurrent.futures import ThreadPoolExecutor
+
+                self._spec_pool = ThreadPoolExecutor(max_workers=4, thread_name_prefix="qibo_amd_spec")
+            self._spec_jobs[key] = self._spec_pool.submit(self._specialize_job, bytes(host), count, code,
+                                                          int(from_zero), device)
+        return 0
+
+    def _specialize_job(self, host, count, code, from_zero, device):
+        with torch.cuda.device(device):
+            handle = ctypes.c_uint64(0)
+            rc = self.lib.qsim_blocked_specialize(host, len(host), count, code, from_zero, ctypes.byref(handle))
+            if rc != 0:
+                log.warning("specialised blocked pass not built: %s", self.lib.qsim_last_error().decode())
+            return rc
+
+    def wait_specializations(self, timeout=None):
+        """Wait for the specialised kernels being compiled; returns how many
+        compiled successfully so far (all sequences, this backend)."""
+        with self._spec_lock:
+            jobs = list(self._spec_jobs.values())
+        for job in jobs:
+            job.result(timeout=timeout)
+        return sum(1 for job in jobs if job.done() and job.result() == 0)
+
+    def _launch_blocked(self, state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero,
+                        host=None):
         timer = self._timer
         if timer is not None and timer[0](_BLOCK_OP):
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record(cur)
-        launch = self.lib.qsim_apply_blocked
-        if zero is not None and zero[0]:
-            launch = self.lib.qsim_apply_blocked_from_zero
+        from_zero = zero is not None and zero[0]
+        handle = self._specialized(host, count, code, from_zero, state.device)
+        if handle:
+            rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), nqubits, code,
+                                                         _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
+                                                         count, stream)
+            self.specialized_launches += 1
+        else:
+            launch = self.lib.qsim_apply_blocked_from_zero if from_zero else self.lib.qsim_apply_blocked
+            rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
+                        count, stream)
+        if from_zero:
             zero[0] = False
-        rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
-                    count, stream)
         _lib.check(rc, self.lib)
         if timer is not None and timer[0](_BLOCK_OP):
             ev[1].record(cur)

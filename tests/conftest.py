@@ -30,7 +30,12 @@ def _gpu_backend(dtype):
     from qibo_amd.backend import MI355XBackend
 
     # no fallback: a missing library raises ImportError here and fails the test
-    return MI355XBackend(dtype=dtype)
+    b = MI355XBackend(dtype=dtype)
+    # run-time specialised passes (background hiprtc compiles) are exercised by
+    # test_gpu_specialize.py on backends of their own; the shared backends
+    # interpret every pass, so the other tests need no compile time
+    b.specialize = False
+    return b
 
 
 @pytest.fixture(scope="session")
