@@ -684,14 +684,18 @@ def main():
     pass_flops = [blocking.program_flops(program[e[2]:], e[3], 16) for e in plan if e[0] == "block"]
     flops_per_launch = float(np.mean(pass_flops)) * 2**n if pass_flops else 0.0
     traffic, traffic_src = None, None
-    prof = os.path.join(ROOT, "profiles", "k_blocked_layered30_c128.json")
+    # (the specialised passes' counters when the timed steps ran them)
+    prof_name = "qsim_blocked_spec_layered30_c128.json" if spec_launches else "k_blocked_layered30_c128.json"
+    prof = os.path.join(ROOT, "profiles", prof_name)
     if n == 30 and args.nlayers == 20 and os.path.exists(prof):
         with open(prof) as fh:
             traffic = json.load(fh)["traffic_bytes"]
-        traffic_src = "profiles/k_blocked_layered30_c128.json"
+        traffic_src = "profiles/" + prof_name
     value = ngates * args.steps / elapsed
     avg_dense_ms = float(np.mean(kernel_ms)) if kernel_ms else float("nan")
-    dense_bytes = 2 * (2**n) * itemsize
+    # algorithmic bytes per launch, averaged over the step's passes: each reads
+    # and writes the state once, except the first (from |0...0>), which only writes
+    dense_bytes = (2 * npass - 1) / npass * (2**n) * itemsize if npass else 2 * (2**n) * itemsize
     achieved = dense_bytes / (avg_dense_ms * 1e-3) / 1e9
 
     out = {
