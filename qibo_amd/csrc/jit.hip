@@ -199,9 +199,17 @@ std::string make_source(const std::vector<RecView>& recs, int dtype, int from_ze
   P.re = R(1);
   P.im = R(0);
   bool pdirty = false;
-  __shared__ cplx<R> q_lds[kBT];
+)";
+  // the per-lane factor of LANE_REGPHASE records lives in LDS; a sequence
+  // without them leaves it out, so the kernel needs only the transpose buffer
+  // (32 KiB in complex128: five workgroups per CU instead of four)
+  bool qfactor = false;
+  for (const RecView& r : recs) qfactor |= r.op == kOpLaneRegPhase || r.op == kOpQFlush;
+  if (qfactor)
+    s += R"(  __shared__ cplx<R> q_lds[kBT];
   q_lds[threadIdx.x] = cplx<R>{R(1), R(0)};
-  const uint64_t tb_lo = __builtin_amdgcn_readfirstlane(uint32_t(tile_base));
+)";
+  s += R"(  const uint64_t tb_lo = __builtin_amdgcn_readfirstlane(uint32_t(tile_base));
   const uint64_t tb_hi = __builtin_amdgcn_readfirstlane(uint32_t(tile_base >> 32));
   const uint64_t tile_base_s = tb_lo | (tb_hi << 32);
   (void)tile_base_s;
