@@ -146,34 +146,48 @@ int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int*
                                  size_t program_bytes, int ninstr, void* stream);
 
 /* Blocked passes specialised to their record sequence (no counterpart in the
- * reference: a faster form of qsim_apply_blocked, whose results it
- * reproduces bitwise). The library emits one straight-line kernel per
- * sequence of records -- opcode, argument, entry count and whether a tile
- * condition is present -- with every other field (angles, masks, entries)
- * read from the program at run time, compiles it with hiprtc for gfx950 and
- * loads it into the current device (1-3 s per pass on the host, once per
- * sequence and process). `program` is the host copy of the records.
+ * reference: a faster form of qsim_apply_blocked / _from_zero / _batched and
+ * qsim_adjoint_blocked, whose results it reproduces bitwise). The library
+ * emits one straight-line kernel per sequence of records -- opcode, argument,
+ * entry count and whether a tile condition is present -- with every other
+ * field (angles, masks, entries) read from the program at run time, compiles
+ * it with hiprtc for gfx950 and loads it into the current device (1-3 s per
+ * pass on the host, once per sequence and process). `program` is the host
+ * copy of one program (for a batch: any one of its programs, which share the
+ * sequence). `variant`: QSIM_SPEC_PASS, QSIM_SPEC_PASS_FROM_ZERO (the input
+ * taken to be |0...0>) or QSIM_SPEC_ADJOINT (the adjoint sweep).
  * qsim_blocked_specialize compiles (or finds) the kernel and returns its
  * handle; qsim_blocked_specialized_lookup only finds it (*handle = 0 when the
- * sequence has no kernel yet) and gives a 64-bit hash of the sequence;
- * qsim_apply_blocked_specialized launches it like qsim_apply_blocked (the
- * same arguments and contract) on a device program with the same record
- * sequence (not checked beyond its record count and length: a program of
- * another sequence gives wrong amplitudes). The from-zero form (the input
- * taken to be |0...0>) is a separate kernel (from_zero = 1). A sequence
- * with a conditional mapping change is not specialised (QSIM_EINVAL).
+ * sequence has no kernel yet) and gives a 64-bit hash of the sequence.
+ * qsim_apply_blocked_specialized launches a pass kernel with the arguments
+ * and contract of qsim_apply_blocked_batched (nbatch = 1 for one state);
+ * qsim_adjoint_blocked_specialized an adjoint kernel with those of
+ * qsim_adjoint_blocked. The device programs must have the kernel's record
+ * sequence (checked only through their record count and length: a program
+ * of another sequence gives wrong amplitudes). A sequence with a conditional
+ * mapping change, or whose estimated compile time exceeds 10 s (long
+ * sequences of 2x2, diagonal and inner-product records), is not specialised
+ * by qsim_blocked_specialize / _lookup (QSIM_EINVAL).
  * qsim_blocked_specialized_source returns the kernel's source (length + 1;
  * written to out when cap > 0) and qsim_blocked_compile_check compiles it
  * without a device (both for tests and inspection). */
-int qsim_blocked_specialize(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+#define QSIM_SPEC_PASS 0
+#define QSIM_SPEC_PASS_FROM_ZERO 1
+#define QSIM_SPEC_ADJOINT 2
+int qsim_blocked_specialize(const void* program, size_t program_bytes, int ninstr, int dtype, int variant,
                             uint64_t* handle);
-int qsim_blocked_specialized_lookup(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+int qsim_blocked_specialized_lookup(const void* program, size_t program_bytes, int ninstr, int dtype, int variant,
                                     uint64_t* handle, uint64_t* key_hash);
-int qsim_apply_blocked_specialized(uint64_t handle, void* state, int nqubits, int dtype, const int* tile_bits,
-                                   const void* program_dev, size_t program_bytes, int ninstr, void* stream);
+int qsim_apply_blocked_specialized(uint64_t handle, void* states, int nbatch, int nqubits, int dtype,
+                                   const int* tile_bits, const void* programs_dev, size_t program_stride,
+                                   size_t program_bytes, int ninstr, void* stream);
+int qsim_adjoint_blocked_specialized(uint64_t handle, void* psi, void* lam, int nbatch, int nqubits, int dtype,
+                                     const int* tile_bits, const void* programs_dev, size_t program_stride,
+                                     size_t program_bytes, int ninstr, int nslots, double* partials,
+                                     size_t partials_count, void* stream);
 size_t qsim_blocked_specialized_source(const void* program, size_t program_bytes, int ninstr, int dtype,
-                                       int from_zero, char* out, size_t cap);
-int qsim_blocked_compile_check(const void* program, size_t program_bytes, int ninstr, int dtype, int from_zero,
+                                       int variant, char* out, size_t cap);
+int qsim_blocked_compile_check(const void* program, size_t program_bytes, int ninstr, int dtype, int variant,
                                size_t* code_bytes);
 size_t qsim_blocked_specialized_count(void);
 

@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.pat
                                                           "libqsim_mi355x.so")
 
 C128, C64 = 0, 1
+SPEC_PASS, SPEC_PASS_FROM_ZERO, SPEC_ADJOINT = 0, 1, 2
 F64, F32 = 0, 1
 OK, EINVAL, ENOMEM, EHIP = 0, -1, -2, -3
 
@@ -65,7 +66,9 @@ SIGNATURES = {
     "qsim_apply_blocked_batched": (_i, [_vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
     "qsim_blocked_specialize": (_i, [_vp, _sz, _i, _i, _i, _vp]),
     "qsim_blocked_specialized_lookup": (_i, [_vp, _sz, _i, _i, _i, _vp, _vp]),
-    "qsim_apply_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _i, _i, _c_int_p, _vp, _sz, _i, _vp]),
+    "qsim_apply_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
+    "qsim_adjoint_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i,
+                                              _i, _vp, _sz, _vp]),
     "qsim_blocked_specialized_source": (_sz, [_vp, _sz, _i, _i, _i, ctypes.c_char_p, _sz]),
     "qsim_blocked_compile_check": (_i, [_vp, _sz, _i, _i, _i, _vp]),
     "qsim_blocked_specialized_count": (_sz, []),

@@ -529,8 +529,17 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
         pdev, layout = _upload_programs(tmpl.fwd, tmpl, table, dev)
         keep.append(pdev)
         for p, (off, stride, nbytes) in zip(tmpl.fwd, layout):
-            _lib.check(lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),
-                                                      pdev.data_ptr() + off, stride, nbytes, p.count, stream), lib)
+            # the kernel specialised to the pass's records once compiled
+            # (every set's program has the template row's records)
+            spec = backend._specialized(p.row, p.count, code, _lib.SPEC_PASS, dev) if backend.specialize_sweeps else 0
+            if spec:
+                rc = lib.qsim_apply_blocked_specialized(spec, psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),
+                                                        pdev.data_ptr() + off, stride, nbytes, p.count, stream)
+                backend.specialized_launches += 1
+            else:
+                rc = lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),
+                                                    pdev.data_ptr() + off, stride, nbytes, p.count, stream)
+            _lib.check(rc, lib)
         lam = torch.empty_like(psi)
         for b in range(nb):
             _lib.check(lib.qsim_apply_pauli_sum(psi[b].data_ptr(), n, code, len(c), xm.ctypes.data, zm.ctypes.data,
@@ -542,9 +551,14 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
         for p, (off, stride, nbytes) in zip(tmpl.rev, layout):
             npart = max(1, lib.qsim_adjoint_partials_count(n, nb, len(p.slots)))
             part = torch.empty(npart, dtype=torch.float64, device=dev)
-            _lib.check(lib.qsim_adjoint_blocked(psi.data_ptr(), lam.data_ptr(), nb, n, code, _lib.int_array(p.bits),
-                                                pdev.data_ptr() + off, stride, nbytes, p.count, len(p.slots),
-                                                part.data_ptr(), npart, stream), lib)
+            spec = backend._specialized(p.row, p.count, code, _lib.SPEC_ADJOINT, dev) if backend.specialize_sweeps else 0
+            launch = lib.qsim_adjoint_blocked
+            args = (psi.data_ptr(), lam.data_ptr(), nb, n, code, _lib.int_array(p.bits), pdev.data_ptr() + off,
+                    stride, nbytes, p.count, len(p.slots), part.data_ptr(), npart, stream)
+            if spec:
+                launch, args = lib.qsim_adjoint_blocked_specialized, (spec,) + args
+                backend.specialized_launches += 1
+            _lib.check(launch(*args), lib)
             keep.append(part)
             if p.slots:
                 pof = p.dev_slots.get(dev)

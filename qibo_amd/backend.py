@@ -443,6 +443,10 @@ class MI355XBackend(GatePlanner):
         # straight-line kernel, bitwise equal to the interpreted pass
         self.specialize = os.environ.get("QIBO_AMD_SPECIALIZE", "1") != "0"
         self.specialize_after = 2
+        # the batched sweeps of adjoint gradients too: off by default, their
+        # long runs of 2x2 and diagonal records measured no faster specialised
+        # (HEA-20, 64 sets: 852 against 937 gradients/s in complex128)
+        self.specialize_sweeps = False
         self._spec_seen = {}
         self._spec_jobs = {}
         self._spec_pool = None
@@ -1285,14 +1289,17 @@ class MI355XBackend(GatePlanner):
         return state
 
     # -- specialised blocked passes --------------------------------------------
-    def _specialized(self, host, count, code, from_zero, device):
-        """Handle of the specialised kernel of this program's record sequence,
-        or 0; counts the sighting and starts the compile in the background
-        once the sequence has been launched ``specialize_after`` times."""
+    def _specialized(self, host, count, code, variant, device):
+        """Handle of the specialised kernel of this program's record sequence
+        (``variant``: _lib.SPEC_PASS, SPEC_PASS_FROM_ZERO or SPEC_ADJOINT), or
+        0; counts the sighting and starts the compile in the background once
+        the sequence has been launched ``specialize_after`` times."""
         if not self.specialize or host is None:
             return 0
+        if not isinstance(host, (bytes, bytearray)):
+            host = host.tobytes()
         handle, key_hash = ctypes.c_uint64(0), ctypes.c_uint64(0)
-        rc = self.lib.qsim_blocked_specialized_lookup(host, len(host), count, code, int(from_zero),
+        rc = self.lib.qsim_blocked_specialized_lookup(host, len(host), count, code, int(variant),
                                                       ctypes.byref(handle), ctypes.byref(key_hash))
         if rc != 0:
             return 0  # not specialisable (a conditional mapping change)
@@ -1309,13 +1316,13 @@ class MI355XBackend(GatePlanner):
 
                 self._spec_pool = ThreadPoolExecutor(max_workers=4, thread_name_prefix="qibo_amd_spec")
             self._spec_jobs[key] = self._spec_pool.submit(self._specialize_job, bytes(host), count, code,
-                                                          int(from_zero), device)
+                                                          int(variant), device)
         return 0
 
-    def _specialize_job(self, host, count, code, from_zero, device):
+    def _specialize_job(self, host, count, code, variant, device):
         with torch.cuda.device(device):
             handle = ctypes.c_uint64(0)
-            rc = self.lib.qsim_blocked_specialize(host, len(host), count, code, from_zero, ctypes.byref(handle))
+            rc = self.lib.qsim_blocked_specialize(host, len(host), count, code, variant, ctypes.byref(handle))
             if rc != 0:
                 log.warning("specialised blocked pass not built: %s", self.lib.qsim_last_error().decode())
             return rc
@@ -1336,10 +1343,11 @@ class MI355XBackend(GatePlanner):
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record(cur)
         from_zero = zero is not None and zero[0]
-        handle = self._specialized(host, count, code, from_zero, state.device)
+        variant = _lib.SPEC_PASS_FROM_ZERO if from_zero else _lib.SPEC_PASS
+        handle = self._specialized(host, count, code, variant, state.device)
         if handle:
-            rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), nqubits, code,
-                                                         _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
+            rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), 1, nqubits, code,
+                                                         _lib.int_array(list(tile_bits)), prog.data_ptr(), 0, nbytes,
                                                          count, stream)
             self.specialized_launches += 1
         else:

@@ -595,4 +595,115 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #define QSIM_BLOCKED_WAVES 4
 #endif
 
+// ---------------------------------------------------------------------------
+// Adjoint sweep: the same records applied to two states of each batch entry
+// (psi and lambda), plus INNER records that add Re <lambda| M |psi> over the
+// tile into per-wave partial sums.
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+  uint64_t ntiles;
+  uint64_t nbytes;
+  uint64_t state_stride;
+  uint64_t prog_stride;
+  int ninstr;
+  int nslots;  // INNER records per program
+  uint8_t tbits[kBTileBits];
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+template <typename R>
+__device__ __forceinline__ double re_conj_mul(cplx<R> w, double yr, double yi) {
+  return double(w.re) * yr + double(w.im) * yi;
+}
+
+// Re sum over the lane's pairs of conj(w) . (M v) for a 2x2 M on register bit J.
+template <typename R, int J>
+__device__ __forceinline__ double inner1_regs(const cplx<R>* v, const cplx<R>* w, const Hdr& h,
+                                              const cplx<R>* m) {
+  // Re sum conj(w) M v = sum_ij Re(m_ij S_ij) with S_ij = sum over pairs of
+  // conj(w_i) v_j: the lane accumulates the S_ij (4 FMAs each per pair) and
+  // applies M once. Generators of X/Y rotations have m00 = m11 = 0, so only
+  // S_01 and S_10 are formed for them (a uniform branch on the record).
+  const uint32_t rreq = h.rreq;
+  const bool offdiag = m[0].re == R(0) && m[0].im == R(0) && m[3].re == R(0) && m[3].im == R(0);
+  double s00r = 0.0, s00i = 0.0, s01r = 0.0, s01i = 0.0, s10r = 0.0, s10i = 0.0, s11r = 0.0, s11i = 0.0;
+  if (offdiag) {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      if (!reg_ok(r, rreq)) continue;
+      const int r1 = r | (1 << J);
+      const double a0r = w[r].re, a0i = w[r].im, a1r = w[r1].re, a1i = w[r1].im;
+      const double x0r = v[r].re, x0i = v[r].im, x1r = v[r1].re, x1i = v[r1].im;
+      s01r = fma(a0r, x1r, fma(a0i, x1i, s01r));
+      s01i = fma(a0r, x1i, fma(-a0i, x1r, s01i));
+      s10r = fma(a1r, x0r, fma(a1i, x0i, s10r));
+      s10i = fma(a1r, x0i, fma(-a1i, x0r, s10i));
+    }
+  } else {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((r >> J) & 1) continue;
+      if (!reg_ok(r, rreq)) continue;
+      const int r1 = r | (1 << J);
+      const double a0r = w[r].re, a0i = w[r].im, a1r = w[r1].re, a1i = w[r1].im;
+      const double x0r = v[r].re, x0i = v[r].im, x1r = v[r1].re, x1i = v[r1].im;
+      s00r = fma(a0r, x0r, fma(a0i, x0i, s00r));
+      s00i = fma(a0r, x0i, fma(-a0i, x0r, s00i));
+      s01r = fma(a0r, x1r, fma(a0i, x1i, s01r));
+      s01i = fma(a0r, x1i, fma(-a0i, x1r, s01i));
+      s10r = fma(a1r, x0r, fma(a1i, x0i, s10r));
+      s10i = fma(a1r, x0i, fma(-a1i, x0r, s10i));
+      s11r = fma(a1r, x1r, fma(a1i, x1i, s11r));
+      s11i = fma(a1r, x1i, fma(-a1i, x1r, s11i));
+    }
+  }
+  const double m0r = m[0].re, m0i = m[0].im, m1r = m[1].re, m1i = m[1].im;
+  const double m2r = m[2].re, m2i = m[2].im, m3r = m[3].re, m3i = m[3].im;
+  return (m0r * s00r - m0i * s00i) + (m1r * s01r - m1i * s01i) + (m2r * s10r - m2i * s10i) +
+         (m3r * s11r - m3i * s11i);
+}
+
+template <typename R>
+__device__ __forceinline__ double inner_diag_regs(const cplx<R>* v, const cplx<R>* w, const Hdr& h,
+                                                  const cplx<R>* m, uint64_t lane_idx) {
+  uint32_t rm[4];
+  const uint32_t dl = diag_lane_part(h, lane_idx, rm);
+  const uint32_t rreq = h.rreq;
+  double acc = 0.0;
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    if (!reg_ok(r, rreq)) continue;
+    const uint32_t di = dl | ((r & 1) ? rm[0] : 0u) | ((r & 2) ? rm[1] : 0u) | ((r & 4) ? rm[2] : 0u) |
+                        ((r & 8) ? rm[3] : 0u);
+    const cplx<R> d = m[di];
+    const double yr = double(d.re) * double(v[r].re) - double(d.im) * double(v[r].im);
+    const double yi = double(d.re) * double(v[r].im) + double(d.im) * double(v[r].re);
+    acc += re_conj_mul(w[r], yr, yi);
+  }
+  return acc;
+}
+
+template <typename R>
+__device__ __forceinline__ void flush_lane_phase2(cplx<R>* v, cplx<R>* w, cplx<R>& P, bool& pdirty) {
+  if (!pdirty) return;
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) {
+    v[r] = cmul(P, v[r]);
+    w[r] = cmul(P, w[r]);
+  }
+  P.re = R(1);
+  P.im = R(0);
+  pdirty = false;
+}
+
+#ifndef QSIM_ADJOINT_WAVES
+#define QSIM_ADJOINT_WAVES 3
+#endif
+
 }  // namespace qsim

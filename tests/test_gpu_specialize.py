@@ -60,7 +60,7 @@ def _both(lib, n, code, tb, prog, count, state, from_zero):
     a, b = state.clone(), state.clone()
     interp = lib.qsim_apply_blocked_from_zero if from_zero else lib.qsim_apply_blocked
     _lib.check(interp(a.data_ptr(), n, code, tba, dev.data_ptr(), len(prog), count, None), lib)
-    _lib.check(lib.qsim_apply_blocked_specialized(handle.value, b.data_ptr(), n, code, tba, dev.data_ptr(),
+    _lib.check(lib.qsim_apply_blocked_specialized(handle.value, b.data_ptr(), 1, n, code, tba, dev.data_ptr(), 0,
                                                   len(prog), count, None), lib)
     torch.cuda.synchronize()
     return a, b
@@ -149,7 +149,7 @@ def test_specialised_launch_errors(backend):
         (h.value, psi, _lib.C128, len(prog), count - 1),
     ]
     for handle, st, code, nbytes, cnt in bad:
-        assert lib.qsim_apply_blocked_specialized(handle, st.data_ptr(), n, code, tba, dev.data_ptr(), nbytes,
+        assert lib.qsim_apply_blocked_specialized(handle, st.data_ptr(), 1, n, code, tba, dev.data_ptr(), 0, nbytes,
                                                   cnt, None) == _lib.EINVAL
 
 
@@ -181,3 +181,50 @@ def test_backend_specialises_repeated_circuits(dtype):
     ref = OracleBackend().execute(models.random_layered_circuit(n, 6, seed=11, measure=False))
     tol = 1e-12 if dtype == "complex128" else 5e-6
     assert np.max(np.abs(got.state(numpy=True).astype(np.complex128) - ref)) <= tol
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_adjoint_sweeps_specialised_bitwise(dtype):
+    """Energies and gradients of a batch of parameter sets from sweeps whose
+    forward and reverse passes run their specialised kernels equal those of
+    the interpreted sweeps bitwise."""
+    from qibo_amd import hamiltonians
+    from qibo_amd.adjoint import batched_expectation_and_gradient
+
+    from qibo_amd import Circuit, gates
+
+    # small enough that every sweep pass fits the library's compile budget
+    n, nsets = 13, 8
+    c = Circuit(n)
+    for q in range(8):
+        c.add(gates.RY(q, 0.1))
+    for q in range(n - 1):
+        c.add(gates.CNOT(q, q + 1))
+    for q in range(4):
+        c.add(gates.RZ(q, 0.2))
+        c.add(gates.RX(q + 5, 0.3))
+    params = np.random.default_rng(3).uniform(0, 2 * np.pi, (nsets, 16))
+    spec = MI355XBackend(dtype=dtype)
+    spec.specialize, spec.specialize_after, spec.specialize_sweeps = True, 1, True
+    plain = MI355XBackend(dtype=dtype)
+    plain.specialize = False
+    ham_s = hamiltonians.TFIM(n, h=1.0, closed_boundary=False, backend=spec)
+    ham_p = hamiltonians.TFIM(n, h=1.0, closed_boundary=False, backend=plain)
+    batched_expectation_and_gradient(spec, c, ham_s, params)  # compiles start here
+    assert spec.wait_specializations(timeout=600) >= 2
+    from qibo_amd.adjoint import sweep_template
+
+    tmpl = sweep_template(spec, c)
+    code = _lib.C128 if dtype == "complex128" else _lib.C64
+    for p, variant in [(p, _lib.SPEC_PASS) for p in tmpl.fwd] + [(p, _lib.SPEC_ADJOINT) for p in tmpl.rev]:
+        h = ctypes.c_uint64(0)
+        row = p.row.tobytes()
+        assert spec.lib.qsim_blocked_specialized_lookup(row, len(row), p.count, code, variant, ctypes.byref(h),
+                                                        None) == 0
+        assert h.value, "every sweep pass has its kernel"
+    before = spec.specialized_launches
+    e1, g1 = batched_expectation_and_gradient(spec, c, ham_s, params)
+    assert spec.specialized_launches > before
+    e2, g2 = batched_expectation_and_gradient(plain, c, ham_p, params)
+    assert torch.equal(e1, e2)
+    assert torch.equal(g1, g2)

@@ -74,3 +74,23 @@ def test_conditional_mapping_change_is_refused(lib):
     assert lib.qsim_blocked_specialized_source(prog, len(prog), count, _lib.C128, 0, None, 0) == 0
     assert b"conditional mapping change" in lib.qsim_last_error()
     assert lib.qsim_blocked_compile_check(prog, len(prog), count, _lib.C128, 0, None) == _lib.EINVAL
+
+
+def test_adjoint_sweep_sources_compile(lib):
+    """The adjoint sweep's kernels (both states, INNER records at fixed slots)
+    for the forward and reverse passes of a hardware-efficient ansatz."""
+    from qibo_amd import models
+    from qibo_amd.adjoint import SweepTemplate
+    from qibo_amd.backend import GatePlanner
+
+    tmpl = SweepTemplate(GatePlanner("complex64"), models.hardware_efficient_ansatz(13, 1), np.complex64)
+    p = tmpl.rev[0]
+    prog = p.row.tobytes()
+    ops = {op for op, _, _ in record_ops(prog, np.complex64)}
+    assert {8, 9} <= ops  # INNER1 and INNER_DIAG
+    src = _source(lib, prog, p.count, _lib.C64, _lib.SPEC_ADJOINT)
+    assert src.count("wave_sum(acc)") == len(p.slots)
+    size = ctypes.c_size_t(0)
+    assert lib.qsim_blocked_compile_check(prog, len(prog), p.count, _lib.C64, _lib.SPEC_ADJOINT,
+                                          ctypes.byref(size)) == 0, lib.qsim_last_error().decode()[-2000:]
+    assert lib.qsim_blocked_specialized_source(prog, len(prog), p.count, _lib.C64, 3, None, 0) == 0
