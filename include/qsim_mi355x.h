@@ -20,8 +20,27 @@
  *     as the most significant bit, as gate.qubits orders them in qibo.
  *   - Return value: 0 on success, a negative QSIM_E* code otherwise; the
  *     message of the last failure on the calling thread is qsim_last_error().
- *   - Every entry point is re-entrant: no global mutable state is shared
- *     between calls (qibo's parallel.py calls execute_circuit from threads).
+ *   - Every entry point is re-entrant (qibo's parallel.py calls
+ *     execute_circuit from threads). State shared between calls: the table of
+ *     run-time specialised kernels (qsim_blocked_specialize; guarded by a
+ *     mutex) and the device array of qsim_expval_pauli_noalloc (calls that
+ *     overlap on one GPU race, as its comment says).
+ *
+ * Where this ABI departs from a plain one-call-per-reference-method list:
+ *   - controlled gates are qsim_apply_dense / qsim_apply_diag / qsim_apply_swap
+ *     with a control list (no separate "apply_controlled" entry);
+ *   - sampling is qsim_sample_exact (draws from probabilities in one call,
+ *     with np.random.choice's checks), plus qsim_cumsum_exact and
+ *     qsim_search_sorted for callers that need the cumulative sum itself;
+ *   - precision is the state dtype: complex64 states run FP32 (MFMA for fused
+ *     blocks), complex128 FP64; qsim_compare_states measures the accuracy of
+ *     one against the other (no "precision_mode" argument);
+ *   - gradients are qsim_adjoint_blocked + qsim_adjoint_reduce over batched
+ *     blocked passes, with qsim_apply_pauli_sum / qsim_inner_matrix for the
+ *     per-set sweep;
+ *   - serving is the Python executor (qibo_amd/serving.py) over these entry
+ *     points (batched launches, HIP graphs of them); there is no separate
+ *     serving entry point.
  */
 #ifndef QSIM_MI355X_H
 #define QSIM_MI355X_H
