@@ -616,12 +616,48 @@ __device__ __forceinline__ Map block_scan_excl(Map v, Map* wave_tot) {
   return compose(pre, excl);
 }
 
-// (C) per-tile decision and map
+// The 16 consecutive elements [i0, i0 + 16) of one thread, widened to
+// double (0 past n), read with 16-byte loads when the run is whole and the
+// array 16-byte aligned.
+__device__ __forceinline__ void load_run(const double* __restrict__ p, int64_t i0, int64_t n, double* x) {
+  if (i0 + kPer <= n && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    const double2* q = reinterpret_cast<const double2*>(p + i0);
+#pragma unroll
+    for (int k = 0; k < kPer / 2; ++k) {
+      const double2 v = q[k];
+      x[2 * k] = v.x;
+      x[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) x[k] = i0 + k < n ? p[i0 + k] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void load_run(const float* __restrict__ p, int64_t i0, int64_t n, double* x) {
+  if (i0 + kPer <= n && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    const float4* q = reinterpret_cast<const float4*>(p + i0);
+#pragma unroll
+    for (int k = 0; k < kPer / 4; ++k) {
+      const float4 v = q[k];
+      x[4 * k] = double(v.x);
+      x[4 * k + 1] = double(v.y);
+      x[4 * k + 2] = double(v.z);
+      x[4 * k + 3] = double(v.w);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) x[k] = i0 + k < n ? double(p[i0 + k]) : 0.0;
+  }
+}
+
+// (C) per-tile decision and map. Each thread reads its own 16 consecutive
+// elements with wide loads (every byte of the tile's cache lines is used,
+// without an LDS staging pass), so the kernel runs at 8 waves per SIMD.
 template <typename P>
 __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_t n, const double* __restrict__ astart,
                                                  const double* __restrict__ tsum, TileInfo* __restrict__ info,
                                                  Map* __restrict__ xmap) {
-  __shared__ double tile[kTile];
   __shared__ Map wave_tot[kT / 64];
   const int64_t t = blockIdx.x;
   const double A = astart[t];
@@ -634,10 +670,11 @@ __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_
     if (threadIdx.x == 0) info[t] = {0, 0, Eb, 0};
     return;
   }
-  stage_tile(p, n, t * kTile, tile);
+  double x[kPer];
+  load_run(p, t * kTile + int64_t(threadIdx.x) * kPer, n, x);
   DMap fd = {0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) fd = compose_fp(fd, elem_map_fp(tile[threadIdx.x * kPer + k], Eb));
+  for (int k = 0; k < kPer; ++k) fd = compose_fp(fd, elem_map_fp(x[k], Eb));
   const Map f = to_int_map(fd);
   const Map excl = block_scan_excl(f, wave_tot);
   xmap[t * kT + threadIdx.x] = excl;  // reused by k_tile_apply (same tile, same grid Eb)

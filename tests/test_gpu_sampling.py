@@ -206,3 +206,21 @@ def test_tile_statistics_not_used_after_an_in_place_change(backend, rng):
     np.random.seed(3)
     u = np.random.random_sample(20000)
     np.testing.assert_array_equal(got, np.searchsorted(cdf / cdf[-1], u, side="right"))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sample_exact_unaligned_probabilities(backend, rng, dtype):
+    """Probabilities that start off a 16-byte boundary (a view one entry in)
+    take the element-wise reads and give the same draws as numpy."""
+    p = rng.exponential(size=2**16 + 1).astype(dtype)
+    p[0] = 0.0
+    p /= p.astype(np.float64).sum()
+    full = torch.as_tensor(p, device=backend.torch_device)
+    view = full[1:]
+    assert view.data_ptr() % 16 != 0
+    q = p[1:]
+    u = _uniforms(q, rng, k=5000)
+    got, _ = _sample(backend, view, u)
+    cdf = np.cumsum(q.astype(np.float64))
+    np.testing.assert_array_equal(got, np.searchsorted(cdf / cdf[-1], u, side="right"))
+    np.testing.assert_array_equal(got, _search_full(backend, view.contiguous(), u))
