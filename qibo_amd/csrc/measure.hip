@@ -659,7 +659,8 @@ __global__ __launch_bounds__(kT) void k_tile_map(const P* __restrict__ p, int64_
                                                  const double* __restrict__ tsum, TileInfo* __restrict__ info,
                                                  Map* __restrict__ xmap) {
   __shared__ Map wave_tot[kT / 64];
-  const int64_t t = blockIdx.x;
+  const int64_t t = int64_t(xcd_block());  // each XCD walks one contiguous range of tiles
+  if (t >= (n + kTile - 1) / kTile) return;
   const double A = astart[t];
   const double Z = A + tsum[t];
   const int Eb = binade_of(A);
@@ -1046,7 +1047,7 @@ static int cumsum_impl(const void* probs, int64_t n, double* cdf, void* ws, hipS
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_approx_prefix, dim3(1), dim3(1024), 0, s, c.tsum, T, c.astart);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info, c.xmap);
+  hipLaunchKernelGGL(k_tile_map<P>, launch_grid(uint64_t(T)), dim3(kT), 0, s, p, n, c.astart, c.tsum, c.info, c.xmap);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, c.info, T, c.ginfo);
   QSIM_CHECK_LAUNCH();
@@ -1111,15 +1112,20 @@ template <typename R>
 __global__ __launch_bounds__(kT) void k_probs_tiles(const cplx<R>* __restrict__ psi, R* __restrict__ out, int64_t n,
                                                     double* __restrict__ tsum, int2* __restrict__ tbad) {
   __shared__ double lds[kT];
-  const int64_t base = int64_t(blockIdx.x) * kTile;
+  // each XCD walks one contiguous range of tiles (see xcd_block)
+  const int64_t tile = int64_t(xcd_block());
+  if (tile >= (n + kTile - 1) / kTile) return;  // a 2-D grid's last row may run past the tiles
+  const int64_t base = tile * kTile;
   double acc = 0.0;
   int neg = 0, nan = 0;
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int64_t i = base + k * kT + threadIdx.x;
     if (i < n) {
-      const R v = np_abs2(psi[i]);
-      out[i] = v;
+      // streaming loads and stores: the state is read once here and p is
+      // read once more by the maps, neither stays in the caches usefully
+      const R v = np_abs2(ld_nt(psi, uint64_t(i)));
+      __builtin_nontemporal_store(v, out + i);
       const double x = double(v);
       acc += x;
       neg += x < 0.0 ? 1 : 0;
@@ -1130,8 +1136,8 @@ __global__ __launch_bounds__(kT) void k_probs_tiles(const cplx<R>* __restrict__ 
   const double nneg = block_sum_fixed(double(neg), lds);
   const double nnan = block_sum_fixed(double(nan), lds);
   if (threadIdx.x == 0) {
-    tsum[blockIdx.x] = tot;
-    tbad[blockIdx.x] = make_int2(int(nneg), int(nnan));
+    tsum[tile] = tot;
+    tbad[tile] = make_int2(int(nneg), int(nnan));
   }
 }
 
@@ -1400,7 +1406,7 @@ static int sample_impl(const void* probs, int64_t n, const void* tiles, const do
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_scan_starts, dim3(unsigned(NP)), dim3(kT), 0, s, a.c.tsum, T, a.pstart, a.c.astart);
   QSIM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_tile_map<P>, dim3(unsigned(T)), dim3(kT), 0, s, p, n, a.c.astart, a.c.tsum, a.c.info,
+  hipLaunchKernelGGL(k_tile_map<P>, launch_grid(uint64_t(T)), dim3(kT), 0, s, p, n, a.c.astart, a.c.tsum, a.c.info,
                      a.c.xmap);
   QSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_group_map, dim3(unsigned(G)), dim3(64), 0, s, a.c.info, T, a.c.ginfo);
@@ -1687,10 +1693,10 @@ int qsim_probabilities_tiles(const void* state, int nqubits, int dtype, void* pr
   int2* tbad = reinterpret_cast<int2*>(reinterpret_cast<char*>(tile_stats) + align256(T * sizeof(double)));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (dtype == QSIM_C128)
-    hipLaunchKernelGGL(k_probs_tiles<double>, dim3(unsigned(T)), dim3(kT), 0, s,
+    hipLaunchKernelGGL(k_probs_tiles<double>, launch_grid(uint64_t(T)), dim3(kT), 0, s,
                        reinterpret_cast<const cplx<double>*>(state), reinterpret_cast<double*>(probs), n, tsum, tbad);
   else
-    hipLaunchKernelGGL(k_probs_tiles<float>, dim3(unsigned(T)), dim3(kT), 0, s,
+    hipLaunchKernelGGL(k_probs_tiles<float>, launch_grid(uint64_t(T)), dim3(kT), 0, s,
                        reinterpret_cast<const cplx<float>*>(state), reinterpret_cast<float*>(probs), n, tsum, tbad);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
