@@ -641,6 +641,7 @@ def main():
 
     # events around every blocked-pass launch on the stream it runs on
     backend.block_stats.clear()
+    backend.block_bytes.clear()
     hits0 = backend.program_cache_hits
     timer = backend.enable_kernel_timer("block")
     torch.cuda.synchronize()
@@ -652,6 +653,7 @@ def main():
     elapsed = time.perf_counter() - t0
     kernel_ms = backend.disable_kernel_timer()
     passes = list(backend.block_stats)
+    launch_bytes = list(backend.block_bytes)  # (algorithmic bytes, over every tile) per launch
     cache_hits = backend.program_cache_hits - hits0
     spec_launches = backend.specialized_launches - spec0
 
@@ -686,7 +688,11 @@ def main():
     # read the state and write 8 B per amplitude; the sampling maps read the
     # probabilities again and write 1 B per amplitude (16 B per 16 entries)
     npass = len(passes) / args.steps if passes else 0.0
-    step_bytes = (2 * npass - 1) * (2**n) * itemsize + (2**n) * (itemsize + 8) + (2**n) * 9
+    # the passes' bytes as launched: the first (from |0...0>) writes the state
+    # once (its zero fill) and the next ones run only the tiles of the support
+    # built so far (the tiles outside it are zero); see backend.block_bytes
+    pass_bytes = sum(b for b, _ in launch_bytes) / args.steps if launch_bytes else 0.0
+    step_bytes = pass_bytes + (2**n) * (itemsize + 8) + (2**n) * 9
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of
     # this same workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scales read off a
     # plain copy of one state buffer); only quoted for the workload it measured
@@ -696,6 +702,9 @@ def main():
     ops = [backend.plan_gate(g) for g in circuit.queue if g.__class__.__name__ != "M"]
     plan, program = blocking.compile_circuit(ops, n, np.complex128, 3)
     pass_flops = [blocking.program_flops(program[e[2]:], e[3], 16) for e in plan if e[0] == "block"]
+    step_dense = [d for _, d in launch_bytes[: len(pass_flops)]]
+    if len(step_dense) == len(pass_flops) and any(step_dense):  # the passes over every tile
+        pass_flops = [f for f, d in zip(pass_flops, step_dense) if d]
     flops_per_launch = float(np.mean(pass_flops)) * 2**n if pass_flops else 0.0
     traffic, traffic_src = None, None
     # (the specialised passes' counters when the timed steps ran them)
@@ -706,10 +715,12 @@ def main():
             traffic = json.load(fh)["traffic_bytes"]
         traffic_src = "profiles/" + prof_name
     value = ngates * args.steps / elapsed
-    avg_dense_ms = float(np.mean(kernel_ms)) if kernel_ms else float("nan")
-    # algorithmic bytes per launch, averaged over the step's passes: each reads
-    # and writes the state once, except the first (from |0...0>), which only writes
-    dense_bytes = (2 * npass - 1) / npass * (2**n) * itemsize if npass else 2 * (2**n) * itemsize
+    # the roofline of the pass over every tile (the state read and written
+    # once): the launches that ran on part of the tiles are left out
+    dense_ms = [ms for ms, (_, dense) in zip(kernel_ms, launch_bytes) if dense] if len(kernel_ms) == len(
+        launch_bytes) else list(kernel_ms)
+    avg_dense_ms = float(np.mean(dense_ms)) if dense_ms else float("nan")
+    dense_bytes = 2 * (2**n) * itemsize
     achieved = dense_bytes / (avg_dense_ms * 1e-3) / 1e9
 
     out = {
@@ -736,9 +747,10 @@ def main():
         "hbm_GBs_step": {
             "bytes_per_step": step_bytes,
             "GBs": step_bytes / (elapsed / args.steps) / 1e9,
-            "counted": "per blocked pass one read and one write of the state (the first pass "
-                       "only writes), probabilities (16 B read + 8 B written per amplitude), "
-                       "sampling maps (8 B read + 1 B written per amplitude)",
+            "counted": "per blocked pass one read and one write of the tiles it runs (the first, "
+                       "from |0...0>, writes the state once; a pass on part of the tiles moves only "
+                       "those), probabilities (16 B read + 8 B written per amplitude), sampling maps "
+                       "(8 B read + 1 B written per amplitude)",
         },
         # the backend keeps the compiled passes of a repeated identical gate
         # queue (same gates, qubits and parameter values); every timed step
@@ -770,7 +782,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "launches_timed": len(kernel_ms),
+            "launches_timed": len(dense_ms),
+            "launches_on_part_of_the_tiles": len(kernel_ms) - len(dense_ms),
             "avg_launch_ms": avg_dense_ms,
             "gates_per_launch": (sum(passes) / len(passes)) if passes else None,
             "passes_per_step": len(passes) / args.steps,

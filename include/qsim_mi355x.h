@@ -160,9 +160,21 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
 /* qsim_apply_blocked with the input taken to be |0...0>: the buffer's old
  * contents are never read, only the result is written. The first pass of
  * Backend._execute_circuit from zero_state (abstract.py:2243-2275 then
- * 3321-3322) without writing the zero state first. */
+ * 3321-3322). |0...0> is nonzero only in tile 0 and the pass acts on its
+ * tile bits only, so the call writes zeros everywhere (qsim_zero_state) and
+ * runs the pass on tile 0 alone. */
 int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                                  size_t program_bytes, int ninstr, void* stream);
+
+/* qsim_apply_blocked for an input known to vanish wherever an index bit
+ * outside `support` is 1 (bit b of the mask = index bit b, i.e. qubit
+ * n-1-b): the pass runs only the tiles whose bits outside the tile bits lie
+ * in `support`, and leaves the others, which stay zero, untouched. A state
+ * built by passes from |0...0> has the union of their tile bits as its
+ * support (no counterpart in the reference: it skips amplitudes that are
+ * zero; all ones is qsim_apply_blocked). */
+int qsim_apply_blocked_support(void* state, int nqubits, int dtype, const int* tile_bits, uint64_t support,
+                               const void* program_dev, size_t program_bytes, int ninstr, void* stream);
 
 /* Blocked passes specialised to their record sequence (no counterpart in the
  * reference: a faster form of qsim_apply_blocked / _from_zero / _batched and
@@ -179,7 +191,10 @@ int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int*
  * handle; qsim_blocked_specialized_lookup only finds it (*handle = 0 when the
  * sequence has no kernel yet) and gives a 64-bit hash of the sequence.
  * qsim_apply_blocked_specialized launches a pass kernel with the arguments
- * and contract of qsim_apply_blocked_batched (nbatch = 1 for one state);
+ * and contract of qsim_apply_blocked_batched (nbatch = 1 for one state) and
+ * the `support` of qsim_apply_blocked_support (all ones: every tile; a
+ * from-zero kernel writes |0...0> first and ignores it, as
+ * qsim_apply_blocked_from_zero does);
  * qsim_adjoint_blocked_specialized an adjoint kernel with those of
  * qsim_adjoint_blocked. The device programs must have the kernel's record
  * sequence (checked only through their record count and length: a program
@@ -198,8 +213,8 @@ int qsim_blocked_specialize(const void* program, size_t program_bytes, int ninst
 int qsim_blocked_specialized_lookup(const void* program, size_t program_bytes, int ninstr, int dtype, int variant,
                                     uint64_t* handle, uint64_t* key_hash);
 int qsim_apply_blocked_specialized(uint64_t handle, void* states, int nbatch, int nqubits, int dtype,
-                                   const int* tile_bits, const void* programs_dev, size_t program_stride,
-                                   size_t program_bytes, int ninstr, void* stream);
+                                   const int* tile_bits, uint64_t support, const void* programs_dev,
+                                   size_t program_stride, size_t program_bytes, int ninstr, void* stream);
 int qsim_adjoint_blocked_specialized(uint64_t handle, void* psi, void* lam, int nbatch, int nqubits, int dtype,
                                      const int* tile_bits, const void* programs_dev, size_t program_stride,
                                      size_t program_bytes, int ninstr, int nslots, double* partials,

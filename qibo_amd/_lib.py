@@ -66,7 +66,9 @@ SIGNATURES = {
     "qsim_apply_blocked_batched": (_i, [_vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
     "qsim_blocked_specialize": (_i, [_vp, _sz, _i, _i, _i, _vp]),
     "qsim_blocked_specialized_lookup": (_i, [_vp, _sz, _i, _i, _i, _vp, _vp]),
-    "qsim_apply_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i, _vp]),
+    "qsim_apply_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _i, _i, _i, _c_int_p, ctypes.c_uint64, _vp, _sz,
+                                            _sz, _i, _vp]),
+    "qsim_apply_blocked_support": (_i, [_vp, _i, _i, _c_int_p, ctypes.c_uint64, _vp, _sz, _i, _vp]),
     "qsim_adjoint_blocked_specialized": (_i, [ctypes.c_uint64, _vp, _vp, _i, _i, _i, _c_int_p, _vp, _sz, _sz, _i,
                                               _i, _vp, _sz, _vp]),
     "qsim_blocked_specialized_source": (_sz, [_vp, _sz, _i, _i, _i, ctypes.c_char_p, _sz]),

@@ -534,7 +534,8 @@ def blocked_expectation_and_gradient(backend, circuit, hamiltonian, params: np.n
             spec = backend._specialized(p.row, p.count, code, _lib.SPEC_PASS, dev) if backend.specialize_sweeps else 0
             if spec:
                 rc = lib.qsim_apply_blocked_specialized(spec, psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),
-                                                        pdev.data_ptr() + off, stride, nbytes, p.count, stream)
+                                                        (1 << 64) - 1, pdev.data_ptr() + off, stride, nbytes,
+                                                        p.count, stream)
                 backend.specialized_launches += 1
             else:
                 rc = lib.qsim_apply_blocked_batched(psi.data_ptr(), nb, n, code, _lib.int_array(p.bits),

@@ -428,6 +428,8 @@ class MI355XBackend(GatePlanner):
         # memory segment); None: 3 for complex128, 4 for complex64 (128 bytes)
         self.blocking_low_bits = None
         self.block_stats = collections.deque(maxlen=1 << 16)  # gates per blocked pass (diagnostics)
+        # (algorithmic HBM bytes, over every tile) per blocked launch (diagnostics)
+        self.block_bytes = collections.deque(maxlen=1 << 16)
         # compiled blocked passes of recently executed gate queues, keyed by the
         # gates' kinds, qubits and exact parameter values (see run_queue)
         # one-qubit channels inside density-matrix circuits run as their 4x4
@@ -452,6 +454,10 @@ class MI355XBackend(GatePlanner):
         self._spec_pool = None
         self._spec_lock = threading.Lock()
         self.specialized_launches = 0
+        self.sparse_launches = 0  # blocked passes run on part of the tiles (support below every bit)
+        # passes after a start from |0...0> run only the tiles whose amplitudes
+        # can be nonzero (the union of the earlier passes' tile bits)
+        self.skip_zero_tiles = True
         self._programs = collections.OrderedDict()
         self._programs_lock = threading.Lock()
         self.program_cache_hits = 0
@@ -970,7 +976,11 @@ class MI355XBackend(GatePlanner):
         stand for |0...0>: a first blocked pass produces it itself, anything
         else fills it first.
         """
-        zero = [zero_pending]  # still to be materialised
+        # [|0...0> still to be materialised, support]: the support is the mask
+        # of index bits that may be 1 in a nonzero amplitude (None: any); the
+        # passes from |0...0> grow it by their tile bits and skip the tiles
+        # outside it (all zero)
+        zero = [zero_pending, 0 if zero_pending else None]
         if zero_pending and (not self.blocking or nqubits < blocking.TILE_BITS):
             self._fill_zero(state, nqubits)
             zero[0] = False
@@ -1016,6 +1026,7 @@ class MI355XBackend(GatePlanner):
                     self._cache_put(key, record)
                 return state
             state = stop.pop().apply(self, state, nqubits)
+            zero[1] = None  # a special gate may touch any amplitude
 
     # -- compiled-program cache ---------------------------------------------
     @staticmethod
@@ -1126,6 +1137,8 @@ class MI355XBackend(GatePlanner):
                     self._fill_zero(state, nqubits)
                     zero[0] = False
                 self.apply_op(act[1], state, nqubits)
+                if len(zero) > 1:
+                    zero[1] = None
                 continue
             _, tile_bits, prog, nbytes, count, ngates, host = act
             self._launch_blocked(state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero, host)
@@ -1274,6 +1287,8 @@ class MI355XBackend(GatePlanner):
                     self._fill_zero(state, nqubits)
                     zero[0] = False
                 self.apply_op(payload, state, nqubits)
+                if zero is not None and len(zero) > 1:
+                    zero[1] = None
                 if record is not None:
                     record.append(("single", payload))
                 continue
@@ -1343,19 +1358,50 @@ class MI355XBackend(GatePlanner):
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record(cur)
         from_zero = zero is not None and zero[0]
+        full = (1 << nqubits) - 1
+        tmask = sum(1 << b for b in tile_bits)
+        # amplitudes with a 1 outside the support are zero before the pass and
+        # after it: only the tiles inside it run (from |0...0>: tile 0 alone)
+        support = zero[1] if zero is not None and len(zero) > 1 and zero[1] is not None else full
+        if not self.skip_zero_tiles:
+            support = full
+        if from_zero:
+            support = 0  # (the from-zero entry points fill the state and run tile 0 alone)
+        sparse = not from_zero and (support | tmask) != full
         variant = _lib.SPEC_PASS_FROM_ZERO if from_zero else _lib.SPEC_PASS
         handle = self._specialized(host, count, code, variant, state.device)
+        tba = _lib.int_array(list(tile_bits))
         if handle:
-            rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), 1, nqubits, code,
-                                                         _lib.int_array(list(tile_bits)), prog.data_ptr(), 0, nbytes,
-                                                         count, stream)
+            rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), 1, nqubits, code, tba,
+                                                         support if sparse else _ALL_BITS, prog.data_ptr(), 0,
+                                                         nbytes, count, stream)
             self.specialized_launches += 1
+        elif from_zero:
+            rc = self.lib.qsim_apply_blocked_from_zero(state.data_ptr(), nqubits, code, tba, prog.data_ptr(), nbytes,
+                                                       count, stream)
+        elif sparse:
+            rc = self.lib.qsim_apply_blocked_support(state.data_ptr(), nqubits, code, tba, support, prog.data_ptr(),
+                                                     nbytes, count, stream)
         else:
-            launch = self.lib.qsim_apply_blocked_from_zero if from_zero else self.lib.qsim_apply_blocked
-            rc = launch(state.data_ptr(), nqubits, code, _lib.int_array(list(tile_bits)), prog.data_ptr(), nbytes,
-                        count, stream)
+            rc = self.lib.qsim_apply_blocked(state.data_ptr(), nqubits, code, tba, prog.data_ptr(), nbytes, count,
+                                             stream)
         if from_zero:
             zero[0] = False
+        if zero is not None and len(zero) > 1:
+            zero[1] = (support | tmask) if (support | tmask) != full else None
+        if sparse:
+            self.sparse_launches += 1
+        # algorithmic HBM bytes of the launch: the state read and written once
+        # (from |0...0>: written once, by the fill), or the tiles in the support
+        isz = 16 if code == _lib.C128 else 8
+        if from_zero:
+            moved = (1 << nqubits) * isz
+        elif sparse:
+            nfree = bin(support & ~tmask & full).count("1")
+            moved = 2 * (1 << (nfree + len(tile_bits))) * isz
+        else:
+            moved = 2 * (1 << nqubits) * isz
+        self.block_bytes.append((moved, not (sparse or from_zero)))
         _lib.check(rc, self.lib)
         if timer is not None and timer[0](_BLOCK_OP):
             ev[1].record(cur)
@@ -1675,6 +1721,7 @@ def _dm_pair(op: _Op, nqubits: int) -> List[_Op]:
     col = _Op(op.kind, tuple(q + nqubits for q in op.targets), tuple(q + nqubits for q in op.controls), data)
     return [op, col]
 _BLOCK_OP = _Op("block", (), (), None)
+_ALL_BITS = (1 << 64) - 1  # support mask of a pass over every tile
 
 
 def ctypes_stream(stream) -> int:

@@ -65,7 +65,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   uint64_t gbit[kBTileBits];
 #pragma clang loop unroll(full)
   for (int i = 0; i < kBTileBits; ++i) gbit[i] = uint64_t(1) << a.tbits[i];
-  const uint64_t tile_base = insert_zeros(tile, a.tbits, kBTileBits);
+  const uint64_t tile_base = tile_origin(a, tile);
 
   int map = 0;
   // global index of the lane's amplitudes with the register bits cleared
@@ -259,17 +259,38 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
   for (int r = 0; r < kBReg; ++r) st_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit), v[r]);
 }
 
+// The tiles of a pass with tile bits `tbits` over n qubits whose input
+// vanishes wherever a bit outside `support` is 1 (all ones: every tile).
+// (also used by the specialised passes, jit.hip)
+void set_tiles(BlkArgs& a, int n, const int* tbits, uint64_t support) {
+  uint64_t tmask = 0;
+  for (int i = 0; i < kBTileBits; ++i) tmask |= uint64_t(1) << tbits[i];
+  const uint64_t all = (uint64_t(1) << n) - 1;
+  const uint64_t outer = all & ~tmask;
+  const uint64_t freeb = outer & support;
+  if (freeb == outer) {
+    a.nfree = kDenseTiles;
+    a.ntiles = uint64_t(1) << (n - kBTileBits);
+    return;
+  }
+  a.nfree = 0;
+  for (int b = 0; b < n; ++b)
+    if ((freeb >> b) & 1) a.fbits[a.nfree++] = uint8_t(b);
+  a.ntiles = uint64_t(1) << a.nfree;
+}
+
 template <typename R>
 static int launch_blocked(void* state, int n, const int* tbits, const void* prog, size_t nbytes, int ninstr,
-                          int nbatch, size_t prog_stride, hipStream_t s, bool from_zero = false) {
+                          int nbatch, size_t prog_stride, hipStream_t s, bool from_zero = false,
+                          uint64_t support = ~uint64_t(0)) {
   BlkArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.ntiles = uint64_t(1) << (n - kBTileBits);
   a.nbytes = nbytes;
   a.state_stride = uint64_t(1) << n;
   a.prog_stride = prog_stride;
   a.ninstr = ninstr;
   for (int i = 0; i < kBTileBits; ++i) a.tbits[i] = uint8_t(tbits[i]);
+  set_tiles(a, n, tbits, support);
   dim3 grid = launch_grid(a.ntiles);
   grid.z = unsigned(nbatch);
   if (from_zero)
@@ -589,9 +610,29 @@ int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int*
                          "qsim_apply_blocked_from_zero: bad arguments") != QSIM_OK)
     return QSIM_EINVAL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // |0...0> has its only nonzero amplitude in tile 0 and the pass acts on the
+  // tile bits only: every other tile of the output is zero. Fill the state
+  // with |0...0> (zeros at streaming speed) and run the pass on tile 0 alone.
+  const int zrc = qsim_zero_state(state, nqubits, dtype, stream);
+  if (zrc != QSIM_OK) return zrc;
   if (dtype == QSIM_C128)
-    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true);
-  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true);
+    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true, 0);
+  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true, 0);
+}
+
+int qsim_apply_blocked_support(void* state, int nqubits, int dtype, const int* tile_bits, uint64_t support,
+                               const void* program_dev, size_t program_bytes, int ninstr, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_apply_blocked_support: null state");
+  if (qsim::check_blocked_args(nqubits, dtype, tile_bits, program_dev, program_bytes, ninstr,
+                               "qsim_apply_blocked_support: bad arguments") != QSIM_OK)
+    return QSIM_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, false,
+                                  support);
+  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, false,
+                               support);
 }
 
 int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,

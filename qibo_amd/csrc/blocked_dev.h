@@ -49,6 +49,9 @@ static_assert(sizeof(BlkHdr) == 32, "record header is 32 bytes");
 // Entries: DIAG / LANE_PHASE 2^nt; DENSE1*: 2x2 row-major; REG_PHASE: 16;
 // DENSE_LDS: 2^k x 2^k row-major; MAP / NEG: none.
 
+constexpr uint8_t kDenseTiles = 0xFF;               // BlkArgs::nfree of a pass over every tile
+constexpr int kMaxOuterBits = 36 - kBTileBits;      // QSIM_MAX_QUBITS - 12
+
 struct BlkArgs {
   uint64_t ntiles;
   uint64_t nbytes;        // program size: records past it are never read
@@ -56,7 +59,20 @@ struct BlkArgs {
   uint64_t prog_stride;   // bytes between the programs of a batch
   int ninstr;
   uint8_t tbits[kBTileBits];  // global bit of local tile bit i, ascending
+  // Tiles of the pass: every tile (nfree = kDenseTiles), or only those whose
+  // bits outside the tile are zero except on the nfree bits fbits (ascending):
+  // the input vanishes wherever another outer bit is 1, and so does the output.
+  uint8_t nfree;
+  uint8_t fbits[kMaxOuterBits];
 };
+
+// Global index of the first amplitude of tile number `tile`.
+__device__ __forceinline__ uint64_t tile_origin(const BlkArgs& a, uint64_t tile) {
+  if (a.nfree == kDenseTiles) return insert_zeros(tile, a.tbits, kBTileBits);
+  uint64_t b = 0;
+  for (int i = 0; i < int(a.nfree); ++i) b |= ((tile >> i) & 1) << a.fbits[i];
+  return b;
+}
 
 // A header decoded from its eight dwords. gfx950 has no scalar byte loads, so
 // the header is read as whole dwords (uniform address: scalar loads) and the

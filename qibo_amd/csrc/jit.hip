@@ -32,6 +32,7 @@ namespace qsim {
 
 int check_blocked_args(int nqubits, int dtype, const int* tile_bits, const void* program_dev, size_t program_bytes,
                        int ninstr, const char* what);
+void set_tiles(BlkArgs& a, int n, const int* tbits, uint64_t support);
 
 namespace {
 
@@ -338,7 +339,7 @@ std::string make_source(const std::vector<RecView>& recs, int dtype, int variant
   uint64_t gbit[kBTileBits];
 #pragma clang loop unroll(full)
   for (int i = 0; i < kBTileBits; ++i) gbit[i] = uint64_t(1) << a.tbits[i];
-  const uint64_t tile_base = insert_zeros(tile, a.tbits, kBTileBits);
+  const uint64_t tile_base = tile_origin(a, tile);
   uint64_t lane_idx = tile_base | deposit(local_index(0, threadIdx.x, 0), gbit);
   cplx<R> v[kBReg];
 )";
@@ -539,14 +540,14 @@ int qsim_blocked_specialize(const void* program, size_t program_bytes, int ninst
   const char* name = variant == kVariantAdjoint ? "qsim_adjoint_spec" : "qsim_blocked_spec";
   const hipError_t e = hipModuleGetFunction(&fn, mod, name);
   if (e != hipSuccess) {
-    hipModuleUnload(mod);
+    (void)hipModuleUnload(mod);
     set_error(std::string("qsim_blocked_specialize: hipModuleGetFunction: ") + hipGetErrorString(e));
     return QSIM_EHIP;
   }
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_index.find(key);
   if (it != g_index.end()) {  // another thread finished the same sequence first
-    hipModuleUnload(mod);
+    (void)hipModuleUnload(mod);
     *handle = it->second;
     return QSIM_OK;
   }
@@ -579,8 +580,8 @@ int qsim_blocked_specialized_lookup(const void* program, size_t program_bytes, i
 }
 
 int qsim_apply_blocked_specialized(uint64_t handle, void* states, int nbatch, int nqubits, int dtype,
-                                   const int* tile_bits, const void* programs_dev, size_t program_stride,
-                                   size_t program_bytes, int ninstr, void* stream) {
+                                   const int* tile_bits, uint64_t support, const void* programs_dev,
+                                   size_t program_stride, size_t program_bytes, int ninstr, void* stream) {
   clear_error();
   QSIM_CHECK_ARG(states != nullptr && nbatch >= 1 && nbatch <= 65535, "qsim_apply_blocked_specialized: bad batch");
   QSIM_CHECK_ARG(program_stride % 16 == 0 && (nbatch == 1 || program_stride >= program_bytes),
@@ -599,14 +600,21 @@ int qsim_apply_blocked_specialized(uint64_t handle, void* states, int nbatch, in
   if (spec_for(handle, dtype, sp.variant, program_bytes, ninstr, sp,
                "qsim_apply_blocked_specialized: kernel for another device, dtype or program") != QSIM_OK)
     return QSIM_EINVAL;
+  if (sp.variant == kVariantFromZero) {
+    // as qsim_apply_blocked_from_zero: |0...0> written, the pass run on tile 0
+    QSIM_CHECK_ARG(nbatch == 1, "qsim_apply_blocked_specialized: a from-zero pass takes one state");
+    const int zrc = qsim_zero_state(states, nqubits, dtype, stream);
+    if (zrc != QSIM_OK) return zrc;
+    support = 0;
+  }
   BlkArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.ntiles = uint64_t(1) << (nqubits - kBTileBits);
   a.nbytes = program_bytes;
   a.state_stride = uint64_t(1) << nqubits;
   a.prog_stride = program_stride;
   a.ninstr = ninstr;
   for (int i = 0; i < kBTileBits; ++i) a.tbits[i] = uint8_t(tile_bits[i]);
+  set_tiles(a, nqubits, tile_bits, support);
   const dim3 grid = launch_grid(a.ntiles);
   const uint8_t* prog = reinterpret_cast<const uint8_t*>(programs_dev);
   void* args[] = {&a, &prog, &states};

@@ -25,6 +25,8 @@ from tests.specialize_programs import programs, record_ops
 
 pytestmark = pytest.mark.gpu
 
+ALL = (1 << 64) - 1  # support of a pass over every tile
+
 
 def _upload(prog):
     return torch.from_numpy(np.frombuffer(prog + bytes(blocking.PAD_BYTES), dtype=np.uint8).copy()).cuda()
@@ -60,7 +62,7 @@ def _both(lib, n, code, tb, prog, count, state, from_zero):
     a, b = state.clone(), state.clone()
     interp = lib.qsim_apply_blocked_from_zero if from_zero else lib.qsim_apply_blocked
     _lib.check(interp(a.data_ptr(), n, code, tba, dev.data_ptr(), len(prog), count, None), lib)
-    _lib.check(lib.qsim_apply_blocked_specialized(handle.value, b.data_ptr(), 1, n, code, tba, dev.data_ptr(), 0,
+    _lib.check(lib.qsim_apply_blocked_specialized(handle.value, b.data_ptr(), 1, n, code, tba, ALL, dev.data_ptr(), 0,
                                                   len(prog), count, None), lib)
     torch.cuda.synchronize()
     return a, b
@@ -149,8 +151,8 @@ def test_specialised_launch_errors(backend):
         (h.value, psi, _lib.C128, len(prog), count - 1),
     ]
     for handle, st, code, nbytes, cnt in bad:
-        assert lib.qsim_apply_blocked_specialized(handle, st.data_ptr(), 1, n, code, tba, dev.data_ptr(), 0, nbytes,
-                                                  cnt, None) == _lib.EINVAL
+        assert lib.qsim_apply_blocked_specialized(handle, st.data_ptr(), 1, n, code, tba, ALL, dev.data_ptr(), 0,
+                                                  nbytes, cnt, None) == _lib.EINVAL
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
