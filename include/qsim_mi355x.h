@@ -161,8 +161,9 @@ int qsim_apply_blocked(void* state, int nqubits, int dtype, const int* tile_bits
  * contents are never read, only the result is written. The first pass of
  * Backend._execute_circuit from zero_state (abstract.py:2243-2275 then
  * 3321-3322). |0...0> is nonzero only in tile 0 and the pass acts on its
- * tile bits only, so the call writes zeros everywhere (qsim_zero_state) and
- * runs the pass on tile 0 alone. */
+ * tile bits only, so for 22 qubits and more the call writes zeros everywhere
+ * (qsim_zero_state) and runs the pass on tile 0 alone; below, it runs every
+ * tile from zeros generated in registers (one launch). */
 int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int* tile_bits, const void* program_dev,
                                  size_t program_bytes, int ninstr, void* stream);
 

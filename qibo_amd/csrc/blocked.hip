@@ -611,13 +611,21 @@ int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int*
     return QSIM_EINVAL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   // |0...0> has its only nonzero amplitude in tile 0 and the pass acts on the
-  // tile bits only: every other tile of the output is zero. Fill the state
-  // with |0...0> (zeros at streaming speed) and run the pass on tile 0 alone.
-  const int zrc = qsim_zero_state(state, nqubits, dtype, stream);
-  if (zrc != QSIM_OK) return zrc;
+  // tile bits only: every other tile of the output is zero. A large state is
+  // filled with |0...0> (zeros at streaming speed) and the pass runs on tile
+  // 0 alone; a small one (a launch costs more than its tiles) runs every tile
+  // from zeros generated in registers, in one launch.
+  uint64_t support = ~uint64_t(0);
+  if (nqubits >= kFillFromZeroQubits) {
+    const int zrc = qsim_zero_state(state, nqubits, dtype, stream);
+    if (zrc != QSIM_OK) return zrc;
+    support = 0;
+  }
   if (dtype == QSIM_C128)
-    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true, 0);
-  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true, 0);
+    return launch_blocked<double>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true,
+                                  support);
+  return launch_blocked<float>(state, nqubits, tile_bits, program_dev, program_bytes, ninstr, 1, 0, s, true,
+                               support);
 }
 
 int qsim_apply_blocked_support(void* state, int nqubits, int dtype, const int* tile_bits, uint64_t support,

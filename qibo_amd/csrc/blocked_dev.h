@@ -50,6 +50,9 @@ static_assert(sizeof(BlkHdr) == 32, "record header is 32 bytes");
 // DENSE_LDS: 2^k x 2^k row-major; MAP / NEG: none.
 
 constexpr uint8_t kDenseTiles = 0xFF;               // BlkArgs::nfree of a pass over every tile
+// From |0...0>, states of at least this many qubits are zero-filled and the
+// first pass runs tile 0 alone; smaller ones run every tile in one launch.
+constexpr int kFillFromZeroQubits = 22;
 constexpr int kMaxOuterBits = 36 - kBTileBits;      // QSIM_MAX_QUBITS - 12
 
 struct BlkArgs {

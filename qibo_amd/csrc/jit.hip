@@ -601,11 +601,15 @@ int qsim_apply_blocked_specialized(uint64_t handle, void* states, int nbatch, in
                "qsim_apply_blocked_specialized: kernel for another device, dtype or program") != QSIM_OK)
     return QSIM_EINVAL;
   if (sp.variant == kVariantFromZero) {
-    // as qsim_apply_blocked_from_zero: |0...0> written, the pass run on tile 0
+    // as qsim_apply_blocked_from_zero: a large state filled with |0...0> and
+    // the pass run on tile 0, a small one run on every tile
     QSIM_CHECK_ARG(nbatch == 1, "qsim_apply_blocked_specialized: a from-zero pass takes one state");
-    const int zrc = qsim_zero_state(states, nqubits, dtype, stream);
-    if (zrc != QSIM_OK) return zrc;
-    support = 0;
+    support = ~uint64_t(0);
+    if (nqubits >= kFillFromZeroQubits) {
+      const int zrc = qsim_zero_state(states, nqubits, dtype, stream);
+      if (zrc != QSIM_OK) return zrc;
+      support = 0;
+    }
   }
   BlkArgs a;
   std::memset(&a, 0, sizeof(a));
