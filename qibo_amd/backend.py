@@ -1359,7 +1359,9 @@ class MI355XBackend(GatePlanner):
             ev[0].record(cur)
         from_zero = zero is not None and zero[0]
         full = (1 << nqubits) - 1
-        tmask = sum(1 << b for b in tile_bits)
+        tmask = _TILE_MASKS.get(tile_bits)
+        if tmask is None:
+            tmask = _TILE_MASKS.setdefault(tuple(tile_bits), sum(1 << b for b in tile_bits))
         # amplitudes with a 1 outside the support are zero before the pass and
         # after it: only the tiles inside it run (from |0...0>: tile 0 alone)
         support = zero[1] if zero is not None and len(zero) > 1 and zero[1] is not None else full
@@ -1397,8 +1399,7 @@ class MI355XBackend(GatePlanner):
         if from_zero:
             moved = (1 << nqubits) * isz
         elif sparse:
-            nfree = bin(support & ~tmask & full).count("1")
-            moved = 2 * (1 << (nfree + len(tile_bits))) * isz
+            moved = 2 * (1 << (bin(support & ~tmask & full).count("1") + len(tile_bits))) * isz
         else:
             moved = 2 * (1 << nqubits) * isz
         self.block_bytes.append((moved, not (sparse or from_zero)))
@@ -1722,6 +1723,7 @@ def _dm_pair(op: _Op, nqubits: int) -> List[_Op]:
     return [op, col]
 _BLOCK_OP = _Op("block", (), (), None)
 _ALL_BITS = (1 << 64) - 1  # support mask of a pass over every tile
+_TILE_MASKS = {}  # tile bits -> their index mask
 
 
 def ctypes_stream(stream) -> int:
