@@ -167,6 +167,53 @@ __device__ __forceinline__ uint32_t local_index(int map, uint32_t lane, uint32_t
 // own part XOR a per-register constant.
 __device__ __forceinline__ uint32_t swz(uint32_t u) { return u ^ ((u >> 4) & 15u) ^ (((u >> 8) & 3u) << 4); }
 
+// LDS slots of 8-byte components: an additive layout, slot(u) = the sum of
+// kSlotW[i] over the set bits i of u (u + 1, 2, 4 for bits 5-7 and + 16, 40,
+// 63, 126 for bits 8-11). The weights grow faster than the sum of the smaller
+// ones, so the layout is a bijection onto a subset of [0, kSlotsF64), and
+// slot(a | b) = slot(a) + slot(b) for disjoint a, b: a lane's slots are its
+// own part plus a per-register constant, which the LDS instructions take as
+// their immediate offset (no address arithmetic per access). Banks: 64 of 4
+// bytes, two per 8-byte slot, so the slot residue mod 32 names the bank pair.
+// The residues of the weights mod 32 are 1, 2, 4, 8, 16 on bits 0-4, 1, 2, 4
+// on bits 5-7, 16 on bit 8 and 8 on bit 9: each half-wave (32 lanes) of
+// mapping 0 (lane bits on u bits 0-4) and mapping 1 (u bits 0-3 and 8) hits 32
+// distinct bank pairs; under mapping 2 (u bits 4-8) bits 4 and 8 share a
+// residue and pairs of lanes share a bank pair (a two-way conflict), while
+// each whole wave still spreads evenly over the 64 banks. No additive layout
+// does better on all three mappings: a half-wave is conflict-free only if its
+// five weights have residues of 2-adic valuation 0, 1, 2, 3, 4, so bits 4 and
+// 8 would need the same valuation (they complete bits 0-3 under mappings 0
+// and 1) and then collide under mapping 2.
+constexpr uint32_t kSlotW[kBTileBits] = {1, 2, 4, 8, 16, 33, 66, 132, 272, 552, 1087, 2174};
+constexpr uint32_t kSlotsF64 = 4348;  // 1 + the sum of the weights (34 784 bytes)
+
+__device__ __forceinline__ constexpr uint32_t slot_f64(uint32_t u) {
+  uint32_t s = 0;
+  for (int i = 0; i < kBTileBits; ++i)
+    if ((u >> i) & 1) s += kSlotW[i];
+  return s;
+}
+
+// Slots of one LDS buffer of 4096 components of type R: the additive layout
+// for 8-byte components in the kernels specialised to one record sequence
+// (which define QSIM_LDS_ADDITIVE to 1: their mappings are constants), the XOR
+// swizzle otherwise -- for 4-byte components, whose whole-wave accesses it
+// spreads over all 64 banks (no additive layout does), and in the interpreter,
+// whose run-time mappings would need a copy of every transpose per mapping
+// (more scalar registers than the interpreter has left).
+#ifndef QSIM_LDS_ADDITIVE
+#define QSIM_LDS_ADDITIVE 0
+#endif
+template <typename R>
+struct LdsSlots {
+  static constexpr bool kAdd = QSIM_LDS_ADDITIVE && sizeof(R) == 8;
+  static constexpr uint32_t kLen = kAdd ? kSlotsF64 : uint32_t(kBTile);
+  __device__ static __forceinline__ uint32_t of(uint32_t u) { return kAdd ? slot_f64(u) : swz(u); }
+  // Slot of a | b for disjoint a, b from their slots.
+  __device__ static __forceinline__ uint32_t join(uint32_t sa, uint32_t sb) { return kAdd ? sa + sb : sa ^ sb; }
+};
+
 // Local index of register 0 of `lane` under mapping `map` (register bits zero).
 __device__ __forceinline__ uint32_t lane_local(int map, uint32_t lane) { return local_index(map, lane, 0); }
 
@@ -180,37 +227,74 @@ __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
 }
 
 // The workgroup's LDS buffer holds one real component of the tile (4096
-// values: 32 KB in complex128, 16 KB in complex64), so a transpose moves the
-// real parts, then the imaginary parts. Half the LDS of a whole complex tile
-// lets twice as many workgroups share a CU.
-// Byte offset of register r's slot: the lane's part XOR a uniform constant,
-// formed by one VALU instruction right before each access (volatile, so the
-// 16 offsets are not kept live across the transpose: they would cost 16
-// registers each way).
+// values: 34 KB in complex128 with the additive layout's gaps, 16 KB in
+// complex64), so a transpose moves the real parts, then the imaginary parts.
+// Half the LDS of a whole complex tile lets twice as many workgroups share a
+// CU.
+// XOR swizzle: byte offset of register r's slot = the lane's part XOR a
+// uniform constant, formed by one VALU instruction right before each access
+// (volatile, so the 16 offsets are not kept live across the transpose: they
+// would cost 16 registers each way).
 __device__ __forceinline__ uint32_t slot_bytes(uint32_t lane_bytes, uint32_t reg_bytes) {
   uint32_t a;
   asm volatile("v_xor_b32 %0, %1, %2" : "=v"(a) : "s"(reg_bytes), "v"(lane_bytes));
   return a;
 }
 
+// Additive layout: register r's slot is the lane's base plus a constant after
+// unrolling, the immediate offset of the LDS instruction. The mapping must be
+// a constant for that, so a run-time mapping (the interpreter) branches to
+// one of three copies.
+template <int MAP, typename R>
+__device__ __forceinline__ void lds_put_add(R* lds, const R* x) {
+  R* base = lds + slot_f64(lane_local(MAP, threadIdx.x));
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) base[slot_f64(uint32_t(r) << reg_base(MAP))] = x[r];
+}
+
+template <int MAP, typename R>
+__device__ __forceinline__ void lds_get_add(const R* lds, R* x) {
+  const R* base = lds + slot_f64(lane_local(MAP, threadIdx.x));
+#pragma clang loop unroll(full)
+  for (int r = 0; r < kBReg; ++r) x[r] = base[slot_f64(uint32_t(r) << reg_base(MAP))];
+}
+
 template <typename R>
 __device__ __forceinline__ void lds_put(R* lds, const R* x, int map) {
-  const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
-  const int r0 = reg_base(map);
-  char* base = reinterpret_cast<char*>(lds);
+  if constexpr (LdsSlots<R>::kAdd) {
+    if (map == 0)
+      lds_put_add<0>(lds, x);
+    else if (map == 1)
+      lds_put_add<1>(lds, x);
+    else
+      lds_put_add<2>(lds, x);
+  } else {
+    const int r0 = reg_base(map);
+    const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
+    char* base = reinterpret_cast<char*>(lds);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r)
-    *reinterpret_cast<R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R)))) = x[r];
+    for (int r = 0; r < kBReg; ++r)
+      *reinterpret_cast<R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R)))) = x[r];
+  }
 }
 
 template <typename R>
 __device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
-  const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
-  const int r0 = reg_base(map);
-  const char* base = reinterpret_cast<const char*>(lds);
+  if constexpr (LdsSlots<R>::kAdd) {
+    if (map == 0)
+      lds_get_add<0>(lds, x);
+    else if (map == 1)
+      lds_get_add<1>(lds, x);
+    else
+      lds_get_add<2>(lds, x);
+  } else {
+    const int r0 = reg_base(map);
+    const uint32_t s = swz(lane_local(map, threadIdx.x)) * uint32_t(sizeof(R));
+    const char* base = reinterpret_cast<const char*>(lds);
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r)
-    x[r] = *reinterpret_cast<const R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R))));
+    for (int r = 0; r < kBReg; ++r)
+      x[r] = *reinterpret_cast<const R*>(base + slot_bytes(s, swz(uint32_t(r) << r0) * uint32_t(sizeof(R))));
+  }
 }
 
 // Change of register mapping: amplitude (lane, r) of mapping `from` moves to
@@ -554,10 +638,21 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
     base[i] = b;
     act[i] = ((tile_base | deposit(b, gbit)) & in.cmask) == in.cmask;
   }
+  using L = LdsSlots<R>;
   uint32_t slot[kBReg];  // the lane's amplitudes in the current mapping
-  const uint32_t s0 = swz(lane_local(map, threadIdx.x));
+  const uint32_t s0 = L::of(lane_local(map, threadIdx.x));
 #pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) slot[r] = s0 ^ swz(uint32_t(r) << reg_base(map));
+  for (int r = 0; r < kBReg; ++r) {  // per-register constants, selected by the uniform mapping
+    const uint32_t rs =
+        map == 0 ? L::of(uint32_t(r) << 8) : (map == 1 ? L::of(uint32_t(r) << 4) : L::of(uint32_t(r)));
+    slot[r] = L::join(s0, rs);
+  }
+  // group i's input c sits at base[i] | off[c] (disjoint bits)
+  uint32_t bslot[NG], oslot[D];
+#pragma clang loop unroll(full)
+  for (int i = 0; i < NG; ++i) bslot[i] = L::of(base[i]);
+#pragma clang loop unroll(full)
+  for (int c = 0; c < D; ++c) oslot[c] = L::of(off[c]);
   cplx<R> g[kBReg];  // g[i * D + c]: input c of group i
   // gather: real parts, then imaginary parts
 #pragma clang loop unroll(full)
@@ -566,7 +661,7 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #pragma clang loop unroll(full)
   for (int i = 0; i < NG; ++i)
 #pragma clang loop unroll(full)
-    for (int c = 0; c < D; ++c) g[i * D + c].re = lds[swz(base[i] | off[c])];
+    for (int c = 0; c < D; ++c) g[i * D + c].re = lds[L::join(bslot[i], oslot[c])];
   __syncthreads();
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) lds[slot[r]] = v[r].im;
@@ -574,7 +669,7 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #pragma clang loop unroll(full)
   for (int i = 0; i < NG; ++i)
 #pragma clang loop unroll(full)
-    for (int c = 0; c < D; ++c) g[i * D + c].im = lds[swz(base[i] | off[c])];
+    for (int c = 0; c < D; ++c) g[i * D + c].im = lds[L::join(bslot[i], oslot[c])];
   __syncthreads();
   // multiply (the arithmetic of the one-gate kernels: cmul, then cfma in column order)
 #pragma clang loop unroll(full)
@@ -595,7 +690,7 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #pragma clang loop unroll(full)
   for (int i = 0; i < NG; ++i)
 #pragma clang loop unroll(full)
-    for (int c = 0; c < D; ++c) lds[swz(base[i] | off[c])] = g[i * D + c].re;
+    for (int c = 0; c < D; ++c) lds[L::join(bslot[i], oslot[c])] = g[i * D + c].re;
   __syncthreads();
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) v[r].re = lds[slot[r]];
@@ -603,7 +698,7 @@ __device__ __forceinline__ void dense_lds(R* lds, cplx<R>* v, int map, const Hdr
 #pragma clang loop unroll(full)
   for (int i = 0; i < NG; ++i)
 #pragma clang loop unroll(full)
-    for (int c = 0; c < D; ++c) lds[swz(base[i] | off[c])] = g[i * D + c].im;
+    for (int c = 0; c < D; ++c) lds[L::join(bslot[i], oslot[c])] = g[i * D + c].im;
   __syncthreads();
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) v[r].im = lds[slot[r]];

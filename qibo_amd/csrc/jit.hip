@@ -197,13 +197,13 @@ std::string make_adjoint_source(const std::vector<RecView>& recs, int dtype) {
   const bool dbl = dtype == QSIM_C128;
   std::string s;
   s.reserve(2048 + recs.size() * 500);
-  s += "#include \"blocked_dev.h\"\nnamespace qsim {\n";
+  s += "#define QSIM_LDS_ADDITIVE 1\n#include \"blocked_dev.h\"\nnamespace qsim {\n";
   s += "extern \"C\" __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_ADJOINT_WAVES)))\n";
   s += dbl ? "void qsim_adjoint_spec(const AdjArgs a, const uint8_t* __restrict__ prog, cplx<double>* __restrict__ psi, "
              "cplx<double>* __restrict__ lam, double* __restrict__ part) {\n  using R = double;\n"
            : "void qsim_adjoint_spec(const AdjArgs a, const uint8_t* __restrict__ prog, cplx<float>* __restrict__ psi, "
              "cplx<float>* __restrict__ lam, double* __restrict__ part) {\n  using R = float;\n";
-  s += R"(  __shared__ R lds[kBTile];
+  s += R"(  __shared__ R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   const uint64_t bz = blockIdx.z;
@@ -325,13 +325,13 @@ std::string make_source(const std::vector<RecView>& recs, int dtype, int variant
   const bool dbl = dtype == QSIM_C128;
   std::string s;
   s.reserve(2048 + recs.size() * 400);
-  s += "#include \"blocked_dev.h\"\nnamespace qsim {\n";
+  s += "#define QSIM_LDS_ADDITIVE 1\n#include \"blocked_dev.h\"\nnamespace qsim {\n";
   s += "extern \"C\" __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_BLOCKED_WAVES)))\n";
   s += "void qsim_blocked_spec(const BlkArgs a, const uint8_t* __restrict__ prog, cplx<";
   s += dbl ? "double" : "float";
   s += ">* __restrict__ psi) {\n";
   s += dbl ? "  using R = double;\n" : "  using R = float;\n";
-  s += R"(  __shared__ R lds[kBTile];
+  s += R"(  __shared__ R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   psi += blockIdx.z * a.state_stride;
