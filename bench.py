@@ -684,13 +684,14 @@ def main():
 
     ms_per_step = 1000.0 * elapsed / args.steps
     # HBM bytes the step's kernels move: each blocked pass reads and writes the
-    # state once (the first, from |0...0>, only writes it); the probabilities
+    # state once (the first ones, from |0...0>, read only the support built so
+    # far and write the tiles that contain it); the probabilities
     # read the state and write 8 B per amplitude; the sampling maps read the
     # probabilities again and write 1 B per amplitude (16 B per 16 entries)
     npass = len(passes) / args.steps if passes else 0.0
-    # the passes' bytes as launched: the first (from |0...0>) writes the state
-    # once (its zero fill) and the next ones run only the tiles of the support
-    # built so far (the tiles outside it are zero); see backend.block_bytes
+    # the passes' bytes as launched: the first (from |0...0>) runs tile 0 and
+    # the next ones read only the support built so far and run only the tiles
+    # that contain it (the state outside is zero); see backend.block_bytes
     pass_bytes = sum(b for b, _ in launch_bytes) / args.steps if launch_bytes else 0.0
     step_bytes = pass_bytes + (2**n) * (itemsize + 8) + (2**n) * 9
     # HBM bytes per k_blocked launch, from the committed rocprofv3 counter runs of

@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define QSIM_ABI_VERSION 10
+#define QSIM_ABI_VERSION 11
 
 /* state dtypes */
 #define QSIM_C128 0 /* complex128: double re, double im */
@@ -80,6 +80,12 @@ const char* qsim_last_error(void);
 /* |0...0>: zero the 2^n amplitudes and set amplitude 0 to 1.
  * Replaces Backend.zero_state (abstract.py:2243-2275). */
 int qsim_zero_state(void* state, int nqubits, int dtype, void* stream);
+
+/* Zeros at the amplitudes whose index has a 1 outside `support` (bit b of the
+ * mask = index bit b); the others are neither read nor written. Completes a
+ * state that passes from |0...0> wrote only on their support (see
+ * qsim_apply_blocked_support). No counterpart in the reference. */
+int qsim_zero_outside_support(void* state, int nqubits, int dtype, uint64_t support, void* stream);
 
 /* Dense 2^k x 2^k matrix on `targets` (k = ntargets, 1..3), applied only on
  * the amplitudes whose `controls` bits are all 1, in place.
@@ -170,10 +176,14 @@ int qsim_apply_blocked_from_zero(void* state, int nqubits, int dtype, const int*
 /* qsim_apply_blocked for an input known to vanish wherever an index bit
  * outside `support` is 1 (bit b of the mask = index bit b, i.e. qubit
  * n-1-b): the pass runs only the tiles whose bits outside the tile bits lie
- * in `support`, and leaves the others, which stay zero, untouched. A state
- * built by passes from |0...0> has the union of their tile bits as its
- * support (no counterpart in the reference: it skips amplitudes that are
- * zero; all ones is qsim_apply_blocked). */
+ * in `support`, and leaves the others, which stay zero, untouched; inside
+ * those tiles it reads only the amplitudes in the support (the others are
+ * taken as zero, whatever their memory holds) and writes every amplitude.
+ * A state built by passes from |0...0> has the union of their tile bits as
+ * its support, and need be written only there: one amplitude set to 1, then
+ * passes with the growing support, then qsim_zero_outside_support unless a
+ * pass has covered every tile (no counterpart in the reference: it skips
+ * amplitudes that are zero; all ones is qsim_apply_blocked). */
 int qsim_apply_blocked_support(void* state, int nqubits, int dtype, const int* tile_bits, uint64_t support,
                                const void* program_dev, size_t program_bytes, int ninstr, void* stream);
 

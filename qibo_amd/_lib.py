@@ -11,7 +11,7 @@ import ctypes
 import os
 import threading
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 # QIBO_AMD_LIB names another build of the library (an A/B comparison of two
 # builds on one device); by default the one next to this file
 LIB_PATH = os.environ.get("QIBO_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -33,6 +33,7 @@ SIGNATURES = {
     "qsim_version": (_i, []),
     "qsim_last_error": (ctypes.c_char_p, []),
     "qsim_zero_state": (_i, [_vp, _i, _i, _vp]),
+    "qsim_zero_outside_support": (_i, [_vp, _i, _i, ctypes.c_uint64, _vp]),
     "qsim_collapse": (_i, [_vp, _i, _i, _c_int_p, _i, _i64, ctypes.c_double, _vp]),
     "qsim_prob_stats_workspace": (_sz, [_i64]),
     "qsim_prob_stats": (_i, [_vp, _i, _i64, _vp, _vp, _sz, _vp]),

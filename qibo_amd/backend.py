@@ -976,11 +976,13 @@ class MI355XBackend(GatePlanner):
         stand for |0...0>: a first blocked pass produces it itself, anything
         else fills it first.
         """
-        # [|0...0> still to be materialised, support]: the support is the mask
-        # of index bits that may be 1 in a nonzero amplitude (None: any); the
-        # passes from |0...0> grow it by their tile bits and skip the tiles
-        # outside it (all zero)
-        zero = [zero_pending, 0 if zero_pending else None]
+        # [|0...0> still to be materialised, support, unwritten]: the support
+        # is the mask of index bits that may be 1 in a nonzero amplitude (None:
+        # any); the passes from |0...0> grow it by their tile bits, skip the
+        # tiles outside it (all zero) and read only the amplitudes inside it.
+        # unwritten: the amplitudes outside the support are not in memory yet
+        # (written as zeros before anything else reads the state)
+        zero = [zero_pending, 0 if zero_pending else None, False]
         if zero_pending and (not self.blocking or nqubits < blocking.TILE_BITS):
             self._fill_zero(state, nqubits)
             zero[0] = False
@@ -1018,9 +1020,7 @@ class MI355XBackend(GatePlanner):
 
         while True:
             state = self._run_blocked(segment(), state, nqubits, zero, record)
-            if zero[0]:  # no blocked pass produced it
-                self._fill_zero(state, nqubits)
-                zero[0] = False
+            self._complete(state, nqubits, zero)  # (no blocked pass produced |0...0>, or left it unwritten)
             if not stop:
                 if record is not None:
                     self._cache_put(key, record)
@@ -1133,18 +1133,14 @@ class MI355XBackend(GatePlanner):
         cur = torch.cuda.current_stream(state.device)
         for act in actions:
             if act[0] == "single":
-                if zero[0]:
-                    self._fill_zero(state, nqubits)
-                    zero[0] = False
+                self._complete(state, nqubits, zero)
                 self.apply_op(act[1], state, nqubits)
                 if len(zero) > 1:
                     zero[1] = None
                 continue
             _, tile_bits, prog, nbytes, count, ngates, host = act
             self._launch_blocked(state, nqubits, code, tile_bits, prog, nbytes, count, ngates, stream, cur, zero, host)
-        if zero[0]:
-            self._fill_zero(state, nqubits)
-            zero[0] = False
+        self._complete(state, nqubits, zero)
         return state
 
     def _run_small(self, queue, state: torch.Tensor, nqubits: int) -> torch.Tensor:
@@ -1268,6 +1264,18 @@ class MI355XBackend(GatePlanner):
         _lib.check(self.lib.qsim_zero_state(state.data_ptr(), nqubits, self._dtype_code(state),
                                             self._stream(state)), self.lib)
 
+    def _complete(self, state: torch.Tensor, nqubits: int, zero):
+        """Write every amplitude of a state that passes from |0...0> have left
+        partly unwritten (``zero`` as in run_queue), before an operation that
+        may read any of them."""
+        if zero[0]:
+            self._fill_zero(state, nqubits)
+            zero[0] = False
+        elif len(zero) > 2 and zero[2]:
+            _lib.check(self.lib.qsim_zero_outside_support(state.data_ptr(), nqubits, self._dtype_code(state),
+                                                          zero[1], self._stream(state)), self.lib)
+            zero[2] = False
+
     def _run_blocked(self, ops, state: torch.Tensor, nqubits: int, zero=None, record=None) -> torch.Tensor:
         """Partition ``ops`` (an iterable of planned gates) into passes and
         launch each pass as soon as it is compiled, so the host plans and
@@ -1283,9 +1291,8 @@ class MI355XBackend(GatePlanner):
         cur = torch.cuda.current_stream(state.device)
         for kind, payload in blocking.partition(ops, nqubits, low, ordered):
             if kind == "single":
-                if zero is not None and zero[0]:
-                    self._fill_zero(state, nqubits)
-                    zero[0] = False
+                if zero is not None:
+                    self._complete(state, nqubits, zero)
                 self.apply_op(payload, state, nqubits)
                 if zero is not None and len(zero) > 1:
                     zero[1] = None
@@ -1358,6 +1365,14 @@ class MI355XBackend(GatePlanner):
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record(cur)
         from_zero = zero is not None and zero[0]
+        if from_zero and len(zero) > 2 and nqubits >= _FILL_FROM_ZERO_QUBITS and self.skip_zero_tiles:
+            # |0...0> of a large state: write its one nonzero amplitude and run
+            # the pass on tile 0 reading it alone; the other amplitudes stay
+            # unwritten while the passes that follow keep to the support
+            state[:1].fill_(1)
+            zero[0] = from_zero = False
+            zero[1] = 0
+            zero[2] = True
         full = (1 << nqubits) - 1
         tmask = _TILE_MASKS.get(tile_bits)
         if tmask is None:
@@ -1369,19 +1384,22 @@ class MI355XBackend(GatePlanner):
             support = full
         if from_zero:
             support = 0  # (the from-zero entry points fill the state and run tile 0 alone)
-        sparse = not from_zero and (support | tmask) != full
+        # sparse: tiles outside the support are skipped; partial: the input
+        # is read only inside the support (the rest may be unwritten)
+        partial = not from_zero and support != full
+        sparse = partial and (support | tmask) != full
         variant = _lib.SPEC_PASS_FROM_ZERO if from_zero else _lib.SPEC_PASS
         handle = self._specialized(host, count, code, variant, state.device)
         tba = _lib.int_array(list(tile_bits))
         if handle:
             rc = self.lib.qsim_apply_blocked_specialized(handle, state.data_ptr(), 1, nqubits, code, tba,
-                                                         support if sparse else _ALL_BITS, prog.data_ptr(), 0,
+                                                         support if partial else _ALL_BITS, prog.data_ptr(), 0,
                                                          nbytes, count, stream)
             self.specialized_launches += 1
         elif from_zero:
             rc = self.lib.qsim_apply_blocked_from_zero(state.data_ptr(), nqubits, code, tba, prog.data_ptr(), nbytes,
                                                        count, stream)
-        elif sparse:
+        elif partial:
             rc = self.lib.qsim_apply_blocked_support(state.data_ptr(), nqubits, code, tba, support, prog.data_ptr(),
                                                      nbytes, count, stream)
         else:
@@ -1391,18 +1409,22 @@ class MI355XBackend(GatePlanner):
             zero[0] = False
         if zero is not None and len(zero) > 1:
             zero[1] = (support | tmask) if (support | tmask) != full else None
+            if zero[1] is None and len(zero) > 2:
+                zero[2] = False  # every tile ran: the whole state is written
         if sparse:
             self.sparse_launches += 1
         # algorithmic HBM bytes of the launch: the state read and written once
-        # (from |0...0>: written once, by the fill), or the tiles in the support
+        # (from |0...0>: written once, by the fill), or the support read and
+        # the tiles that contain it written
         isz = 16 if code == _lib.C128 else 8
         if from_zero:
             moved = (1 << nqubits) * isz
-        elif sparse:
-            moved = 2 * (1 << (bin(support & ~tmask & full).count("1") + len(tile_bits))) * isz
+        elif partial:
+            moved = ((1 << bin(support & full).count("1"))
+                     + (1 << (bin(support & ~tmask & full).count("1") + len(tile_bits)))) * isz
         else:
             moved = 2 * (1 << nqubits) * isz
-        self.block_bytes.append((moved, not (sparse or from_zero)))
+        self.block_bytes.append((moved, not (partial or from_zero)))
         _lib.check(rc, self.lib)
         if timer is not None and timer[0](_BLOCK_OP):
             ev[1].record(cur)
@@ -1723,6 +1745,7 @@ def _dm_pair(op: _Op, nqubits: int) -> List[_Op]:
     return [op, col]
 _BLOCK_OP = _Op("block", (), (), None)
 _ALL_BITS = (1 << 64) - 1  # support mask of a pass over every tile
+_FILL_FROM_ZERO_QUBITS = 22  # from this size on the first pass from |0...0> runs tile 0 alone
 _TILE_MASKS = {}  # tile bits -> their index mask
 
 

@@ -79,8 +79,7 @@ void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
       v[r].im = R(0);
     }
   } else {
-#pragma clang loop unroll(full)
-    for (int r = 0; r < kBReg; ++r) v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
+    load_lane(a, psi, lane_idx, gbit, v);
   }
 
   cplx<R> P;  // pending per-lane factor of LANE_PHASE records
@@ -268,6 +267,9 @@ void set_tiles(BlkArgs& a, int n, const int* tbits, uint64_t support) {
   const uint64_t all = (uint64_t(1) << n) - 1;
   const uint64_t outer = all & ~tmask;
   const uint64_t freeb = outer & support;
+  a.zload = 0;
+  for (int i = 0; i < kBTileBits; ++i)
+    if (!((support >> tbits[i]) & 1)) a.zload |= 1u << i;
   if (freeb == outer) {
     a.nfree = kDenseTiles;
     a.ntiles = uint64_t(1) << (n - kBTileBits);

@@ -67,6 +67,10 @@ struct BlkArgs {
   // the input vanishes wherever another outer bit is 1, and so does the output.
   uint8_t nfree;
   uint8_t fbits[kMaxOuterBits];
+  // Local bits outside the input's support: an amplitude whose local index
+  // has one of them set is zero and is not read (its memory may hold
+  // anything); 0 reads the whole tile.
+  uint32_t zload;
 };
 
 // Global index of the first amplitude of tile number `tile`.
@@ -224,6 +228,26 @@ __device__ __forceinline__ uint64_t deposit(uint32_t u, const uint64_t* gbit) {
   for (int i = 0; i < kBTileBits; ++i)
     if ((u >> i) & 1) g |= gbit[i];
   return g;
+}
+
+// The lane's amplitudes under mapping 0 (lane_idx: their global index with
+// the register bits cleared), reading only those inside the support.
+template <typename R>
+__device__ __forceinline__ void load_lane(const BlkArgs& a, const cplx<R>* psi, uint64_t lane_idx,
+                                          const uint64_t* gbit, cplx<R>* v) {
+  if (a.zload == 0) {
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
+  } else {
+    const uint32_t lz = local_index(0, threadIdx.x, 0) & a.zload;
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kBReg; ++r) {
+      if ((lz | ((uint32_t(r) << 8) & a.zload)) == 0)
+        v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
+      else
+        v[r] = cplx<R>{R(0), R(0)};
+    }
+  }
 }
 
 // The workgroup's LDS buffer holds one real component of the tile (4096

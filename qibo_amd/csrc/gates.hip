@@ -88,6 +88,14 @@ __global__ __launch_bounds__(256) void k_zero_state(cplx<R>* __restrict__ psi, u
   }
 }
 
+// Zeros at the amplitudes whose index has a bit of `outside` set; the others
+// are neither read nor written.
+template <typename R>
+__global__ __launch_bounds__(256) void k_zero_outside(cplx<R>* __restrict__ psi, uint64_t n, uint64_t outside) {
+  const uint64_t i = xcd_block() * 256 + threadIdx.x;
+  if (i < n && (i & outside) != 0) st_nt(psi, i, cplx<R>{R(0), R(0)});
+}
+
 // Collapse onto a measurement outcome: amplitudes whose measured bits equal
 // `value` are divided by `norm`, the others set to zero (one pass).
 template <typename R>
@@ -344,6 +352,24 @@ int qsim_zero_state(void* state, int nqubits, int dtype, void* stream) {
     hipLaunchKernelGGL(k_zero_state<double>, blocks, dim3(256), 0, s, (c128*)state, n);
   else
     hipLaunchKernelGGL(k_zero_state<float>, blocks, dim3(256), 0, s, (c64*)state, n);
+  QSIM_CHECK_LAUNCH();
+  return QSIM_OK;
+}
+
+int qsim_zero_outside_support(void* state, int nqubits, int dtype, uint64_t support, void* stream) {
+  clear_error();
+  QSIM_CHECK_ARG(state != nullptr, "qsim_zero_outside_support: null state");
+  QSIM_CHECK_ARG(nqubits >= 1 && nqubits <= QSIM_MAX_QUBITS, "qsim_zero_outside_support: nqubits out of range");
+  QSIM_CHECK_ARG(dtype == QSIM_C128 || dtype == QSIM_C64, "qsim_zero_outside_support: bad dtype");
+  const uint64_t n = uint64_t(1) << nqubits;
+  const uint64_t outside = (n - 1) & ~support;
+  if (outside == 0) return QSIM_OK;
+  const dim3 blocks = launch_grid(ceil_div_u(n, 256));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == QSIM_C128)
+    hipLaunchKernelGGL(k_zero_outside<double>, blocks, dim3(256), 0, s, (c128*)state, n, outside);
+  else
+    hipLaunchKernelGGL(k_zero_outside<float>, blocks, dim3(256), 0, s, (c64*)state, n, outside);
   QSIM_CHECK_LAUNCH();
   return QSIM_OK;
 }

@@ -351,9 +351,7 @@ std::string make_source(const std::vector<RecView>& recs, int dtype, int variant
   }
 )";
   else
-    s += R"(#pragma clang loop unroll(full)
-  for (int r = 0; r < kBReg; ++r) v[r] = ld_nt(psi, lane_idx | deposit(uint32_t(r) << 8, gbit));
-)";
+    s += "  load_lane(a, psi, lane_idx, gbit, v);\n";
   s += R"(  cplx<R> P;
   P.re = R(1);
   P.im = R(0);

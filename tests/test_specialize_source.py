@@ -39,7 +39,7 @@ def test_source_lists_every_record(lib, cdtype, code):
         assert "switch" not in src
         assert ("cplx<double>" in src) == (code == _lib.C128)
         zsrc = _source(lib, prog, count, code, 1)
-        assert "ld_nt(psi" not in zsrc and "ld_nt(psi" in src
+        assert "load_lane(a, psi" not in zsrc and "load_lane(a, psi" in src
 
 
 @pytest.mark.parametrize("cdtype,code,ordered", [(np.complex128, _lib.C128, False), (np.complex64, _lib.C64, False),
