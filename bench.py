@@ -748,10 +748,10 @@ def main():
         "hbm_GBs_step": {
             "bytes_per_step": step_bytes,
             "GBs": step_bytes / (elapsed / args.steps) / 1e9,
-            "counted": "per blocked pass one read and one write of the tiles it runs (the first, "
-                       "from |0...0>, writes the state once; a pass on part of the tiles moves only "
-                       "those), probabilities (16 B read + 8 B written per amplitude), sampling maps "
-                       "(8 B read + 1 B written per amplitude)",
+            "counted": "per blocked pass one read and one write of the tiles it runs (the first "
+                       "ones, from |0...0>, read only the support built so far and write the tiles "
+                       "that contain it), probabilities (16 B read + 8 B written per amplitude), "
+                       "sampling maps (8 B read + 1 B written per amplitude)",
         },
         # the backend keeps the compiled passes of a repeated identical gate
         # queue (same gates, qubits and parameter values); every timed step
