@@ -97,10 +97,10 @@ def _poison(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("n", [16, 24])
-def test_circuits_from_zero_skip_zero_tiles(dtype, n):
+@pytest.mark.parametrize("n,spec", [(16, False), (24, False), (24, True)])
+def test_circuits_from_zero_skip_zero_tiles(dtype, n, spec):
     skip = MI355XBackend(dtype=dtype)
-    skip.specialize = False
+    skip.specialize, skip.specialize_after = spec, 1
     full = MI355XBackend(dtype=dtype)
     full.specialize, full.skip_zero_tiles = False, False
     c = models.random_layered_circuit(n, 8, seed=21, measure=False)
@@ -114,6 +114,12 @@ def test_circuits_from_zero_skip_zero_tiles(dtype, n):
     b = full.execute_circuit(models.random_layered_circuit(n, 8, seed=21, measure=False)).state()
     assert full.sparse_launches == 0
     assert torch.equal(a, b)
+    if spec:  # the same circuit again through the specialised passes
+        skip.wait_specializations()
+        _poison(n, dtype)
+        a = skip.execute_circuit(models.random_layered_circuit(n, 8, seed=21, measure=False)).state()
+        assert skip.specialized_launches > 0
+        assert torch.equal(a, b)
     if n <= 16:
         want = OracleBackend().execute(models.random_layered_circuit(n, 8, seed=21, measure=False))
         tol = 1e-12 if dtype == "complex128" else 5e-6
@@ -152,19 +158,23 @@ def _partial_circuit(n, rng, single):
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 @pytest.mark.parametrize("single", [False, True])
-def test_partial_support_completed(dtype, single):
+@pytest.mark.parametrize("spec", [False, True])
+def test_partial_support_completed(dtype, single, spec):
     n = 24
     skip = MI355XBackend(dtype=dtype)
-    skip.specialize = False
+    skip.specialize, skip.specialize_after = spec, 1
     full = MI355XBackend(dtype=dtype)
     full.specialize, full.skip_zero_tiles = False, False
-    for rep in range(2):  # the second run replays the compiled passes
+    for rep in range(2):  # the second run replays the compiled passes (specialised with spec)
         _poison(n, dtype)
         a = skip.execute_circuit(_partial_circuit(n, np.random.default_rng(5), single)).state()
         b = full.execute_circuit(_partial_circuit(n, np.random.default_rng(5), single)).state()
         assert torch.equal(a, b), rep
         assert not torch.isnan(a).any()
+        if spec:
+            skip.wait_specializations()
     assert skip.sparse_launches > 0
+    assert (skip.specialized_launches > 0) == spec
     small = OracleBackend().execute(_partial_circuit(14, np.random.default_rng(5), single))
     # qubit q of the 24 is index bit 23 - q: the 14-qubit state sits on the
     # indices whose 10 low bits are zero
