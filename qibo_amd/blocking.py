@@ -312,9 +312,18 @@ _HDR_STRUCT = struct.Struct("<BBBBIQ8BQ")  # the same layout
 assert _HDR_STRUCT.size == HEADER_BYTES
 
 
-def rotation_form(m: np.ndarray):
-    """(mode, t, s, negated) when the 2x2 ``m`` is exactly one of the ROT
-    forms, else None. negated: m = -(the form with c >= 0 that is encoded)."""
+# |c^2 + s^2 - 1| up to which a 2x2 of the ROT forms runs as a rotation: the
+# shears reproduce only an exact rotation, so complex128 matrices must be one
+# to 1e-14; complex64 matrices hold float32 roundings of cos and sin (|c^2 +
+# s^2 - 1| up to ~1.2e-7), and their rotation by atan2(s, c) differs from the
+# rounded matrix by less than float32's own rounding
+ROT_TOL = {np.dtype(np.complex128): 1e-14, np.dtype(np.complex64): 4e-7}
+
+
+def rotation_form(m: np.ndarray, tol: float = 1e-14):
+    """(mode, t, s, negated) when the 2x2 ``m`` is one of the ROT forms with
+    |c^2 + s^2 - 1| <= tol, else None. negated: m = -(the form with c >= 0
+    that is encoded)."""
     a, b, cc, d = np.asarray(m, dtype=np.complex128).ravel().tolist()  # Python complex: the same values
     real = a.imag == 0 and b.imag == 0 and cc.imag == 0 and d.imag == 0
     if real and a.real == d.real and b.real == -cc.real:
@@ -326,7 +335,7 @@ def rotation_form(m: np.ndarray):
         mode, c, s = ROT_RX, a.real, -b.imag
     else:
         return None
-    if abs(c * c + s * s - 1.0) > 1e-14:  # the shears reproduce only an exact rotation
+    if abs(c * c + s * s - 1.0) > tol:
         return None
     neg = bool(c < 0)
     if neg:
@@ -670,7 +679,7 @@ def compile_pass(tile_bits: Sequence[int], ops: Sequence, nqubits: int, cdtype,
             cm, rreq = split_cond(op, cur)
             m = np.asarray(op.data)
             j = _REG_GROUPS[cur].index(lb)
-            rot = None if (ordered or cm or rreq) else rotation_form(m)
+            rot = None if (ordered or cm or rreq) else rotation_form(m, ROT_TOL[np.dtype(cdtype)])
             if rot is not None:
                 mode, t, sn, neg = rot
                 _pack_rot(buf, j | (mode << 2), t, sn, cdtype)

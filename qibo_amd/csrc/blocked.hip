@@ -57,7 +57,7 @@ template <typename R, bool FROM_ZERO>
 __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_BLOCKED_WAVES)))
 void k_blocked(const BlkArgs a, const uint8_t* __restrict__ prog,
                                                  cplx<R>* __restrict__ psi) {
-  __shared__ R lds[LdsSlots<R>::kLen];
+  __shared__ __align__(16) R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   psi += blockIdx.z * a.state_stride;  // state and program of this batch entry
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(QSIM_ADJOIN
 void k_blocked_adjoint(const AdjArgs a, const uint8_t* __restrict__ prog,
                                                          cplx<R>* __restrict__ psi, cplx<R>* __restrict__ lam,
                                                          double* __restrict__ part) {
-  __shared__ R lds[LdsSlots<R>::kLen];
+  __shared__ __align__(16) R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   const uint64_t bz = blockIdx.z;

@@ -212,7 +212,11 @@ __device__ __forceinline__ constexpr uint32_t slot_f64(uint32_t u) {
 template <typename R>
 struct LdsSlots {
   static constexpr bool kAdd = QSIM_LDS_ADDITIVE && sizeof(R) == 8;
-  static constexpr uint32_t kLen = kAdd ? kSlotsF64 : uint32_t(kBTile);
+  // complex64 transposes of the specialised kernels move whole amplitudes
+  // (8 bytes) in the additive layout: one exchange instead of one per
+  // component; the other LDS uses keep 4-byte swizzled slots
+  static constexpr bool kPair = QSIM_LDS_ADDITIVE && sizeof(R) == 4;
+  static constexpr uint32_t kLen = kAdd ? kSlotsF64 : (kPair ? 2 * kSlotsF64 : uint32_t(kBTile));
   __device__ static __forceinline__ uint32_t of(uint32_t u) { return kAdd ? slot_f64(u) : swz(u); }
   // Slot of a | b for disjoint a, b from their slots.
   __device__ static __forceinline__ uint32_t join(uint32_t sa, uint32_t sb) { return kAdd ? sa + sb : sa ^ sb; }
@@ -322,9 +326,18 @@ __device__ __forceinline__ void lds_get(const R* lds, R* x, int map) {
 }
 
 // Change of register mapping: amplitude (lane, r) of mapping `from` moves to
-// its place under mapping `to`, through LDS, one component at a time.
+// its place under mapping `to`, through LDS, one component at a time (whole
+// amplitudes for complex64 in the additive layout).
 template <typename R>
 __device__ __forceinline__ void remap(R* lds, cplx<R>* v, int from, int to) {
+  if constexpr (LdsSlots<R>::kPair) {
+    cplx<R>* l = reinterpret_cast<cplx<R>*>(lds);  // 8-byte aligned (the buffer is 16-byte aligned)
+    lds_put(l, v, from);
+    __syncthreads();
+    lds_get(l, v, to);
+    __syncthreads();
+    return;
+  }
   R x[kBReg];
 #pragma clang loop unroll(full)
   for (int r = 0; r < kBReg; ++r) x[r] = v[r].re;

@@ -175,8 +175,11 @@ struct Writer {
   }
 };
 
-// (mode, t, s, negated) of a 2x2 that is exactly one of the ROT forms.
-bool rotation_form(const C* m, int* mode, double* t, double* s, bool* neg) {
+// (mode, t, s, negated) of a 2x2 of the ROT forms with |c^2 + s^2 - 1| <= tol
+// (1e-14 for complex128 programs; 4e-7 for complex64 ones, whose entries are
+// float32 roundings of cos and sin: the rotation by atan2(s, c) differs from
+// the rounded matrix by less than float32's own rounding).
+bool rotation_form(const C* m, double tol, int* mode, double* t, double* s, bool* neg) {
   const C a = m[0], b = m[1], cc = m[2], d = m[3];
   const bool real = a.im == 0 && b.im == 0 && cc.im == 0 && d.im == 0;
   double c, sn;
@@ -189,7 +192,7 @@ bool rotation_form(const C* m, int* mode, double* t, double* s, bool* neg) {
   } else {
     return false;
   }
-  if (std::fabs(c * c + sn * sn - 1.0) > 1e-14) return false;
+  if (std::fabs(c * c + sn * sn - 1.0) > tol) return false;
   *neg = c < 0;
   if (*neg) c = -c, sn = -sn;
   *t = sn / (1.0 + c);
@@ -460,7 +463,7 @@ int compile(int n, const int* tile_bits, std::vector<Op>& items, bool ordered, W
       int mode;
       double t, sn;
       bool neg;
-      if (!ordered && cm == 0 && rreq == 0 && rotation_form(o.data, &mode, &t, &sn, &neg)) {
+      if (!ordered && cm == 0 && rreq == 0 && rotation_form(o.data, w.itemsize == 16 ? 1e-14 : 4e-7, &mode, &t, &sn, &neg)) {
         w.emit_rot(j | (mode << 2), t, sn);
         sign ^= neg;
         continue;

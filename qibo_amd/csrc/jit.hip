@@ -203,7 +203,7 @@ std::string make_adjoint_source(const std::vector<RecView>& recs, int dtype) {
              "cplx<double>* __restrict__ lam, double* __restrict__ part) {\n  using R = double;\n"
            : "void qsim_adjoint_spec(const AdjArgs a, const uint8_t* __restrict__ prog, cplx<float>* __restrict__ psi, "
              "cplx<float>* __restrict__ lam, double* __restrict__ part) {\n  using R = float;\n";
-  s += R"(  __shared__ R lds[LdsSlots<R>::kLen];
+  s += R"(  __shared__ __align__(16) R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   const uint64_t bz = blockIdx.z;
@@ -331,7 +331,7 @@ std::string make_source(const std::vector<RecView>& recs, int dtype, int variant
   s += dbl ? "double" : "float";
   s += ">* __restrict__ psi) {\n";
   s += dbl ? "  using R = double;\n" : "  using R = float;\n";
-  s += R"(  __shared__ R lds[LdsSlots<R>::kLen];
+  s += R"(  __shared__ __align__(16) R lds[LdsSlots<R>::kLen];
   const uint64_t tile = xcd_block();
   if (tile >= a.ntiles) return;
   psi += blockIdx.z * a.state_stride;
