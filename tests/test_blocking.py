@@ -430,3 +430,20 @@ def test_small_fused_groups_run_as_their_gates(rng):
     big = models.QFT(12).fuse(max_qubits=5)
     five = [g for g in big.queue if len(g.qubits) == 5]
     assert five and all(len(gp.plan_ops(g)) == 1 for g in five)
+
+
+@pytest.mark.parametrize("theta", [0.3, 2.0, 3.1, -1.7, np.pi])
+def test_rotation_forms_complex64(theta):
+    """complex64 matrices hold float32 roundings of cos and sin: a ROT form at
+    float32 tolerance, not at complex128's; the shears' rotation matches the
+    rounded matrix to float32 rounding."""
+    m = GatePlanner("complex64").matrix(gates.RY(0, theta))
+    assert m.dtype == np.complex64
+    tol = blocking.ROT_TOL[np.dtype(np.complex64)]
+    form = blocking.rotation_form(m, tol)
+    assert form is not None and form[0] == blocking.ROT_RY
+    _, t, s, neg = form
+    rot = np.array([[1 - t * s, -t * (2 - t * s)], [t * (2 - t * s), 1 - t * s]]) * (-1 if neg else 1)
+    assert np.max(np.abs(rot - m.real)) < 2e-7
+    # a matrix further from a rotation than float32 rounding is not one
+    assert blocking.rotation_form(m * np.complex64(1 + 1e-5), tol) is None
